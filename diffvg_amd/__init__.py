@@ -1,0 +1,11 @@
+"""diffvg_amd -- MI355X-native implementation of diffvg's render path.
+
+Import it where pydiffvg was imported (``import diffvg_amd as pydiffvg``): the data
+classes, ``RenderFunction`` and the device helpers keep pydiffvg's names and
+signatures.  Rendering runs hand-written HIP kernels for gfx950 through the C-ABI
+library ``libdiffvg_amd.so``; there is no CPU fallback.
+"""
+from .device import *  # noqa: F401,F403
+from .shape import *  # noqa: F401,F403
+from .color import *  # noqa: F401,F403
+from .pixel_filter import *  # noqa: F401,F403
