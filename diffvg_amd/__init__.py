@@ -9,3 +9,5 @@ from .device import *  # noqa: F401,F403
 from .shape import *  # noqa: F401,F403
 from .color import *  # noqa: F401,F403
 from .pixel_filter import *  # noqa: F401,F403
+from . import diffvg  # noqa: F401
+from .render_pytorch import RenderFunction, OutputType, set_print_timing, set_render_mode, get_render_mode  # noqa: F401

@@ -1,0 +1,53 @@
+// dvg_internal.h -- host-side scene object of the library (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/diffvg_amd.h"
+#include "dvg_scene.h"
+
+namespace dvg {
+
+struct EdgeSample;
+
+int set_error(const std::string &msg);  // records the message, returns 1
+
+// Per-scene scratch reused across renders of the same or smaller size.
+struct Workspace {
+    size_t npix_cap = 0, nsamp_cap = 0, ntiles_cap = 0;
+    float *weight = nullptr;
+    float4 *colors = nullptr;
+    EdgeSample *edges = nullptr;
+    int *edge_tile = nullptr;
+    int *edge_sorted = nullptr;
+    int *tile_count = nullptr;
+    int *tile_off = nullptr;
+    int *tile_cursor = nullptr;
+    int reserve(size_t npix, size_t nsamp, size_t ntiles);
+    void release();
+};
+
+struct SceneHost {
+    int device = 0;
+    int mode = DVG_MODE_TILED;
+    int cw = 0, ch = 0, num_shapes = 0, num_groups = 0;
+    int num_shape_params = 0, num_points_total = 0, num_color_params = 0;
+    void *block = nullptr;        // scene arrays
+    size_t block_bytes = 0;
+    float *grad = nullptr;        // gradient arrays
+    size_t grad_floats = 0;
+    size_t g_params = 0, g_points = 0, g_thick = 0, g_sw = 0, g_color = 0, g_xform = 0, g_filter = 0;
+    int *err = nullptr;
+    hipStream_t last_stream = nullptr;
+    DevScene dev{};
+    Workspace ws;
+};
+
+int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc);
+void free_scene(SceneHost &sc);
+int render_tiled(SceneHost &sc, const dvg_render_args &a);
+int render_plain(SceneHost &sc, const dvg_render_args &a);
+
+}  // namespace dvg

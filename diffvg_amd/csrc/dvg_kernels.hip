@@ -1,0 +1,999 @@
+// dvg_kernels.hip -- render kernels for gfx950 and their launch sequences.
+//
+// TILED schedule (default):
+//   k_weight      per pixel: gathers the filter weights of the samples in its footprint
+//                 (the reference's weight_kernel, diffvg.cpp:1115-1158, as a gather)
+//   k_color       one 256-lane workgroup (four wave64) per pixel tile: the workgroup
+//                 bins the groups whose inflated top-level box meets the tile into LDS,
+//                 in increasing group id, then every lane shades one sample against
+//                 that list, compositing fragments as they are found
+//   k_resolve     per pixel: filter-weighted gather of the sample colours
+//   k_color_bwd   per tile: recompute fragments, reverse compositing sweep, paint
+//                 adjoints, filter-radius adjoint reduced per workgroup
+//   k_edge_sample / k_edge_bin / k_edge   boundary samples binned by pixel tile, both
+//                 sides of each boundary shaded against the tile's candidate list
+// PLAIN schedule: one thread per sample in 64-thread blocks with per-sample top-level
+// BVH traversal, fragment arrays sorted by insertion, atomic splats and Morton-sorted
+// edge samples -- the reference's kernel sequence (diffvg.cpp:1477-1649).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "dvg_internal.h"
+#include "dvg_shade.h"
+
+namespace dvg {
+
+constexpr int kBlock = 256;
+constexpr int kCandCap = 2048;
+
+// Render geometry shared by the kernels.
+struct Geom {
+    int W, H, nsx, nsy, spp;
+    uint64_t seed;
+    int tw, th, tiles_x, tiles_y;
+    float sx_scale, sy_scale;  // canvas / render
+    float margin;
+};
+
+DVG_DEV V2 sample_pos(const Geom &G, int x, int y, int sx, int sy, int idx, bool centre) {
+    Pcg rng = pcg_init(idx, G.seed);
+    float rx = pcg_float(rng);
+    float ry = pcg_float(rng);
+    if (centre) rx = ry = 0.5f;
+    return V2{x + ((float)sx + rx) / G.nsx, y + ((float)sy + ry) / G.nsy};
+}
+// render-pixel position -> canvas position, as render_kernel + sample_color do it
+DVG_DEV V2 to_canvas(const DevScene &S, const Geom &G, V2 pt) {
+    V2 npt{pt.x / G.W, pt.y / G.H};
+    return V2{npt.x * S.cw, npt.y * S.ch};
+}
+
+// Ordered compaction of the groups in [g0, g1) whose inflated top-level box meets `tb`.
+__device__ int build_candidates(const DevScene &S, float4 tb, int g0, int g1, int *list, int *wsum) {
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+    int total = 0;
+    for (int base = g0; base < g1; base += blockDim.x) {
+        int g = base + tid;
+        bool hit = false;
+        if (g < g1) {
+            float4 b = S.g_box[g];
+            float r = S.g_maxr[g];
+            // a single group is the root of the top-level BVH and is visited untested
+            hit = S.ng == 1 || !(b.x - r > tb.z || b.z + r < tb.x || b.y - r > tb.w || b.w + r < tb.y);
+        }
+        uint64_t m = __ballot(hit);
+        int pre = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wid] = __popcll(m);
+        __syncthreads();
+        int off = total, chunk = 0;
+        for (int w = 0; w < nw; w++) {
+            if (w < wid) off += wsum[w];
+            chunk += wsum[w];
+        }
+        if (hit) list[off + pre] = g;
+        total += chunk;
+        __syncthreads();
+    }
+    return total;
+}
+
+DVG_DEV float4 tile_box(const DevScene &S, const Geom &G, int tx, int ty, float extra) {
+    int x0 = tx * G.tw, y0 = ty * G.th;
+    int x1 = min(G.W, x0 + G.tw), y1 = min(G.H, y0 + G.th);
+    float m = G.margin + extra;
+    return make_float4((float)x0 * G.sx_scale - m, (float)y0 * G.sy_scale - m, (float)x1 * G.sx_scale + m,
+                       (float)y1 * G.sy_scale + m);
+}
+
+// Sample of local index `si` in tile (tx, ty); returns false past the image edge.
+DVG_DEV bool tile_sample(const Geom &G, int tx, int ty, int si, int &x, int &y, int &sx, int &sy, int &idx) {
+    int tile_px = G.tw * G.th;
+    if (si >= tile_px * G.spp) return false;
+    int p = si / G.spp, sub = si - p * G.spp;
+    x = tx * G.tw + p % G.tw;
+    y = ty * G.th + p / G.tw;
+    if (x >= G.W || y >= G.H) return false;
+    sx = sub % G.nsx;
+    sy = sub / G.nsx;
+    idx = ((y * G.W + x) * G.nsy + sy) * G.nsx + sx;
+    return true;
+}
+
+// Shades one canvas point against a candidate list (sample_color, diffvg.cpp:525-653).
+DVG_DEV void shade_list(const DevScene &S, V2 pt, const int *list, int n, Compositor &comp, EdgeQuery *eq,
+                        bool &trav_hit) {
+    int target = eq ? eq->group : -1;
+    for (int i = 0; i < n; i++) {
+        int g = list[i];
+        if (S.ng > 1 && !in_box(S.g_box[g], pt, S.g_maxr[g])) continue;
+        M3 c2s = m3_load(S.c2s + 9 * g);
+        V2 local = xform_pt(c2s, pt);
+        int st = S.stroke_type[g];
+        if (st >= 0) {
+            EdgeQuery q{target, eq ? eq->shape : -1, false};
+            if (within_distance_local(S, g, local, eq ? &q : nullptr)) {
+                comp.add(sample_paint(S, st, S.stroke_off[g], S.stroke_ns[g], pt), g, target);
+            }
+            trav_hit |= q.hit;
+        }
+        int ft = S.fill_type[g];
+        if (ft >= 0) {
+            EdgeQuery q{target, eq ? eq->shape : -1, false};
+            if (is_inside_local(S, g, local, eq ? &q : nullptr)) {
+                comp.add(sample_paint(S, ft, S.fill_off[g], S.fill_ns[g], pt), g, target);
+            }
+            trav_hit |= q.hit;
+        }
+    }
+}
+
+// Prefiltered shading of one candidate list (sample_color_prefiltered, diffvg.cpp:835-994).
+template <bool RECORD>
+DVG_DEV void shade_list_prefiltered(const DevScene &S, V2 pt, const int *list, int n, Compositor &comp,
+                                    PFragRec *frags, int &overflow) {
+    const float inf = __builtin_huge_valf();
+    for (int i = 0; i < n; i++) {
+        int g = list[i];
+        if (S.ng > 1 && !in_box(S.g_box[g], pt, S.g_maxr[g])) continue;
+        int st = S.stroke_type[g];
+        if (st >= 0) {
+            int ms = -1;
+            V2 cp{0, 0};
+            PathInfo info{-1, -1, 0.f};
+            float d = inf;
+            compute_distance(S, g, pt, inf, ms, cp, info, d);
+            if (ms >= 0) {
+                float sw = S.stroke_width[ms];
+                float ad = (float)fabs((double)d);
+                float w = smoothstep(ad + sw) - smoothstep(ad - sw);
+                if (w > 0) {
+                    V4 c = sample_paint(S, st, S.stroke_off[g], S.stroke_ns[g], pt);
+                    c.w *= w;
+                    if (RECORD) {
+                        if (comp.nf < kMaxPFrags)
+                            frags[comp.nf] = PFragRec{V3{c.x, c.y, c.z}, c.w, comp.color, comp.alpha, 2 * g + 1, ms,
+                                                      d, cp, info, 1};
+                        else
+                            overflow = 1;
+                    }
+                    comp.add(c, g, -1);
+                }
+            } else {
+                overflow = 2;  // the reference asserts here (diffvg.cpp:868)
+            }
+        }
+        int ft = S.fill_type[g];
+        if (ft >= 0) {
+            int ms = -1;
+            V2 cp{0, 0};
+            PathInfo info{-1, -1, 0.f};
+            float d = inf;
+            bool found = compute_distance(S, g, pt, 1.f, ms, cp, info, d);
+            M3 c2s = m3_load(S.c2s + 9 * g);
+            bool in = is_inside_local(S, g, xform_pt(c2s, pt), nullptr);
+            if (found || in) {
+                if (!in) d = -d;
+                float w = smoothstep(d);
+                if (w > 0) {
+                    V4 c = sample_paint(S, ft, S.fill_off[g], S.fill_ns[g], pt);
+                    c.w *= w;
+                    if (RECORD) {
+                        if (comp.nf < kMaxPFrags)
+                            frags[comp.nf] = PFragRec{V3{c.x, c.y, c.z}, c.w, comp.color, comp.alpha, 2 * g, ms, d,
+                                                      cp, info, found ? 1 : 0};
+                        else
+                            overflow = 1;
+                    }
+                    comp.add(c, g, -1);
+                }
+            }
+        }
+    }
+}
+
+// Shading with the fragment list kept for the adjoint (non-prefiltered).
+DVG_DEV void shade_list_record(const DevScene &S, V2 pt, const int *list, int n, Compositor &comp, FragRec *frags,
+                               int &overflow) {
+    for (int i = 0; i < n; i++) {
+        int g = list[i];
+        if (S.ng > 1 && !in_box(S.g_box[g], pt, S.g_maxr[g])) continue;
+        M3 c2s = m3_load(S.c2s + 9 * g);
+        V2 local = xform_pt(c2s, pt);
+        int st = S.stroke_type[g];
+        if (st >= 0 && within_distance_local(S, g, local, nullptr)) {
+            V4 c = sample_paint(S, st, S.stroke_off[g], S.stroke_ns[g], pt);
+            if (comp.nf < kMaxFrags)
+                frags[comp.nf] = FragRec{V3{c.x, c.y, c.z}, c.w, comp.color, comp.alpha, 2 * g + 1};
+            else
+                overflow = 1;
+            comp.add(c, g, -1);
+        }
+        int ft = S.fill_type[g];
+        if (ft >= 0 && is_inside_local(S, g, local, nullptr)) {
+            V4 c = sample_paint(S, ft, S.fill_off[g], S.fill_ns[g], pt);
+            if (comp.nf < kMaxFrags)
+                frags[comp.nf] = FragRec{V3{c.x, c.y, c.z}, c.w, comp.color, comp.alpha, 2 * g};
+            else
+                overflow = 1;
+            comp.add(c, g, -1);
+        }
+    }
+}
+
+DVG_DEV V4 load_bg(const float *bg, int pix) {
+    float4 b = reinterpret_cast<const float4 *>(bg)[pix];
+    return V4{b.x, b.y, b.z, b.w};
+}
+
+// ------------------------------------------------------------------ TILED kernels
+__global__ __launch_bounds__(kBlock) void k_weight(DevScene S, Geom G, float *weight) {
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= G.W * G.H) return;
+    int px = pix % G.W, py = pix / G.W;
+    float radius = S.filter_radius;
+    int ri = (int)ceil((double)radius);
+    float xc = px + 0.5f, yc = py + 0.5f;
+    float acc = 0.f;
+    for (int y = py - ri; y <= py + ri; y++) {
+        if (y < 0 || y >= G.H) continue;
+        for (int x = px - ri; x <= px + ri; x++) {
+            if (x < 0 || x >= G.W) continue;
+            for (int sy = 0; sy < G.nsy; sy++)
+                for (int sx = 0; sx < G.nsx; sx++) {
+                    int idx = ((y * G.W + x) * G.nsy + sy) * G.nsx + sx;
+                    V2 pt = sample_pos(G, x, y, sx, sy, idx, false);  // always jittered (Q1)
+                    acc += filter_weight(S.filter_type, radius, xc - pt.x, yc - pt.y);
+                }
+        }
+    }
+    weight[pix] = acc;
+}
+
+template <int PRE>
+__global__ __launch_bounds__(kBlock) void k_color(DevScene S, Geom G, const float *bg, float4 *colors, int *err) {
+    constexpr int prefilter = PRE;
+    __shared__ int s_list[kCandCap];
+    __shared__ int s_w[kBlock / 64];
+    int tx = blockIdx.x % G.tiles_x, ty = blockIdx.x / G.tiles_x;
+    int x, y, sx, sy, idx;
+    bool valid = tile_sample(G, tx, ty, threadIdx.x, x, y, sx, sy, idx);
+    V2 pt{0, 0};
+    if (valid) pt = to_canvas(S, G, sample_pos(G, x, y, sx, sy, idx, prefilter != 0));
+    bool has_bg = bg != nullptr;
+    V4 bgc = (valid && has_bg) ? load_bg(bg, y * G.W + x) : V4{0, 0, 0, 0};
+    Compositor comp;
+    comp.init(has_bg, bgc);
+    float4 tb = tile_box(S, G, tx, ty, 0.f);
+    int overflow = 0;
+    for (int c0 = 0; c0 < S.ng; c0 += kCandCap) {
+        int n = build_candidates(S, tb, c0, min(S.ng, c0 + kCandCap), s_list, s_w);
+        if (valid) {
+            if constexpr (PRE) {
+                shade_list_prefiltered<false>(S, pt, s_list, n, comp, nullptr, overflow);
+            } else {
+                bool th = false;
+                shade_list(S, pt, s_list, n, comp, nullptr, th);
+            }
+        }
+        __syncthreads();
+    }
+    if (valid) {
+        V4 c = comp.result(has_bg, bgc);
+        colors[idx] = make_float4(c.x, c.y, c.z, c.w);
+    }
+    if (overflow == 2) atomicOr(err, 2);
+}
+
+__global__ __launch_bounds__(kBlock) void k_resolve(DevScene S, Geom G, int prefilter, const float *weight,
+                                                    const float4 *colors, float *out) {
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= G.W * G.H) return;
+    float ws = weight[pix];
+    if (!(ws > 0)) return;
+    int px = pix % G.W, py = pix / G.W;
+    float radius = S.filter_radius;
+    int ri = (int)ceil((double)radius);
+    float xc = px + 0.5f, yc = py + 0.5f;
+    V4 acc{0, 0, 0, 0};
+    for (int y = py - ri; y <= py + ri; y++) {
+        if (y < 0 || y >= G.H) continue;
+        for (int x = px - ri; x <= px + ri; x++) {
+            if (x < 0 || x >= G.W) continue;
+            for (int sy = 0; sy < G.nsy; sy++)
+                for (int sx = 0; sx < G.nsx; sx++) {
+                    int idx = ((y * G.W + x) * G.nsy + sy) * G.nsx + sx;
+                    V2 pt = sample_pos(G, x, y, sx, sy, idx, prefilter != 0);
+                    float fw = filter_weight(S.filter_type, radius, xc - pt.x, yc - pt.y);
+                    float4 c = colors[idx];
+                    acc += vdiv(fw * V4{c.x, c.y, c.z, c.w}, ws);
+                }
+        }
+    }
+    float4 *o = reinterpret_cast<float4 *>(out) + pix;
+    float4 prev = *o;
+    *o = make_float4(prev.x + acc.x, prev.y + acc.y, prev.z + acc.z, prev.w + acc.w);
+}
+
+template <int PRE>
+__global__ __launch_bounds__(kBlock) void k_color_bwd(DevScene S, Geom G, const float *bg, const float *weight,
+                                                      const float *d_img, float *d_bg, float *d_translation,
+                                                      int *err) {
+    constexpr int prefilter = PRE;
+    __shared__ int s_list[kCandCap];
+    __shared__ int s_w[kBlock / 64];
+    __shared__ float s_red[kBlock / 64];
+    __shared__ float4 s_dbg[kBlock];
+    __shared__ int s_dbg_kind[kBlock];
+    int tx = blockIdx.x % G.tiles_x, ty = blockIdx.x / G.tiles_x;
+    int x, y, sx, sy, idx;
+    bool valid = tile_sample(G, tx, ty, threadIdx.x, x, y, sx, sy, idx);
+    V2 rpt{0, 0}, pt{0, 0};
+    if (valid) {
+        rpt = sample_pos(G, x, y, sx, sy, idx, prefilter != 0);
+        pt = to_canvas(S, G, rpt);
+    }
+    bool has_bg = bg != nullptr;
+    V4 bgc = (valid && has_bg) ? load_bg(bg, y * G.W + x) : V4{0, 0, 0, 0};
+    Compositor comp;
+    comp.init(has_bg, bgc);
+    FragRec frags[PRE ? 1 : kMaxFrags];
+    PFragRec pfrags[PRE ? kMaxPFrags : 1];
+    int overflow = 0;
+    float4 tb = tile_box(S, G, tx, ty, 0.f);
+    for (int c0 = 0; c0 < S.ng; c0 += kCandCap) {
+        int n = build_candidates(S, tb, c0, min(S.ng, c0 + kCandCap), s_list, s_w);
+        if (valid) {
+            if constexpr (PRE)
+                shade_list_prefiltered<true>(S, pt, s_list, n, comp, pfrags, overflow);
+            else
+                shade_list_record(S, pt, s_list, n, comp, frags, overflow);
+        }
+        __syncthreads();
+    }
+    float d_radius = 0.f;
+    int kind = -1;
+    V4 dbg_val{0, 0, 0, 0};
+    if (valid) {
+        V4 d_color = gather_d_color(S.filter_type, S.filter_radius, d_img, weight, G.W, G.H, rpt);
+        float *d_tr = d_translation ? d_translation + 2 * (y * G.W + x) : nullptr;
+        V4 color = comp.result(has_bg, bgc);
+        if (comp.nf == 0) {
+            kind = 0;
+            dbg_val = d_color;
+        } else {
+            V3 fc{color.x, color.y, color.z};
+            V4 d_back;
+            if constexpr (PRE)
+                d_back = composite_adjoint_prefiltered(S, pfrags, min(comp.nf, kMaxPFrags), fc, color.w, d_color, pt,
+                                                       d_tr);
+            else
+                d_back = composite_adjoint(S, frags, min(comp.nf, kMaxFrags), fc, color.w, d_color, pt, d_tr);
+            kind = 1;
+            dbg_val = d_back;
+        }
+        // filter-radius adjoint of the splat (diffvg.cpp:1250-1268)
+        float radius = S.filter_radius;
+        int ri = (int)ceil((double)radius);
+        for (int dy = -ri; dy <= ri; dy++)
+            for (int dx = -ri; dx <= ri; dx++) {
+                int xx = x + dx, yy = y + dy;
+                if (xx >= 0 && xx < G.W && yy >= 0 && yy < G.H) {
+                    float ws = weight[yy * G.W + xx];
+                    if (ws > 0) {
+                        float xc = xx + 0.5f, yc = yy + 0.5f;
+                        float fw = filter_weight(S.filter_type, radius, xc - rpt.x, yc - rpt.y);
+                        float4 dp = reinterpret_cast<const float4 *>(d_img)[yy * G.W + xx];
+                        float dpc = dot(V4{dp.x, dp.y, dp.z, dp.w}, color);
+                        float d_weight = (dpc * ws - fw * dpc * (ws - fw)) / (ws * ws);
+                        d_radius += d_filter_weight(S.filter_type, radius, xc - rpt.x, yc - rpt.y, d_weight);
+                    }
+                }
+            }
+    }
+    // workgroup reduction of the filter-radius adjoint, one atomic per tile
+    float v = d_radius;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+    if (d_bg && has_bg) {
+        s_dbg[threadIdx.x] = make_float4(dbg_val.x, dbg_val.y, dbg_val.z, dbg_val.w);
+        s_dbg_kind[threadIdx.x] = kind;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.f;
+        for (int w = 0; w < kBlock / 64; w++) t += s_red[w];
+        if (t != 0.f) aadd(S.d_filter_radius, t);
+    }
+    // d_background: a sample without fragments assigns, one with fragments adds
+    // (diffvg.cpp:650-656, 700-704); folded per pixel in sample order.
+    if (d_bg && has_bg && threadIdx.x < G.tw * G.th && G.spp * G.tw * G.th <= kBlock) {
+        int p = threadIdx.x;
+        int px = tx * G.tw + p % G.tw, py = ty * G.th + p / G.tw;
+        if (px < G.W && py < G.H) {
+            float4 *dst = reinterpret_cast<float4 *>(d_bg) + (py * G.W + px);
+            float4 acc = *dst;
+            for (int k = 0; k < G.spp; k++) {
+                int si = p * G.spp + k;
+                float4 v4 = s_dbg[si];
+                if (s_dbg_kind[si] == 0)
+                    acc = v4;
+                else if (s_dbg_kind[si] == 1)
+                    acc = make_float4(acc.x + v4.x, acc.y + v4.y, acc.z + v4.z, acc.w + v4.w);
+            }
+            *dst = acc;
+        }
+    }
+    if (overflow) atomicOr(err, overflow);
+}
+
+__global__ __launch_bounds__(kBlock) void k_edge_sample(DevScene S, Geom G, int count, EdgeSample *edges,
+                                                        int *edge_tile, int *tile_count) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    EdgeSample es;
+    int tile = -1;
+    if (make_edge_sample(S, idx, G.seed, es)) {
+        int bx = (int)(es.px * G.W), by = (int)(es.py * G.H);
+        if (bx >= 0 && bx < G.W && by >= 0 && by < G.H) {
+            tile = (by / G.th) * G.tiles_x + (bx / G.tw);
+            atomicAdd(tile_count + tile, 1);
+        }
+    }
+    edges[idx] = es;
+    edge_tile[idx] = tile;
+}
+
+// Exclusive scan of tile counts in one workgroup.
+__global__ __launch_bounds__(1024) void k_scan(const int *in, int *out, int *cursor, int n) {
+    __shared__ int s[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < n ? in[i] : 0;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int t = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+            __syncthreads();
+            s[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < n) {
+            out[i] = carry + s[threadIdx.x] - v;
+            cursor[i] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += s[1023];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_edge_bin(int count, const int *edge_tile, const int *tile_off,
+                                                     int *cursor, int *sorted) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    int t = edge_tile[idx];
+    if (t < 0) return;
+    int pos = tile_off[t] + atomicAdd(cursor + t, 1);
+    sorted[pos] = idx;
+}
+
+__global__ __launch_bounds__(kBlock) void k_edge(DevScene S, Geom G, const EdgeSample *edges, const int *tile_off,
+                                                 const int *tile_count, const int *sorted, const float *bg,
+                                                 const float *weight, const float *d_img, float *d_translation) {
+    __shared__ int s_list[kCandCap];
+    __shared__ int s_w[kBlock / 64];
+    int tile = blockIdx.x;
+    int cnt = tile_count[tile];
+    if (cnt == 0) return;
+    int tx = tile % G.tiles_x, ty = tile / G.tiles_x;
+    float ext = 1e-4f * (float)max(S.cw, S.ch) * 1.01f;
+    float4 tb = tile_box(S, G, tx, ty, ext);
+    int off = tile_off[tile];
+    bool has_bg = bg != nullptr;
+    for (int b0 = 0; b0 < cnt; b0 += kBlock) {
+        int k = b0 + threadIdx.x;
+        bool valid = k < cnt;
+        EdgeSample es;
+        if (valid) es = edges[sorted[off + k]];
+        V2 bp{es.px, es.py}, n{es.nx, es.ny};
+        V2 qi = bp - 1e-4f * n, qo = bp + 1e-4f * n;
+        V2 pin{qi.x * S.cw, qi.y * S.ch}, pout{qo.x * S.cw, qo.y * S.ch};
+        int bx = (int)(es.px * G.W), by = (int)(es.py * G.H);
+        V4 bgc = (valid && has_bg) ? load_bg(bg, by * G.W + bx) : V4{0, 0, 0, 0};
+        Compositor ci, co;
+        ci.init(has_bg, bgc);
+        co.init(has_bg, bgc);
+        EdgeQuery qin{valid ? es.group : -1, valid ? es.shape : -1, false};
+        EdgeQuery qout = qin;
+        bool hin = false, hout = false;
+        for (int c0 = 0; c0 < S.ng; c0 += kCandCap) {
+            int nc = build_candidates(S, tb, c0, min(S.ng, c0 + kCandCap), s_list, s_w);
+            if (valid) {
+                shade_list(S, pin, s_list, nc, ci, &qin, hin);
+                shade_list(S, pout, s_list, nc, co, &qout, hout);
+            }
+            __syncthreads();
+        }
+        if (!valid) continue;
+        bool in_hit = ci.edge_hit(hin), out_hit = co.edge_hit(hout);
+        if (!in_hit && !out_hit) continue;
+        V4 cin = ci.result(has_bg, bgc), cout = co.result(has_bg, bgc);
+        if (!in_hit) {
+            n = -n;
+            V4 tmp = cin;
+            cin = cout;
+            cout = tmp;
+        }
+        V2 sb{es.px * G.W, es.py * G.H};
+        V4 d_color = gather_d_color(S.filter_type, S.filter_radius, d_img, weight, G.W, G.H, sb);
+        d_color = vdiv(d_color, (float)(S.cw * S.ch));
+        float contrib = dot(cin - cout, d_color) / es.pdf;
+        accumulate_boundary_gradient(S, es.group, es.shape, contrib, es.t, n, es.data, V2{es.lx, es.ly});
+        if (d_translation) {
+            aadd(d_translation + 2 * (by * G.W + bx) + 0, n.x * contrib);
+            aadd(d_translation + 2 * (by * G.W + bx) + 1, n.y * contrib);
+        }
+    }
+}
+
+// SDF (sample_distance, diffvg.cpp:709-775) -- a linear scan over groups, as the
+// reference does; one thread per sample.
+__global__ __launch_bounds__(kBlock) void k_sdf(DevScene S, Geom G, const float *eval_positions, int num_eval,
+                                                float *sdf, const float *d_sdf, float *d_translation) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int count = eval_positions ? num_eval : G.W * G.H * G.spp;
+    if (idx >= count) return;
+    V2 rpt;
+    int x, y;
+    if (eval_positions) {
+        rpt = V2{eval_positions[2 * idx], eval_positions[2 * idx + 1]};
+        x = (int)rpt.x;
+        y = (int)rpt.y;
+    } else {
+        int sx = idx % G.nsx;
+        int sy = (idx / G.nsx) % G.nsy;
+        x = (idx / (G.nsx * G.nsy)) % G.W;
+        y = idx / (G.nsx * G.nsy * G.W);
+        rpt = sample_pos(G, x, y, sx, sy, idx, false);
+    }
+    V2 pt = to_canvas(S, G, rpt);
+    float weight = eval_positions ? 1.f : 1.f / G.spp;
+    const float inf = __builtin_huge_valf();
+    int min_group = -1, min_shape = -1;
+    float min_distance = 0.f;
+    V2 closest{0, 0};
+    PathInfo min_info{-1, -1, 0.f};
+    for (int g = S.ng - 1; g >= 0; g--) {
+        int s = -1;
+        V2 p{0, 0};
+        PathInfo li{-1, -1, 0.f};
+        float d = inf;
+        if (compute_distance(S, g, pt, inf, s, p, li, d)) {
+            if (min_group == -1 || d < min_distance) {
+                min_distance = d;
+                min_group = g;
+                min_shape = s;
+                closest = p;
+                min_info = li;
+            }
+        }
+    }
+    float dist_out = min_distance;
+    if (min_group >= 0) {
+        dist_out = min_distance * weight;
+        bool in = false;
+        if (S.fill_type[min_group] >= 0) {
+            M3 c2s = m3_load(S.c2s + 9 * min_group);
+            in = is_inside_local(S, min_group, xform_pt(c2s, pt), nullptr);
+            if (in) dist_out = -dist_out;
+        }
+        if (d_sdf) {
+            float dd = eval_positions ? d_sdf[idx] : d_sdf[y * G.W + x];
+            float d_abs = in ? -dd : dd;
+            // the reference indexes d_translation by the sample pixel even in eval mode;
+            // positions outside the image have no pixel and are skipped here
+            bool in_img = x >= 0 && x < G.W && y >= 0 && y < G.H;
+            float *d_tr = (d_translation && in_img) ? d_translation + 2 * (y * G.W + x) : nullptr;
+            d_compute_distance(S, min_group, min_shape, pt, closest, min_info, d_abs, d_tr);
+        }
+    }
+    if (sdf) aadd(eval_positions ? sdf + idx : sdf + (y * G.W + x), dist_out);
+}
+
+// ------------------------------------------------------------------ PLAIN kernels
+// One thread per sample in 64-thread blocks.  Each sample walks the top-level BVH
+// (child1 first, stack of 64, diffvg.cpp:548-595), collects the reached groups and sorts
+// them by insertion, then shades them with the same per-group code as the tiled path.
+constexpr int kPlainCand = 1024;
+
+DVG_DEV int plain_candidates(const DevScene &S, V2 pt, int *list, int &overflow) {
+    int n = 0;
+    if (S.ng == 0) return 0;
+    int stack[64];
+    int sp = 0;
+    stack[sp++] = 2 * S.ng - 2;
+    while (sp > 0) {
+        const BNode node = S.top[stack[--sp]];
+        if (node.child1 < 0) {
+            if (n < kPlainCand)
+                list[n++] = node.child0;
+            else
+                overflow = 1;
+        } else {
+            const BNode c0 = S.top[node.child0];
+            const BNode c1 = S.top[node.child1];
+            if (in_box(make_float4(c0.minx, c0.miny, c0.maxx, c0.maxy), pt, c0.max_radius) && sp < 64)
+                stack[sp++] = node.child0;
+            if (in_box(make_float4(c1.minx, c1.miny, c1.maxx, c1.maxy), pt, c1.max_radius) && sp < 64)
+                stack[sp++] = node.child1;
+        }
+    }
+    for (int i = 1; i < n; i++) {
+        int j = i, v = list[i];
+        while (j > 0 && list[j - 1] > v) {
+            list[j] = list[j - 1];
+            j--;
+        }
+        list[j] = v;
+    }
+    return n;
+}
+
+__global__ __launch_bounds__(64) void kp_weight(DevScene S, Geom G, float *weight) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= G.W * G.H * G.spp) return;
+    int sx = idx % G.nsx, sy = (idx / G.nsx) % G.nsy;
+    int x = (idx / (G.nsx * G.nsy)) % G.W, y = idx / (G.nsx * G.nsy * G.W);
+    V2 pt = sample_pos(G, x, y, sx, sy, idx, false);
+    float radius = S.filter_radius;
+    int ri = (int)ceil((double)radius);
+    for (int dy = -ri; dy <= ri; dy++)
+        for (int dx = -ri; dx <= ri; dx++) {
+            int xx = x + dx, yy = y + dy;
+            if (xx >= 0 && xx < G.W && yy >= 0 && yy < G.H)
+                aadd(weight + yy * G.W + xx, filter_weight(S.filter_type, radius, xx + 0.5f - pt.x, yy + 0.5f - pt.y));
+        }
+}
+
+template <int PRE>
+__global__ __launch_bounds__(64) void kp_render(DevScene S, Geom G, const float *bg, float *img, const float *weight,
+                                                const float *d_img, float *d_bg, float *d_translation, int *err) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= G.W * G.H * G.spp) return;
+    int sx = idx % G.nsx, sy = (idx / G.nsx) % G.nsy;
+    int x = (idx / (G.nsx * G.nsy)) % G.W, y = idx / (G.nsx * G.nsy * G.W);
+    V2 rpt = sample_pos(G, x, y, sx, sy, idx, PRE != 0);
+    V2 pt = to_canvas(S, G, rpt);
+    bool has_bg = bg != nullptr;
+    V4 bgc = has_bg ? load_bg(bg, y * G.W + x) : V4{0, 0, 0, 0};
+    float *d_tr = d_translation ? d_translation + 2 * (y * G.W + x) : nullptr;
+    int overflow = 0;
+    int list[kPlainCand];
+    int n = plain_candidates(S, pt, list, overflow);
+    Compositor comp;
+    comp.init(has_bg, bgc);
+    FragRec frags[PRE ? 1 : kMaxFrags];
+    PFragRec pfrags[PRE ? kMaxPFrags : 1];
+    if constexpr (PRE) {
+        if (d_img)
+            shade_list_prefiltered<true>(S, pt, list, n, comp, pfrags, overflow);
+        else
+            shade_list_prefiltered<false>(S, pt, list, n, comp, pfrags, overflow);
+    } else {
+        if (d_img) {
+            shade_list_record(S, pt, list, n, comp, frags, overflow);
+        } else {
+            bool th = false;
+            shade_list(S, pt, list, n, comp, nullptr, th);
+        }
+    }
+    V4 color = comp.result(has_bg, bgc);
+    if (d_img) {
+        V4 d_color = gather_d_color(S.filter_type, S.filter_radius, d_img, weight, G.W, G.H, rpt);
+        if (comp.nf == 0) {
+            if (d_bg && has_bg)
+                *reinterpret_cast<float4 *>(d_bg + 4 * (y * G.W + x)) =
+                    make_float4(d_color.x, d_color.y, d_color.z, d_color.w);
+        } else {
+            V4 d_back;
+            V3 fc{color.x, color.y, color.z};
+            if constexpr (PRE)
+                d_back = composite_adjoint_prefiltered(S, pfrags, min(comp.nf, kMaxPFrags), fc, color.w, d_color, pt,
+                                                       d_tr);
+            else
+                d_back = composite_adjoint(S, frags, min(comp.nf, kMaxFrags), fc, color.w, d_color, pt, d_tr);
+            if (d_bg && has_bg) aadd4(d_bg + 4 * (y * G.W + x), d_back);
+        }
+    }
+    float radius = S.filter_radius;
+    int ri = (int)ceil((double)radius);
+    for (int dy = -ri; dy <= ri; dy++)
+        for (int dx = -ri; dx <= ri; dx++) {
+            int xx = x + dx, yy = y + dy;
+            if (xx >= 0 && xx < G.W && yy >= 0 && yy < G.H && weight[yy * G.W + xx] > 0) {
+                float ws = weight[yy * G.W + xx];
+                float xc = xx + 0.5f, yc = yy + 0.5f;
+                float fw = filter_weight(S.filter_type, radius, xc - rpt.x, yc - rpt.y);
+                if (img) aadd4(img + 4 * (yy * G.W + xx), vdiv(fw * color, ws));
+                if (d_img) {
+                    float4 dp = reinterpret_cast<const float4 *>(d_img)[yy * G.W + xx];
+                    float dpc = dot(V4{dp.x, dp.y, dp.z, dp.w}, color);
+                    float d_weight = (dpc * ws - fw * dpc * (ws - fw)) / (ws * ws);
+                    aadd(S.d_filter_radius, d_filter_weight(S.filter_type, radius, xc - rpt.x, yc - rpt.y, d_weight));
+                }
+            }
+        }
+    if (overflow) atomicOr(err, overflow);
+}
+
+__global__ __launch_bounds__(64) void kp_edge_sample(DevScene S, Geom G, int count, EdgeSample *edges, int *ids,
+                                                     uint32_t *keys) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    EdgeSample es;
+    uint32_t key = 0;
+    if (make_edge_sample(S, idx, G.seed, es)) {
+        uint32_t xi = f2u32_x86(es.px * 1023), yi = f2u32_x86(es.py * 1023);
+        key = (expand_bits(xi) << 1u) | expand_bits(yi);
+    }
+    edges[idx] = es;
+    ids[idx] = idx;
+    keys[idx] = key;
+}
+
+__global__ __launch_bounds__(64) void kp_edge(DevScene S, Geom G, const EdgeSample *edges, const int *ids,
+                                              const float *bg, const float *weight, const float *d_img,
+                                              float *d_translation, int *err) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= G.W * G.H * G.spp) return;
+    EdgeSample es = edges[ids[k]];
+    if (es.shape < 0) return;
+    int bx = (int)(es.px * G.W), by = (int)(es.py * G.H);
+    if (bx < 0 || bx >= G.W || by < 0 || by >= G.H) return;
+    bool has_bg = bg != nullptr;
+    V4 bgc = has_bg ? load_bg(bg, by * G.W + bx) : V4{0, 0, 0, 0};
+    V2 bp{es.px, es.py}, n{es.nx, es.ny};
+    V2 qi = bp - 1e-4f * n, qo = bp + 1e-4f * n;
+    V2 pin{qi.x * S.cw, qi.y * S.ch}, pout{qo.x * S.cw, qo.y * S.ch};
+    int overflow = 0;
+    int list[kPlainCand];
+    EdgeQuery qin{es.group, es.shape, false}, qout{es.group, es.shape, false};
+    bool hin = false, hout = false;
+    Compositor ci, co;
+    ci.init(has_bg, bgc);
+    co.init(has_bg, bgc);
+    int nc = plain_candidates(S, pin, list, overflow);
+    shade_list(S, pin, list, nc, ci, &qin, hin);
+    nc = plain_candidates(S, pout, list, overflow);
+    shade_list(S, pout, list, nc, co, &qout, hout);
+    if (overflow) atomicOr(err, overflow);
+    bool in_hit = ci.edge_hit(hin), out_hit = co.edge_hit(hout);
+    if (!in_hit && !out_hit) return;
+    V4 cin = ci.result(has_bg, bgc), cout = co.result(has_bg, bgc);
+    if (!in_hit) {
+        n = -n;
+        V4 tmp = cin;
+        cin = cout;
+        cout = tmp;
+    }
+    V4 d_color = gather_d_color(S.filter_type, S.filter_radius, d_img, weight, G.W, G.H, V2{es.px * G.W, es.py * G.H});
+    d_color = vdiv(d_color, (float)(S.cw * S.ch));
+    float contrib = dot(cin - cout, d_color) / es.pdf;
+    accumulate_boundary_gradient(S, es.group, es.shape, contrib, es.t, n, es.data, V2{es.lx, es.ly});
+    if (d_translation) {
+        aadd(d_translation + 2 * (by * G.W + bx) + 0, n.x * contrib);
+        aadd(d_translation + 2 * (by * G.W + bx) + 1, n.y * contrib);
+    }
+}
+
+// ------------------------------------------------------------------ launch sequences
+int Workspace::reserve(size_t npix, size_t nsamp, size_t ntiles) {
+    if (npix <= npix_cap && nsamp <= nsamp_cap && ntiles <= ntiles_cap) return 0;
+    release();
+    auto al = [](void **p, size_t bytes) { return hipMalloc(p, bytes < 256 ? 256 : bytes); };
+    if (al((void **)&weight, npix * sizeof(float)) != hipSuccess ||
+        al((void **)&colors, nsamp * sizeof(float4)) != hipSuccess ||
+        al((void **)&edges, nsamp * sizeof(EdgeSample)) != hipSuccess ||
+        al((void **)&edge_tile, nsamp * sizeof(int)) != hipSuccess ||
+        al((void **)&edge_sorted, nsamp * sizeof(int)) != hipSuccess ||
+        al((void **)&tile_count, ntiles * sizeof(int)) != hipSuccess ||
+        al((void **)&tile_off, ntiles * sizeof(int)) != hipSuccess ||
+        al((void **)&tile_cursor, ntiles * sizeof(int)) != hipSuccess) {
+        release();
+        return set_error("out of device memory for the render workspace");
+    }
+    npix_cap = npix;
+    nsamp_cap = nsamp;
+    ntiles_cap = ntiles;
+    return 0;
+}
+
+void Workspace::release() {
+    void *ps[] = {weight, colors, edges, edge_tile, edge_sorted, tile_count, tile_off, tile_cursor};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    weight = nullptr;
+    colors = nullptr;
+    edges = nullptr;
+    edge_tile = edge_sorted = tile_count = tile_off = tile_cursor = nullptr;
+    npix_cap = nsamp_cap = ntiles_cap = 0;
+}
+
+static Geom make_geom(const SceneHost &sc, const dvg_render_args &a) {
+    Geom G;
+    G.W = a.width;
+    G.H = a.height;
+    G.nsx = a.num_samples_x;
+    G.nsy = a.num_samples_y;
+    G.spp = G.nsx * G.nsy;
+    G.seed = a.seed;
+    int px = kBlock / G.spp;
+    if (px < 1) px = 1;
+    int tw = 1;
+    while ((tw * 2) * (tw * 2) <= px) tw *= 2;
+    int th = px / tw;
+    if (th < 1) th = 1;
+    G.tw = tw;
+    G.th = th;
+    G.tiles_x = (G.W + tw - 1) / tw;
+    G.tiles_y = (G.H + th - 1) / th;
+    G.sx_scale = (float)sc.cw / (float)G.W;
+    G.sy_scale = (float)sc.ch / (float)G.H;
+    float mx = (float)std::max(std::max(sc.cw, sc.ch), std::max(G.W, G.H));
+    G.margin = 1e-3f + 4e-6f * mx;
+    return G;
+}
+
+#define DVG_CHECK(x)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (x);                                                              \
+        if (e_ != hipSuccess) return set_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static int check_err_flag(SceneHost &sc, hipStream_t st) {
+    int h = 0;
+    DVG_CHECK(hipMemcpyAsync(&h, sc.err, sizeof(int), hipMemcpyDeviceToHost, st));
+    DVG_CHECK(hipStreamSynchronize(st));
+    if (h & 2) return set_error("prefiltered stroke of a group without a path or rect shape (the reference asserts)");
+    if (h & 1) return set_error("fragment list overflow: more overlapping fragments than the reference supports");
+    return 0;
+}
+
+int render_tiled(SceneHost &sc, const dvg_render_args &a) {
+    hipStream_t st = (hipStream_t)a.stream;
+    Geom G = make_geom(sc, a);
+    const bool eval_mode = a.eval_positions && a.num_eval_positions > 0;
+    const int pre = a.use_prefiltering ? 1 : 0;
+    const size_t npix = (size_t)G.W * G.H;
+    const size_t nsamp = npix * G.spp;
+    DevScene S = sc.dev;
+    if (eval_mode && (a.render_image || a.d_render_image))
+        return set_error("eval_positions are supported for the SDF output only (as in the reference)");
+    DVG_CHECK(hipMemsetAsync(sc.err, 0, sizeof(int), st));
+    if (sc.num_shapes == 0) {
+        if (a.render_image && a.background_image)
+            DVG_CHECK(hipMemcpyAsync(a.render_image, a.background_image, npix * 16, hipMemcpyDeviceToDevice, st));
+        return 0;
+    }
+    const bool color_out = a.render_image || a.d_render_image;
+    const bool sdf_out = a.render_sdf || a.d_render_sdf;
+    int ntiles = G.tiles_x * G.tiles_y;
+    if (color_out) {
+        if (int e = sc.ws.reserve(npix, nsamp, ntiles)) return e;
+        k_weight<<<(unsigned)((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(S, G, sc.ws.weight);
+        if (a.render_image) {
+            if (pre)
+                k_color<1><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, sc.ws.colors, sc.err);
+            else
+                k_color<0><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, sc.ws.colors, sc.err);
+            k_resolve<<<(unsigned)((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(S, G, pre, sc.ws.weight,
+                                                                                  sc.ws.colors, a.render_image);
+        }
+        if (a.d_render_image) {
+            if (pre)
+                k_color_bwd<1><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, sc.ws.weight, a.d_render_image,
+                                                          a.d_background_image, a.d_translation, sc.err);
+            else
+                k_color_bwd<0><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, sc.ws.weight, a.d_render_image,
+                                                          a.d_background_image, a.d_translation, sc.err);
+            if (!pre) {
+                DVG_CHECK(hipMemsetAsync(sc.ws.tile_count, 0, sizeof(int) * ntiles, st));
+                k_edge_sample<<<(unsigned)((nsamp + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+                    S, G, (int)nsamp, sc.ws.edges, sc.ws.edge_tile, sc.ws.tile_count);
+                k_scan<<<1, 1024, 0, st>>>(sc.ws.tile_count, sc.ws.tile_off, sc.ws.tile_cursor, ntiles);
+                k_edge_bin<<<(unsigned)((nsamp + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+                    (int)nsamp, sc.ws.edge_tile, sc.ws.tile_off, sc.ws.tile_cursor, sc.ws.edge_sorted);
+                k_edge<<<ntiles, kBlock, 0, st>>>(S, G, sc.ws.edges, sc.ws.tile_off, sc.ws.tile_count,
+                                                  sc.ws.edge_sorted, a.background_image, sc.ws.weight,
+                                                  a.d_render_image, a.d_translation);
+            }
+        }
+    }
+    if (sdf_out) {
+        int count = eval_mode ? a.num_eval_positions : (int)nsamp;
+        k_sdf<<<(count + kBlock - 1) / kBlock, kBlock, 0, st>>>(S, G, eval_mode ? a.eval_positions : nullptr,
+                                                                 a.num_eval_positions, a.render_sdf, a.d_render_sdf,
+                                                                 a.d_translation);
+    }
+    DVG_CHECK(hipGetLastError());
+    if (a.d_render_image || pre) return check_err_flag(sc, st);
+    return 0;
+}
+
+int render_plain(SceneHost &sc, const dvg_render_args &a) {
+    hipStream_t st = (hipStream_t)a.stream;
+    Geom G = make_geom(sc, a);
+    const bool eval_mode = a.eval_positions && a.num_eval_positions > 0;
+    const int pre = a.use_prefiltering ? 1 : 0;
+    const size_t npix = (size_t)G.W * G.H;
+    const size_t nsamp = npix * G.spp;
+    DevScene S = sc.dev;
+    if (eval_mode && (a.render_image || a.d_render_image))
+        return set_error("eval_positions are supported for the SDF output only (as in the reference)");
+    DVG_CHECK(hipMemsetAsync(sc.err, 0, sizeof(int), st));
+    if (sc.num_shapes == 0) {
+        if (a.render_image && a.background_image)
+            DVG_CHECK(hipMemcpyAsync(a.render_image, a.background_image, npix * 16, hipMemcpyDeviceToDevice, st));
+        DVG_CHECK(hipStreamSynchronize(st));
+        return 0;
+    }
+    float *weight = nullptr;
+    if (!eval_mode) {
+        DVG_CHECK(hipMalloc(&weight, npix * sizeof(float)));
+        DVG_CHECK(hipMemsetAsync(weight, 0, npix * sizeof(float), st));
+    }
+    const bool color_out = a.render_image || a.d_render_image;
+    const bool sdf_out = a.render_sdf || a.d_render_sdf;
+    if ((color_out || sdf_out) && weight) {
+        kp_weight<<<(unsigned)((nsamp + 63) / 64), 64, 0, st>>>(S, G, weight);
+    }
+    if (color_out) {
+        if (pre)
+            kp_render<1><<<(unsigned)((nsamp + 63) / 64), 64, 0, st>>>(S, G, a.background_image, a.render_image,
+                                                                       weight, a.d_render_image, a.d_background_image,
+                                                                       a.d_translation, sc.err);
+        else
+            kp_render<0><<<(unsigned)((nsamp + 63) / 64), 64, 0, st>>>(S, G, a.background_image, a.render_image,
+                                                                       weight, a.d_render_image, a.d_background_image,
+                                                                       a.d_translation, sc.err);
+    }
+    if (sdf_out) {
+        int count = eval_mode ? a.num_eval_positions : (int)nsamp;
+        k_sdf<<<(count + 63) / 64, 64, 0, st>>>(S, G, eval_mode ? a.eval_positions : nullptr, a.num_eval_positions,
+                                                a.render_sdf, a.d_render_sdf, a.d_translation);
+    }
+    if (!pre && a.d_render_image) {
+        EdgeSample *edges = nullptr;
+        int *ids = nullptr, *ids2 = nullptr;
+        uint32_t *keys = nullptr, *keys2 = nullptr;
+        void *tmp = nullptr;
+        size_t tmp_bytes = 0;
+        DVG_CHECK(hipMalloc(&edges, nsamp * sizeof(EdgeSample)));
+        DVG_CHECK(hipMalloc(&ids, nsamp * sizeof(int)));
+        DVG_CHECK(hipMalloc(&ids2, nsamp * sizeof(int)));
+        DVG_CHECK(hipMalloc(&keys, nsamp * sizeof(uint32_t)));
+        DVG_CHECK(hipMalloc(&keys2, nsamp * sizeof(uint32_t)));
+        kp_edge_sample<<<(unsigned)((nsamp + 63) / 64), 64, 0, st>>>(S, G, (int)nsamp, edges, ids, keys);
+        DVG_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys2, ids, ids2, (int)nsamp, 0, 32, st));
+        DVG_CHECK(hipMalloc(&tmp, tmp_bytes));
+        DVG_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys2, ids, ids2, (int)nsamp, 0, 32, st));
+        kp_edge<<<(unsigned)((nsamp + 63) / 64), 64, 0, st>>>(S, G, edges, ids2, a.background_image, weight,
+                                                              a.d_render_image, a.d_translation, sc.err);
+        DVG_CHECK(hipStreamSynchronize(st));
+        DVG_CHECK(hipFree(tmp));
+        DVG_CHECK(hipFree(edges));
+        DVG_CHECK(hipFree(ids));
+        DVG_CHECK(hipFree(ids2));
+        DVG_CHECK(hipFree(keys));
+        DVG_CHECK(hipFree(keys2));
+    }
+    DVG_CHECK(hipGetLastError());
+    DVG_CHECK(hipStreamSynchronize(st));
+    if (weight) DVG_CHECK(hipFree(weight));
+    return check_err_flag(sc, st);
+}
+
+}  // namespace dvg

@@ -1,0 +1,481 @@
+"""PyTorch front end with the interface of pydiffvg/render_pytorch.py.
+
+``RenderFunction.serialize_scene`` produces the reference's positional argument list
+(render_pytorch.py:22-172) and ``RenderFunction.apply(width, height, num_samples_x,
+num_samples_y, seed, background_image, *args)`` renders it (render_pytorch.py:174-428);
+``backward`` returns gradients in the reference's order (render_pytorch.py:668-868) and
+``render_grad`` returns the screen-space translation gradient (render_pytorch.py:430-666).
+
+Differences from the reference, all deliberate:
+* scene tensors are not copied to the CPU one by one; the forward gathers them with one
+  concatenation per kind and one host copy, and gradients come back on the device of
+  the corresponding input;
+* rendering runs on the GPU only (``set_use_gpu(False)`` makes ``apply`` raise);
+* the backward clears the scene's gradient buffers before it runs, so a second backward
+  through the same graph does not add to the first;
+* ``render_grad`` with a background image works (the reference reads an undefined
+  variable there, render_pytorch.py:638).
+"""
+import time
+import warnings
+from enum import IntEnum
+
+import numpy as np
+import torch
+
+from .device import get_device, get_use_gpu
+from . import diffvg
+from . import _lib
+from .color import LinearGradient, RadialGradient
+from .pixel_filter import PixelFilter
+from .shape import Circle, Ellipse, Path, Polygon, Rect
+
+print_timing = False
+
+_render_mode = _lib.MODE_TILED
+
+
+def set_print_timing(val):
+    global print_timing
+    print_timing = val
+
+
+def set_render_mode(mode):
+    """Selects the TILED (default) or PLAIN render schedule for RenderFunction."""
+    global _render_mode
+    if isinstance(mode, str):
+        mode = {'tiled': _lib.MODE_TILED, 'plain': _lib.MODE_PLAIN}[mode]
+    _render_mode = int(mode)
+
+
+def get_render_mode():
+    return _render_mode
+
+
+class OutputType(IntEnum):
+    color = 1
+    sdf = 2
+
+
+def _filter_int(t):
+    return int(getattr(t, 'value', t))
+
+
+class _Flat:
+    """Flattened scene: the dvg_scene_desc arrays plus the map from argument positions
+    to gradient slices."""
+
+
+def _gather_float(tensors):
+    """Concatenates float tensors (any devices) into one host float32 array."""
+    if not tensors:
+        return np.zeros(0, np.float32)
+    flat = [t.detach().reshape(-1) for t in tensors]
+    devs = {t.device for t in flat}
+    if len(devs) == 1:
+        cat = torch.cat([t.to(torch.float32) for t in flat])
+        return cat.cpu().numpy()
+    return np.concatenate([t.to(torch.float32).cpu().numpy() for t in flat])
+
+
+def flatten_args(args):
+    """serialize_scene argument list -> (_Flat, canvas info)."""
+    canvas_width, canvas_height, num_shapes, num_shape_groups = args[0], args[1], args[2], args[3]
+    output_type, use_prefiltering, eval_positions = args[4], args[5], args[6]
+    f = _Flat()
+    f.canvas = (int(canvas_width), int(canvas_height))
+    f.output_type = output_type
+    f.use_prefiltering = bool(use_prefiltering)
+    f.eval_positions = eval_positions
+    i = 7
+    ns = num_shapes
+    shape_type = np.zeros(ns, np.int32)
+    param_off = np.full(ns, -1, np.int32)
+    ctrl_off = np.zeros(ns, np.int32)
+    nbase = np.zeros(ns, np.int32)
+    pt_off = np.zeros(ns, np.int32)
+    npts = np.zeros(ns, np.int32)
+    has_thick = np.zeros(ns, np.uint8)
+    closed = np.zeros(ns, np.uint8)
+    approx = np.zeros(ns, np.uint8)
+    # float sources in description order, per kind
+    src_params, src_points, src_thick, src_sw, src_color, src_xform = [], [], [], [], [], []
+    thick_fill = []  # (offset, count) of zero thickness ranges
+    ncp_list = []
+    grad_map = []  # (arg index, kind, offset, count, shape)
+    n_params = n_points = n_ctrl = 0
+    for s in range(ns):
+        t = args[i]
+        i += 1
+        if t == diffvg.ShapeType.circle:
+            radius, center = args[i], args[i + 1]
+            shape_type[s] = 0
+            param_off[s] = n_params
+            src_params += [radius, center]
+            grad_map.append((i, 'shape_params', n_params, 1, tuple(radius.shape)))
+            grad_map.append((i + 1, 'shape_params', n_params + 1, 2, tuple(center.shape)))
+            n_params += 3
+            i += 2
+        elif t == diffvg.ShapeType.ellipse:
+            radius, center = args[i], args[i + 1]
+            shape_type[s] = 1
+            param_off[s] = n_params
+            src_params += [radius, center]
+            grad_map.append((i, 'shape_params', n_params, 2, tuple(radius.shape)))
+            grad_map.append((i + 1, 'shape_params', n_params + 2, 2, tuple(center.shape)))
+            n_params += 4
+            i += 2
+        elif t == diffvg.ShapeType.path:
+            ncp, points, thickness, is_closed, use_approx = args[i], args[i + 1], args[i + 2], args[i + 3], args[i + 4]
+            n = int(points.shape[0])
+            nb = int(ncp.shape[0])
+            shape_type[s] = 2
+            ctrl_off[s] = n_ctrl
+            nbase[s] = nb
+            pt_off[s] = n_points
+            npts[s] = n
+            closed[s] = 1 if is_closed else 0
+            approx[s] = 1 if use_approx else 0
+            ncp_list.append(ncp)
+            src_points.append(points)
+            grad_map.append((i + 1, 'points', 2 * n_points, 2 * n, tuple(points.shape)))
+            if thickness is not None:
+                has_thick[s] = 1
+                src_thick.append((n_points, thickness))
+                grad_map.append((i + 2, 'thickness', n_points, n, tuple(thickness.shape)))
+            n_points += n
+            n_ctrl += nb
+            i += 5
+        elif t == diffvg.ShapeType.rect:
+            p_min, p_max = args[i], args[i + 1]
+            shape_type[s] = 3
+            param_off[s] = n_params
+            src_params += [p_min, p_max]
+            grad_map.append((i, 'shape_params', n_params, 2, tuple(p_min.shape)))
+            grad_map.append((i + 1, 'shape_params', n_params + 2, 2, tuple(p_max.shape)))
+            n_params += 4
+            i += 2
+        else:
+            raise ValueError('unknown shape type %r' % (t,))
+        sw = args[i]
+        src_sw.append(sw)
+        if not (shape_type[s] == 2 and has_thick[s]):
+            grad_map.append((i, 'stroke_width', s, 1, tuple(sw.shape)))
+        i += 1
+    ng = num_shape_groups
+    g_off = np.zeros(ng, np.int32)
+    g_n = np.zeros(ng, np.int32)
+    ids_list = []
+    paint = {k: np.zeros(ng, np.int32) for k in ('fill_type', 'fill_offset', 'fill_num_stops', 'stroke_type',
+                                                  'stroke_offset', 'stroke_num_stops')}
+    paint['fill_type'][:] = -1
+    paint['stroke_type'][:] = -1
+    even_odd = np.zeros(ng, np.uint8)
+    n_color = 0
+    n_ids = 0
+    for g in range(ng):
+        ids = args[i]
+        i += 1
+        ids_list.append(ids)
+        g_off[g] = n_ids
+        g_n[g] = int(ids.shape[0])
+        n_ids += int(ids.shape[0])
+        for kind in ('fill', 'stroke'):
+            ct = args[i]
+            i += 1
+            if ct is None:
+                continue
+            paint[kind + '_offset'][g] = n_color
+            if ct == diffvg.ColorType.constant:
+                c = args[i]
+                paint[kind + '_type'][g] = 0
+                src_color.append(c)
+                grad_map.append((i, 'color_params', n_color, 4, tuple(c.shape)))
+                n_color += 4
+                i += 1
+            else:
+                a, b, offsets, colors = args[i], args[i + 1], args[i + 2], args[i + 3]
+                nstops = int(offsets.shape[0])
+                if int(colors.shape[0]) != nstops:
+                    raise ValueError('gradient offsets and stop colours differ in count')
+                paint[kind + '_type'][g] = 1 if ct == diffvg.ColorType.linear_gradient else 2
+                paint[kind + '_num_stops'][g] = nstops
+                src_color += [a, b, offsets, colors]
+                grad_map.append((i, 'color_params', n_color, 2, tuple(a.shape)))
+                grad_map.append((i + 1, 'color_params', n_color + 2, 2, tuple(b.shape)))
+                grad_map.append((i + 2, 'color_params', n_color + 4, nstops, tuple(offsets.shape)))
+                grad_map.append((i + 3, 'color_params', n_color + 4 + nstops, 4 * nstops, tuple(colors.shape)))
+                n_color += 4 + 5 * nstops
+                i += 4
+        even_odd[g] = 1 if args[i] else 0
+        xf = args[i + 1]
+        src_xform.append(xf)
+        grad_map.append((i + 1, 'shape_to_canvas', 9 * g, 9, tuple(xf.shape)))
+        i += 2
+    filter_type = args[i]
+    filter_radius = args[i + 1]
+    grad_map.append((i + 1, 'filter_radius', 0, 1, tuple(filter_radius.shape)))
+    f.num_args = i + 2
+
+    # one concatenation + host copy for all float data
+    groups = [src_params, src_points, [t for _, t in src_thick], src_sw, src_color, src_xform, [filter_radius]]
+    sizes = [sum(int(t.numel()) for t in grp) for grp in groups]
+    host = _gather_float([t for grp in groups for t in grp])
+    parts = np.split(host, np.cumsum(sizes)[:-1])
+    params, points, thick_vals, sw, color, xform, frad = parts
+    thickness = np.zeros(n_points, np.float32)
+    k = 0
+    for off, t in src_thick:
+        n = int(t.numel())
+        thickness[off:off + n] = thick_vals[k:k + n]
+        k += n
+    sw_arr = np.zeros(ns, np.float32)
+    k = 0
+    for s in range(ns):
+        sw_arr[s] = sw[k] if not (shape_type[s] == 2 and has_thick[s]) else 0.0
+        k += int(src_sw[s].numel())
+    ncp = (torch.cat([c.reshape(-1).to(torch.int32).cpu() for c in ncp_list]).numpy()
+           if ncp_list else np.zeros(0, np.int32))
+    gids = (torch.cat([c.reshape(-1).to(torch.int32).cpu() for c in ids_list]).numpy()
+            if ids_list else np.zeros(0, np.int32))
+    f.desc = dict(
+        canvas_width=int(canvas_width), canvas_height=int(canvas_height), num_shapes=ns,
+        shape_type=shape_type, stroke_width=sw_arr, shape_param_offset=param_off,
+        shape_params=np.ascontiguousarray(params, np.float32), num_shape_params=int(n_params),
+        path_ctrl_offset=ctrl_off, path_num_base_points=nbase, path_point_offset=pt_off, path_num_points=npts,
+        path_has_thickness=has_thick, path_is_closed=closed, path_use_distance_approx=approx,
+        num_control_points=np.ascontiguousarray(ncp, np.int32), num_ctrl_total=int(n_ctrl),
+        points=np.ascontiguousarray(points, np.float32), thickness=thickness, num_points_total=int(n_points),
+        num_groups=ng, group_shape_offset=g_off, group_num_shapes=g_n,
+        group_shape_ids=np.ascontiguousarray(gids, np.int32), num_group_shape_ids=int(n_ids),
+        fill_type=paint['fill_type'], fill_offset=paint['fill_offset'], fill_num_stops=paint['fill_num_stops'],
+        stroke_type=paint['stroke_type'], stroke_offset=paint['stroke_offset'],
+        stroke_num_stops=paint['stroke_num_stops'],
+        color_params=np.ascontiguousarray(color, np.float32), num_color_params=int(n_color),
+        use_even_odd_rule=even_odd, shape_to_canvas=np.ascontiguousarray(xform, np.float32),
+        filter_type=_filter_int(filter_type), filter_radius=float(frad[0]))
+    f.grad_map = grad_map
+    return f
+
+
+def _check_gpu():
+    if not get_use_gpu() or get_device().type != 'cuda':
+        raise RuntimeError('diffvg_amd renders on the GPU only; call diffvg_amd.set_use_gpu(True) '
+                           'on a machine with a ROCm GPU')
+
+
+def _stream():
+    return ctypes_stream(torch.cuda.current_stream())
+
+
+def ctypes_stream(s):
+    return s.cuda_stream
+
+
+def _grad_tensors(flat, grads, args, num_args):
+    """Gradient list in argument order (render_pytorch.py:713-868)."""
+    d = [None] * num_args
+    dev_cache = {}
+    for (ai, kind, off, cnt, shape) in flat.grad_map:
+        src = args[ai]
+        arr = grads[kind][off:off + cnt]
+        key = (kind, src.device)
+        t = torch.from_numpy(np.ascontiguousarray(arr)).reshape(shape)
+        if src.device.type != 'cpu':
+            t = t.to(src.device, non_blocking=False)
+        d[ai] = t.to(src.dtype) if src.dtype != torch.float32 else t
+        dev_cache[key] = True
+    return d
+
+
+class RenderFunction(torch.autograd.Function):
+    """The PyTorch interface of the render path."""
+
+    @staticmethod
+    def serialize_scene(canvas_width,
+                        canvas_height,
+                        shapes,
+                        shape_groups,
+                        filter=PixelFilter(type=diffvg.FilterType.box, radius=torch.tensor(0.5)),
+                        output_type=OutputType.color,
+                        use_prefiltering=False,
+                        eval_positions=torch.tensor([])):
+        """Scene -> flat argument list in the reference's order (render_pytorch.py:22-172)."""
+        num_shapes = len(shapes)
+        num_shape_groups = len(shape_groups)
+        args = [canvas_width, canvas_height, num_shapes, num_shape_groups, output_type, use_prefiltering,
+                eval_positions.to(get_device())]
+        for shape in shapes:
+            use_thickness = False
+            if isinstance(shape, Circle):
+                args += [diffvg.ShapeType.circle, shape.radius, shape.center]
+            elif isinstance(shape, Ellipse):
+                args += [diffvg.ShapeType.ellipse, shape.radius, shape.center]
+            elif isinstance(shape, Path):
+                assert shape.points.shape[1] == 2
+                assert torch.isfinite(shape.points).all()
+                args += [diffvg.ShapeType.path, shape.num_control_points.to(torch.int32), shape.points]
+                if len(shape.stroke_width.shape) > 0 and shape.stroke_width.shape[0] > 1:
+                    assert torch.isfinite(shape.stroke_width).all()
+                    use_thickness = True
+                    args.append(shape.stroke_width)
+                else:
+                    args.append(None)
+                args += [shape.is_closed, shape.use_distance_approx]
+            elif isinstance(shape, Polygon):
+                assert shape.points.shape[1] == 2
+                n = shape.points.shape[0] if shape.is_closed else shape.points.shape[0] - 1
+                args += [diffvg.ShapeType.path, torch.zeros(n, dtype=torch.int32), shape.points, None,
+                         shape.is_closed, False]
+            elif isinstance(shape, Rect):
+                args += [diffvg.ShapeType.rect, shape.p_min, shape.p_max]
+            else:
+                raise TypeError('unsupported shape %r' % (type(shape),))
+            args.append(torch.tensor(0.0) if use_thickness else shape.stroke_width)
+        for shape_group in shape_groups:
+            args.append(shape_group.shape_ids.to(torch.int32))
+            for kind, color in (('fill', shape_group.fill_color), ('stroke', shape_group.stroke_color)):
+                if color is None:
+                    args.append(None)
+                elif isinstance(color, torch.Tensor):
+                    args += [diffvg.ColorType.constant, color]
+                elif isinstance(color, LinearGradient):
+                    args += [diffvg.ColorType.linear_gradient, color.begin, color.end, color.offsets,
+                             color.stop_colors]
+                elif isinstance(color, RadialGradient):
+                    args += [diffvg.ColorType.radial_gradient, color.center, color.radius, color.offsets,
+                             color.stop_colors]
+                else:
+                    raise TypeError('unsupported paint %r' % (type(color),))
+                if kind == 'fill' and color is not None:
+                    for shape_id in shape_group.shape_ids:
+                        if isinstance(shapes[int(shape_id)], Path) and not shapes[int(shape_id)].is_closed:
+                            warnings.warn('Detected non-closed paths with fill color. This might causes '
+                                          'unexpected results.', Warning)
+            args.append(shape_group.use_even_odd_rule)
+            args.append(shape_group.shape_to_canvas.contiguous())
+        args.append(filter.type)
+        args.append(filter.radius)
+        return args
+
+    @staticmethod
+    def _prepare(width, height, background_image, args, output_type=None):
+        _check_gpu()
+        flat = flatten_args(args)
+        dev = get_device()
+        start = time.time()
+        scene = _lib.DeviceScene(flat.desc, dev.index if dev.index is not None else -1, _render_mode)
+        if print_timing:
+            print('Scene construction, time: %.5f s' % (time.time() - start))
+        return flat, scene
+
+    @staticmethod
+    def forward(ctx, width, height, num_samples_x, num_samples_y, seed, background_image, *args):
+        flat, scene = RenderFunction._prepare(width, height, background_image, args)
+        dev = get_device()
+        eval_positions = flat.eval_positions
+        if flat.output_type == OutputType.color:
+            assert eval_positions.shape[0] == 0
+            rendered_image = torch.zeros(height, width, 4, device=dev)
+        else:
+            assert flat.output_type == OutputType.sdf
+            if eval_positions.shape[0] == 0:
+                rendered_image = torch.zeros(height, width, 1, device=dev)
+            else:
+                rendered_image = torch.zeros(eval_positions.shape[0], 1, device=dev)
+        if background_image is not None:
+            background_image = background_image.to(dev)
+            if background_image.shape[2] == 3:
+                raise NotImplementedError('Background image must have 4 channels, not 3. Add a fourth channel with '
+                                          'all ones via torch.ones().')
+            background_image = background_image.contiguous().to(torch.float32)
+            assert background_image.shape[0] == rendered_image.shape[0]
+            assert background_image.shape[1] == rendered_image.shape[1]
+            assert background_image.shape[2] == 4
+        ev = eval_positions.to(dev).to(torch.float32).contiguous() if eval_positions.shape[0] > 0 else None
+        start = time.time()
+        scene.render(width, height, num_samples_x, num_samples_y, seed,
+                     background=background_image,
+                     render_image=rendered_image if flat.output_type == OutputType.color else None,
+                     render_sdf=rendered_image if flat.output_type == OutputType.sdf else None,
+                     use_prefiltering=flat.use_prefiltering, eval_positions=ev,
+                     num_eval_positions=0 if ev is None else ev.shape[0], stream=_stream())
+        if print_timing:
+            torch.cuda.synchronize()
+            print('Forward pass, time: %.5f s' % (time.time() - start))
+        ctx.scene = scene
+        ctx.flat = flat
+        ctx.args = args
+        ctx.background_image = background_image
+        ctx.width = width
+        ctx.height = height
+        ctx.num_samples_x = num_samples_x
+        ctx.num_samples_y = num_samples_y
+        ctx.seed = seed
+        ctx.eval_positions = ev
+        return rendered_image
+
+    @staticmethod
+    def render_grad(grad_img, width, height, num_samples_x, num_samples_y, seed, background_image, *args):
+        """Screen-space gradient of the image w.r.t. a translation of the scene
+        (render_pytorch.py:430-666)."""
+        if not grad_img.is_contiguous():
+            grad_img = grad_img.contiguous()
+        assert torch.isfinite(grad_img).all()
+        flat, scene = RenderFunction._prepare(width, height, background_image, args)
+        dev = get_device()
+        if flat.output_type == OutputType.color:
+            assert grad_img.shape[2] == 4
+        else:
+            assert grad_img.shape[2] == 1
+        if background_image is not None:
+            background_image = background_image.to(dev)
+            if background_image.shape[2] == 3:
+                background_image = torch.cat((background_image, torch.ones(
+                    background_image.shape[0], background_image.shape[1], 1, device=dev)), dim=2)
+            background_image = background_image.contiguous().to(torch.float32)
+            assert background_image.shape[0] == grad_img.shape[0]
+            assert background_image.shape[1] == grad_img.shape[1]
+        grad_img = grad_img.to(dev).to(torch.float32).contiguous()
+        translation_grad_image = torch.zeros(height, width, 2, device=dev)
+        ev = flat.eval_positions
+        ev = ev.to(dev).to(torch.float32).contiguous() if ev.shape[0] > 0 else None
+        start = time.time()
+        scene.render(width, height, num_samples_x, num_samples_y, seed, background=background_image,
+                     d_render_image=grad_img if flat.output_type == OutputType.color else None,
+                     d_render_sdf=grad_img if flat.output_type == OutputType.sdf else None,
+                     d_translation=translation_grad_image, use_prefiltering=flat.use_prefiltering,
+                     eval_positions=ev, num_eval_positions=0 if ev is None else ev.shape[0], stream=_stream())
+        if print_timing:
+            torch.cuda.synchronize()
+            print('Gradient pass, time: %.5f s' % (time.time() - start))
+        assert torch.isfinite(translation_grad_image).all()
+        return translation_grad_image
+
+    @staticmethod
+    def backward(ctx, grad_img):
+        if not grad_img.is_contiguous():
+            grad_img = grad_img.contiguous()
+        assert torch.isfinite(grad_img).all()
+        scene = ctx.scene
+        flat = ctx.flat
+        background_image = ctx.background_image
+        d_background_image = torch.zeros_like(background_image) if background_image is not None else None
+        grad_img = grad_img.to(torch.float32).contiguous()
+        start = time.time()
+        scene.zero_grads(_stream())
+        scene.render(ctx.width, ctx.height, ctx.num_samples_x, ctx.num_samples_y, ctx.seed,
+                     background=background_image, d_background=d_background_image,
+                     d_render_image=grad_img if flat.output_type == OutputType.color else None,
+                     d_render_sdf=grad_img if flat.output_type == OutputType.sdf else None,
+                     use_prefiltering=flat.use_prefiltering, eval_positions=ctx.eval_positions,
+                     num_eval_positions=0 if ctx.eval_positions is None else ctx.eval_positions.shape[0],
+                     stream=_stream())
+        grads = scene.grads()
+        if print_timing:
+            print('Backward pass, time: %.5f s' % (time.time() - start))
+        d_scene = _grad_tensors(flat, grads, ctx.args, flat.num_args)
+        for t in d_scene:
+            if t is not None:
+                assert torch.isfinite(t).all()
+        return tuple([None, None, None, None, None, d_background_image] + d_scene)

@@ -1,0 +1,148 @@
+/*
+ * diffvg_amd.h -- C ABI of the MI355X render path of diffvg.
+ *
+ * The library replaces the native half of diffvg's render path: the pybind11
+ * surface `diffvg.Scene(...)`, `diffvg.render(...)` and the per-shape gradient
+ * getters (reference diffvg.cpp:1651-1792), backed by hand-written HIP kernels for
+ * gfx950.  No torch types appear here: scenes are described by flat host arrays,
+ * images by device pointers, and every call returns 0 on success or a non-zero
+ * code with the message in dvg_last_error().  The library never aborts the process.
+ */
+#ifndef DIFFVG_AMD_H
+#define DIFFVG_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DVG_ABI_VERSION 1
+
+/* ShapeType (reference shape.h:9-14), ColorType (color.h:7-11), FilterType
+ * (filter.h:6-11) plus the Gaussian filter, which the reference does not have. */
+enum { DVG_SHAPE_CIRCLE = 0, DVG_SHAPE_ELLIPSE = 1, DVG_SHAPE_PATH = 2, DVG_SHAPE_RECT = 3 };
+enum { DVG_COLOR_NONE = -1, DVG_COLOR_CONSTANT = 0, DVG_COLOR_LINEAR = 1, DVG_COLOR_RADIAL = 2 };
+enum { DVG_FILTER_BOX = 0, DVG_FILTER_TENT = 1, DVG_FILTER_PARABOLIC = 2, DVG_FILTER_HANN = 3,
+       DVG_FILTER_GAUSSIAN = 4 };
+
+/* Render schedules.  TILED is the MI355X path (tile-binned candidates, one wavefront
+ * per pixel tile, gathered filter resolve, tile-binned edge samples).  PLAIN keeps the
+ * reference's launch structure (one thread per sample in 64-thread blocks, per-sample
+ * top-level BVH traversal, atomic splats, Morton-sorted edge samples, per-call
+ * allocations, synchronous) and serves as the comparison baseline. */
+enum { DVG_MODE_TILED = 0, DVG_MODE_PLAIN = 1 };
+
+/* Flat scene description: the fields of Scene(canvas_width, canvas_height, shapes,
+ * shape_groups, filter) (reference scene.h:19-25, shape.h:16-169, color.h:13-63)
+ * laid out as structure-of-arrays.  All pointers are HOST pointers and are copied
+ * during dvg_scene_create.
+ *   circle params  : radius, cx, cy          ellipse params: rx, ry, cx, cy
+ *   rect params    : x0, y0, x1, y1          path: num_control_points[], points[] (x,y)
+ *   constant paint : r, g, b, a
+ *   linear paint   : bx, by, ex, ey, offsets[n], stop colours[4n]
+ *   radial paint   : cx, cy, rx, ry, offsets[n], stop colours[4n]
+ *   shape_to_canvas: 9 floats per group, row-major. */
+typedef struct dvg_scene_desc {
+    int32_t canvas_width, canvas_height;
+    int32_t num_shapes;
+    const int32_t *shape_type;
+    const float *stroke_width;
+    const int32_t *shape_param_offset;   /* -1 for paths */
+    const float *shape_params;
+    int32_t num_shape_params;
+    const int32_t *path_ctrl_offset;     /* per shape, offset into num_control_points */
+    const int32_t *path_num_base_points; /* per shape, number of segments */
+    const int32_t *path_point_offset;    /* per shape, offset (in points) into points */
+    const int32_t *path_num_points;
+    const uint8_t *path_has_thickness;   /* thickness[] indexed like points */
+    const uint8_t *path_is_closed;
+    const uint8_t *path_use_distance_approx;
+    const int32_t *num_control_points;
+    int32_t num_ctrl_total;
+    const float *points;
+    const float *thickness;              /* may be NULL when no path has thickness */
+    int32_t num_points_total;
+    int32_t num_groups;
+    const int32_t *group_shape_offset;
+    const int32_t *group_num_shapes;
+    const int32_t *group_shape_ids;
+    int32_t num_group_shape_ids;
+    const int32_t *fill_type;            /* DVG_COLOR_* */
+    const int32_t *fill_offset;
+    const int32_t *fill_num_stops;
+    const int32_t *stroke_type;
+    const int32_t *stroke_offset;
+    const int32_t *stroke_num_stops;
+    const float *color_params;
+    int32_t num_color_params;
+    const uint8_t *use_even_odd_rule;
+    const float *shape_to_canvas;
+    int32_t filter_type;
+    float filter_radius;
+} dvg_scene_desc;
+
+/* Arguments of render() (reference diffvg.cpp:1477-1492).  Image pointers are
+ * DEVICE pointers on the scene's GPU (NULL = absent); layouts are the reference's:
+ * images [H][W][4], sdf [H][W][1] or [N][1], d_translation [H][W][2], eval_positions
+ * [N][2].  `stream` is a hipStream_t (NULL = the null stream). */
+typedef struct dvg_render_args {
+    const float *background_image;
+    float *render_image;
+    float *render_sdf;
+    int32_t width, height, num_samples_x, num_samples_y;
+    uint64_t seed;
+    float *d_background_image;
+    const float *d_render_image;
+    const float *d_render_sdf;
+    float *d_translation;
+    int32_t use_prefiltering;
+    const float *eval_positions;
+    int32_t num_eval_positions;
+    void *stream;
+} dvg_render_args;
+
+/* Gradient destinations, HOST pointers, in the layout of dvg_scene_desc (each may be
+ * NULL to skip).  Replaces Scene.get_d_shape / get_d_shape_group /
+ * get_d_filter_radius (reference scene.cpp:1025-1035, diffvg.cpp:1786-1788). */
+typedef struct dvg_grads {
+    float *d_shape_params;     /* [num_shape_params] */
+    float *d_points;           /* [2 * num_points_total] */
+    float *d_thickness;        /* [num_points_total] */
+    float *d_stroke_width;     /* [num_shapes] */
+    float *d_color_params;     /* [num_color_params] */
+    float *d_shape_to_canvas;  /* [9 * num_groups] */
+    float *d_filter_radius;    /* [1] */
+} dvg_grads;
+
+typedef struct dvg_scene dvg_scene;
+
+/* Scene(...) (reference scene.cpp:919-998, bound at diffvg.cpp:1777-1789): copies the
+ * description to the device of `gpu_index` (-1 = current device), derives boundary
+ * lengths, sampling CDFs, boxes and the three BVH levels, and zeroes the gradients. */
+int dvg_scene_create(const dvg_scene_desc *desc, int gpu_index, dvg_scene **out);
+int dvg_scene_destroy(dvg_scene *scene);
+
+/* render(...) (reference diffvg.cpp:1477-1649).  Forward when render_image/render_sdf
+ * are set; backward when d_render_image/d_render_sdf are set, accumulating into the
+ * scene's gradient buffers.  Asynchronous on args->stream in DVG_MODE_TILED;
+ * synchronous in DVG_MODE_PLAIN, like the reference. */
+int dvg_render(dvg_scene *scene, const dvg_render_args *args);
+
+/* Copies the accumulated gradients to host memory (synchronises the scene's last
+ * stream first). */
+int dvg_scene_get_grads(dvg_scene *scene, dvg_grads *out);
+/* Clears the gradient buffers (stream-ordered on `stream`). */
+int dvg_scene_zero_grads(dvg_scene *scene, void *stream);
+/* Selects DVG_MODE_TILED (default) or DVG_MODE_PLAIN for later renders. */
+int dvg_scene_set_mode(dvg_scene *scene, int mode);
+
+/* Last error message of the calling thread ("" when none). */
+const char *dvg_last_error(void);
+int dvg_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
