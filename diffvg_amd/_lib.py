@@ -77,6 +77,12 @@ def lib():
     L.dvg_scene_get_grads.argtypes = [ctypes.c_void_p, ctypes.POINTER(Grads)]
     L.dvg_scene_zero_grads.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.dvg_scene_set_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dvg_scene_grad_layout.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64 * 7),
+                                        ctypes.POINTER(ctypes.c_int64)]
+    L.dvg_scene_copy_grads.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.dvg_profile_enable.argtypes = [ctypes.c_int]
+    L.dvg_profile_read.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_int), ctypes.c_int]
     L.dvg_last_error.restype = ctypes.c_char_p
     L.dvg_abi_version.restype = ctypes.c_int
     _lib = L
@@ -135,6 +141,23 @@ class DeviceScene:
 
     def zero_grads(self, stream=None):
         check(self._lib.dvg_scene_zero_grads(self.handle, stream))
+
+    GRAD_KINDS = ('shape_params', 'points', 'thickness', 'stroke_width', 'color_params', 'shape_to_canvas',
+                  'filter_radius')
+
+    def grad_layout(self):
+        offs = (ctypes.c_int64 * 7)()
+        total = ctypes.c_int64()
+        check(self._lib.dvg_scene_grad_layout(self.handle, ctypes.byref(offs), ctypes.byref(total)))
+        return dict(zip(self.GRAD_KINDS, [int(o) for o in offs])), int(total.value)
+
+    def grads_device(self, torch_device, stream=None):
+        """Gradient block as one device tensor plus the offsets of its arrays."""
+        import torch
+        layout, total = self.grad_layout()
+        block = torch.empty(total, dtype=torch.float32, device=torch_device)
+        check(self._lib.dvg_scene_copy_grads(self.handle, ctypes.c_void_p(block.data_ptr()), stream))
+        return block, layout
 
     def grads(self):
         f = self.flat

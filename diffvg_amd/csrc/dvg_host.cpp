@@ -13,6 +13,8 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -21,6 +23,41 @@
 namespace dvg {
 
 static thread_local std::string t_error;
+
+// ---- device memory cache (size classes of powers of two, per device)
+namespace {
+std::mutex g_cache_mu;
+std::map<std::pair<int, size_t>, std::vector<void *>> g_cache;
+size_t size_class(size_t bytes) {
+    size_t c = 4096;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+}  // namespace
+
+void *cache_alloc(int device, size_t bytes, size_t *granted) {
+    size_t c = size_class(bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto &v = g_cache[{device, c}];
+        if (!v.empty()) {
+            void *p = v.back();
+            v.pop_back();
+            *granted = c;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, c) != hipSuccess) return nullptr;
+    *granted = c;
+    return p;
+}
+
+void cache_free(int device, void *p, size_t bytes) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_cache[{device, size_class(bytes)}].push_back(p);
+}
 
 int set_error(const std::string &msg) {
     t_error = msg;
@@ -478,7 +515,8 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
                o_gb = P.add(g_box), o_gmr = P.add(g_maxr), o_top = P.add(topd), o_len = P.add(shape_length),
                o_cdf = P.add(cdf), o_pmf = P.add(pmf), o_sid = P.add(sid), o_gid = P.add(gid);
         sc.block_bytes = P.host.size();
-        HIP_OK(hipMalloc(&sc.block, sc.block_bytes));
+        sc.block = cache_alloc(sc.device, sc.block_bytes, &sc.block_granted);
+        if (!sc.block) return set_error("out of device memory for the scene");
         HIP_OK(hipMemcpy(sc.block, P.host.data(), sc.block_bytes, hipMemcpyHostToDevice));
         uint8_t *B = (uint8_t *)sc.block;
         DevScene &D = sc.dev;
@@ -546,7 +584,8 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     sc.g_xform = sc.g_color + al(d.num_color_params);
     sc.g_filter = sc.g_xform + al(9 * (size_t)ng);
     sc.grad_floats = sc.g_filter + 64;
-    HIP_OK(hipMalloc(&sc.grad, sc.grad_floats * sizeof(float)));
+    sc.grad = (float *)cache_alloc(sc.device, sc.grad_floats * sizeof(float), &sc.grad_granted);
+    if (!sc.grad) return set_error("out of device memory for the gradients");
     HIP_OK(hipMemset(sc.grad, 0, sc.grad_floats * sizeof(float)));
     D.d_params = sc.grad + sc.g_params;
     D.d_points = sc.grad + sc.g_points;
@@ -555,7 +594,9 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     D.d_color = sc.grad + sc.g_color;
     D.d_xform = sc.grad + sc.g_xform;
     D.d_filter_radius = sc.grad + sc.g_filter;
-    HIP_OK(hipMalloc(&sc.err, sizeof(int)));
+    size_t eg = 0;
+    sc.err = (int *)cache_alloc(sc.device, sizeof(int), &eg);
+    if (!sc.err) return set_error("out of device memory");
     HIP_OK(hipMemset(sc.err, 0, sizeof(int)));
     return 0;
 }
@@ -563,10 +604,9 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
 void free_scene(SceneHost &sc) {
     int prev = -1;
     if (hipGetDevice(&prev) == hipSuccess) (void)hipSetDevice(sc.device);
-    sc.ws.release();
-    if (sc.block) (void)hipFree(sc.block);
-    if (sc.grad) (void)hipFree(sc.grad);
-    if (sc.err) (void)hipFree(sc.err);
+    cache_free(sc.device, sc.block, sc.block_granted);
+    cache_free(sc.device, sc.grad, sc.grad_granted);
+    cache_free(sc.device, sc.err, sizeof(int));
     sc.block = nullptr;
     sc.grad = nullptr;
     sc.err = nullptr;
@@ -606,7 +646,9 @@ int dvg_scene_create(const dvg_scene_desc *desc, int gpu_index, dvg_scene **out)
 
 int dvg_scene_destroy(dvg_scene *scene) {
     if (!scene) return 0;
-    if (scene->h.last_stream) (void)hipStreamSynchronize(scene->h.last_stream);
+    // memory goes back to the cache, so every kernel that may still read it must be done
+    (void)hipSetDevice(scene->h.device);
+    (void)hipStreamSynchronize(scene->h.last_stream);
     free_scene(scene->h);
     delete scene;
     return 0;
@@ -643,6 +685,29 @@ int dvg_scene_zero_grads(dvg_scene *scene, void *stream) {
     SceneHost &h = scene->h;
     hipError_t e = hipMemsetAsync(h.grad, 0, h.grad_floats * sizeof(float), (hipStream_t)stream);
     if (e != hipSuccess) return set_error(std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int dvg_scene_grad_layout(dvg_scene *scene, int64_t *offsets, int64_t *total_floats) {
+    if (!scene) return set_error("null scene");
+    SceneHost &h = scene->h;
+    if (offsets) {
+        offsets[0] = (int64_t)h.g_params;
+        offsets[1] = (int64_t)h.g_points;
+        offsets[2] = (int64_t)h.g_thick;
+        offsets[3] = (int64_t)h.g_sw;
+        offsets[4] = (int64_t)h.g_color;
+        offsets[5] = (int64_t)h.g_xform;
+        offsets[6] = (int64_t)h.g_filter;
+    }
+    if (total_floats) *total_floats = (int64_t)h.grad_floats;
+    return 0;
+}
+
+int dvg_scene_copy_grads(dvg_scene *scene, float *dst, void *stream) {
+    if (!scene || !dst) return set_error("null argument");
+    SceneHost &h = scene->h;
+    HIP_OK(hipMemcpyAsync(dst, h.grad, h.grad_floats * sizeof(float), hipMemcpyDeviceToDevice, (hipStream_t)stream));
     return 0;
 }
 

@@ -29,20 +29,31 @@ struct Workspace {
     void release();
 };
 
+// Device memory cache: blocks freed by destroyed scenes are kept per device and size
+// class and handed to later scenes, so constructing a scene per forward call (as the
+// autograd front end does) does not pay hipMalloc/hipFree.
+void *cache_alloc(int device, size_t bytes, size_t *granted);
+void cache_free(int device, void *p, size_t bytes);
+
+// One render workspace per device, shared by all scenes.  A render holds it from the
+// first to the last kernel it enqueues; its stream first waits for the event recorded
+// by the previous user, so renders on different streams are ordered correctly.
+Workspace &acquire_workspace(int device, hipStream_t st);
+void release_workspace(int device, hipStream_t st);
+
 struct SceneHost {
     int device = 0;
     int mode = DVG_MODE_TILED;
     int cw = 0, ch = 0, num_shapes = 0, num_groups = 0;
     int num_shape_params = 0, num_points_total = 0, num_color_params = 0;
     void *block = nullptr;        // scene arrays
-    size_t block_bytes = 0;
+    size_t block_bytes = 0, block_granted = 0;
     float *grad = nullptr;        // gradient arrays
-    size_t grad_floats = 0;
+    size_t grad_floats = 0, grad_granted = 0;
     size_t g_params = 0, g_points = 0, g_thick = 0, g_sw = 0, g_color = 0, g_xform = 0, g_filter = 0;
     int *err = nullptr;
     hipStream_t last_stream = nullptr;
     DevScene dev{};
-    Workspace ws;
 };
 
 int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc);

@@ -18,6 +18,12 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <cstring>
+#include <map>
+#include <string>
+#include <mutex>
+#include <vector>
+
 #include "dvg_internal.h"
 #include "dvg_shade.h"
 
@@ -789,9 +795,44 @@ __global__ __launch_bounds__(64) void kp_edge(DevScene S, Geom G, const EdgeSamp
     }
 }
 
+
+// ------------------------------------------------------------------ kernel profiler
+// When enabled, every launch is bracketed by HIP events recorded on the stream it runs
+// on; dvg_profile_read() waits for them and reports the summed durations per kernel.
+namespace prof {
+struct Rec {
+    const char *name;
+    hipEvent_t a, b;
+};
+static bool enabled = false;
+static std::vector<Rec> recs;
+static std::mutex mu;
+struct Scope {
+    hipStream_t st;
+    Rec r{};
+    bool on;
+    Scope(hipStream_t s, const char *name) : st(s), on(enabled) {
+        if (!on) return;
+        r.name = name;
+        (void)hipEventCreate(&r.a);
+        (void)hipEventCreate(&r.b);
+        (void)hipEventRecord(r.a, st);
+    }
+    ~Scope() {
+        if (!on) return;
+        (void)hipEventRecord(r.b, st);
+        std::lock_guard<std::mutex> lk(mu);
+        recs.push_back(r);
+    }
+};
+}  // namespace prof
+#define DVG_PROF(st, name) prof::Scope prof_scope_##__LINE__(st, name)
+
 // ------------------------------------------------------------------ launch sequences
 int Workspace::reserve(size_t npix, size_t nsamp, size_t ntiles) {
     if (npix <= npix_cap && nsamp <= nsamp_cap && ntiles <= ntiles_cap) return 0;
+    // growing: earlier renders may still use the old buffers on any stream
+    if (weight) (void)hipDeviceSynchronize();
     release();
     auto al = [](void **p, size_t bytes) { return hipMalloc(p, bytes < 256 ? 256 : bytes); };
     if (al((void **)&weight, npix * sizeof(float)) != hipSuccess ||
@@ -809,6 +850,38 @@ int Workspace::reserve(size_t npix, size_t nsamp, size_t ntiles) {
     nsamp_cap = nsamp;
     ntiles_cap = ntiles;
     return 0;
+}
+
+namespace {
+struct WsSlot {
+    std::mutex mu;
+    Workspace ws;
+    hipEvent_t last = nullptr;
+    bool used = false;
+};
+WsSlot &ws_slot(int device) {
+    static std::mutex m;
+    static std::map<int, WsSlot *> slots;
+    std::lock_guard<std::mutex> lk(m);
+    WsSlot *&s = slots[device];
+    if (!s) s = new WsSlot();
+    return *s;
+}
+}  // namespace
+
+Workspace &acquire_workspace(int device, hipStream_t st) {
+    WsSlot &s = ws_slot(device);
+    s.mu.lock();
+    if (s.used && s.last) (void)hipStreamWaitEvent(st, s.last, 0);
+    return s.ws;
+}
+
+void release_workspace(int device, hipStream_t st) {
+    WsSlot &s = ws_slot(device);
+    if (!s.last) (void)hipEventCreateWithFlags(&s.last, hipEventDisableTiming);
+    (void)hipEventRecord(s.last, st);
+    s.used = true;
+    s.mu.unlock();
 }
 
 void Workspace::release() {
@@ -882,38 +955,68 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
     const bool sdf_out = a.render_sdf || a.d_render_sdf;
     int ntiles = G.tiles_x * G.tiles_y;
     if (color_out) {
-        if (int e = sc.ws.reserve(npix, nsamp, ntiles)) return e;
-        k_weight<<<(unsigned)((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(S, G, sc.ws.weight);
+        Workspace &ws = acquire_workspace(sc.device, st);
+        struct Release {
+            int d;
+            hipStream_t s;
+            ~Release() { release_workspace(d, s); }
+        } release_guard{sc.device, st};
+        if (int e = ws.reserve(npix, nsamp, ntiles)) return e;
+        {
+            DVG_PROF(st, "k_weight");
+            k_weight<<<(unsigned)((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(S, G, ws.weight);
+        }
         if (a.render_image) {
-            if (pre)
-                k_color<1><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, sc.ws.colors, sc.err);
-            else
-                k_color<0><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, sc.ws.colors, sc.err);
-            k_resolve<<<(unsigned)((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(S, G, pre, sc.ws.weight,
-                                                                                  sc.ws.colors, a.render_image);
+            {
+                DVG_PROF(st, "k_color");
+                if (pre)
+                    k_color<1><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.colors, sc.err);
+                else
+                    k_color<0><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.colors, sc.err);
+            }
+            {
+                DVG_PROF(st, "k_resolve");
+                k_resolve<<<(unsigned)((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(S, G, pre, ws.weight,
+                                                                                      ws.colors, a.render_image);
+            }
         }
         if (a.d_render_image) {
-            if (pre)
-                k_color_bwd<1><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, sc.ws.weight, a.d_render_image,
-                                                          a.d_background_image, a.d_translation, sc.err);
-            else
-                k_color_bwd<0><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, sc.ws.weight, a.d_render_image,
-                                                          a.d_background_image, a.d_translation, sc.err);
+            {
+                DVG_PROF(st, "k_color_bwd");
+                if (pre)
+                    k_color_bwd<1><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.weight,
+                                                              a.d_render_image, a.d_background_image,
+                                                              a.d_translation, sc.err);
+                else
+                    k_color_bwd<0><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.weight,
+                                                              a.d_render_image, a.d_background_image,
+                                                              a.d_translation, sc.err);
+            }
             if (!pre) {
-                DVG_CHECK(hipMemsetAsync(sc.ws.tile_count, 0, sizeof(int) * ntiles, st));
-                k_edge_sample<<<(unsigned)((nsamp + kBlock - 1) / kBlock), kBlock, 0, st>>>(
-                    S, G, (int)nsamp, sc.ws.edges, sc.ws.edge_tile, sc.ws.tile_count);
-                k_scan<<<1, 1024, 0, st>>>(sc.ws.tile_count, sc.ws.tile_off, sc.ws.tile_cursor, ntiles);
-                k_edge_bin<<<(unsigned)((nsamp + kBlock - 1) / kBlock), kBlock, 0, st>>>(
-                    (int)nsamp, sc.ws.edge_tile, sc.ws.tile_off, sc.ws.tile_cursor, sc.ws.edge_sorted);
-                k_edge<<<ntiles, kBlock, 0, st>>>(S, G, sc.ws.edges, sc.ws.tile_off, sc.ws.tile_count,
-                                                  sc.ws.edge_sorted, a.background_image, sc.ws.weight,
-                                                  a.d_render_image, a.d_translation);
+                DVG_CHECK(hipMemsetAsync(ws.tile_count, 0, sizeof(int) * ntiles, st));
+                {
+                    DVG_PROF(st, "k_edge_sample");
+                    k_edge_sample<<<(unsigned)((nsamp + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+                        S, G, (int)nsamp, ws.edges, ws.edge_tile, ws.tile_count);
+                }
+                {
+                    DVG_PROF(st, "k_edge_bin");
+                    k_scan<<<1, 1024, 0, st>>>(ws.tile_count, ws.tile_off, ws.tile_cursor, ntiles);
+                    k_edge_bin<<<(unsigned)((nsamp + kBlock - 1) / kBlock), kBlock, 0, st>>>(
+                        (int)nsamp, ws.edge_tile, ws.tile_off, ws.tile_cursor, ws.edge_sorted);
+                }
+                {
+                    DVG_PROF(st, "k_edge");
+                    k_edge<<<ntiles, kBlock, 0, st>>>(S, G, ws.edges, ws.tile_off, ws.tile_count,
+                                                      ws.edge_sorted, a.background_image, ws.weight,
+                                                      a.d_render_image, a.d_translation);
+                }
             }
         }
     }
     if (sdf_out) {
         int count = eval_mode ? a.num_eval_positions : (int)nsamp;
+        DVG_PROF(st, "k_sdf");
         k_sdf<<<(count + kBlock - 1) / kBlock, kBlock, 0, st>>>(S, G, eval_mode ? a.eval_positions : nullptr,
                                                                  a.num_eval_positions, a.render_sdf, a.d_render_sdf,
                                                                  a.d_translation);
@@ -948,9 +1051,11 @@ int render_plain(SceneHost &sc, const dvg_render_args &a) {
     const bool color_out = a.render_image || a.d_render_image;
     const bool sdf_out = a.render_sdf || a.d_render_sdf;
     if ((color_out || sdf_out) && weight) {
+        DVG_PROF(st, "kp_weight");
         kp_weight<<<(unsigned)((nsamp + 63) / 64), 64, 0, st>>>(S, G, weight);
     }
     if (color_out) {
+        DVG_PROF(st, "kp_render");
         if (pre)
             kp_render<1><<<(unsigned)((nsamp + 63) / 64), 64, 0, st>>>(S, G, a.background_image, a.render_image,
                                                                        weight, a.d_render_image, a.d_background_image,
@@ -980,8 +1085,11 @@ int render_plain(SceneHost &sc, const dvg_render_args &a) {
         DVG_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys2, ids, ids2, (int)nsamp, 0, 32, st));
         DVG_CHECK(hipMalloc(&tmp, tmp_bytes));
         DVG_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys2, ids, ids2, (int)nsamp, 0, 32, st));
-        kp_edge<<<(unsigned)((nsamp + 63) / 64), 64, 0, st>>>(S, G, edges, ids2, a.background_image, weight,
-                                                              a.d_render_image, a.d_translation, sc.err);
+        {
+            DVG_PROF(st, "kp_edge");
+            kp_edge<<<(unsigned)((nsamp + 63) / 64), 64, 0, st>>>(S, G, edges, ids2, a.background_image, weight,
+                                                                  a.d_render_image, a.d_translation, sc.err);
+        }
         DVG_CHECK(hipStreamSynchronize(st));
         DVG_CHECK(hipFree(tmp));
         DVG_CHECK(hipFree(edges));
@@ -997,3 +1105,46 @@ int render_plain(SceneHost &sc, const dvg_render_args &a) {
 }
 
 }  // namespace dvg
+
+extern "C" {
+
+int dvg_profile_enable(int on) {
+    dvg::prof::enabled = on != 0;
+    return 0;
+}
+
+// Waits for the recorded events and writes per-kernel totals: names as a
+// newline-separated list into `names` (capacity `cap` bytes), total milliseconds and
+// launch counts into ms[] / counts[] (capacity `max_kernels`).  Clears the records.
+int dvg_profile_read(char *names, int cap, double *ms, int *counts, int max_kernels) {
+    std::lock_guard<std::mutex> lk(dvg::prof::mu);
+    std::map<std::string, std::pair<double, int>> acc;
+    for (auto &r : dvg::prof::recs) {
+        float t = 0.f;
+        (void)hipEventSynchronize(r.b);
+        (void)hipEventElapsedTime(&t, r.a, r.b);
+        auto &e = acc[r.name];
+        e.first += t;
+        e.second += 1;
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    dvg::prof::recs.clear();
+    std::string all;
+    int n = 0;
+    for (auto &kv : acc) {
+        if (n >= max_kernels) break;
+        all += kv.first + "\n";
+        ms[n] = kv.second.first;
+        counts[n] = kv.second.second;
+        n++;
+    }
+    if (names && cap > 0) {
+        size_t k = std::min((size_t)cap - 1, all.size());
+        std::memcpy(names, all.data(), k);
+        names[k] = 0;
+    }
+    return n;
+}
+
+}  // extern "C"

@@ -66,16 +66,51 @@ class _Flat:
     to gradient slices."""
 
 
-def _gather_float(tensors):
-    """Concatenates float tensors (any devices) into one host float32 array."""
+def _cat_kind(tensors):
+    """Concatenates same-kind tensors into one flat float32 tensor with few torch calls
+    (a per-tensor reshape of an autograd tensor costs microseconds, and scenes hold
+    thousands of them)."""
     if not tensors:
-        return np.zeros(0, np.float32)
-    flat = [t.detach().reshape(-1) for t in tensors]
-    devs = {t.device for t in flat}
-    if len(devs) == 1:
-        cat = torch.cat([t.to(torch.float32) for t in flat])
-        return cat.cpu().numpy()
-    return np.concatenate([t.to(torch.float32).cpu().numpy() for t in flat])
+        return None
+    ranks = {t.dim() for t in tensors}
+    if ranks == {0}:
+        out = torch.stack(tensors)
+    elif ranks == {1}:
+        out = torch.cat(tensors)
+    elif len(ranks) == 1 and len({tuple(t.shape[1:]) for t in tensors}) == 1:
+        out = torch.cat(tensors).reshape(-1)
+    else:
+        out = torch.cat([t.reshape(-1) for t in tensors])
+    return out if out.dtype == torch.float32 else out.to(torch.float32)
+
+
+def _gather_kinds(kinds):
+    """List of tensor lists -> list of host float32 arrays (one device-to-host copy when
+    all tensors share a device)."""
+    with torch.no_grad():
+        parts = [_cat_kind(k) for k in kinds]
+        present = [p for p in parts if p is not None]
+        if not present:
+            return [np.zeros(0, np.float32) for _ in kinds]
+        if len({p.device for p in present}) == 1:
+            host = torch.cat(present).cpu().numpy()
+        else:
+            host = np.concatenate([p.cpu().numpy() for p in present])
+    out = []
+    k = 0
+    for p in parts:
+        n = 0 if p is None else int(p.numel())
+        out.append(host[k:k + n])
+        k += n
+    return out
+
+
+def _cat_int(tensors):
+    if not tensors:
+        return np.zeros(0, np.int32)
+    with torch.no_grad():
+        return torch.cat([t.reshape(-1) for t in tensors] if any(t.dim() != 1 for t in tensors)
+                         else tensors).to(torch.int32).cpu().numpy()
 
 
 def flatten_args(args):
@@ -217,27 +252,24 @@ def flatten_args(args):
     grad_map.append((i + 1, 'filter_radius', 0, 1, tuple(filter_radius.shape)))
     f.num_args = i + 2
 
-    # one concatenation + host copy for all float data
-    groups = [src_params, src_points, [t for _, t in src_thick], src_sw, src_color, src_xform, [filter_radius]]
-    sizes = [sum(int(t.numel()) for t in grp) for grp in groups]
-    host = _gather_float([t for grp in groups for t in grp])
-    parts = np.split(host, np.cumsum(sizes)[:-1])
-    params, points, thick_vals, sw, color, xform, frad = parts
+    # one concatenation per kind and one host copy for all float data
+    params, points, thick_vals, sw, color, xform, frad = _gather_kinds(
+        [src_params, src_points, [t for _, t in src_thick], src_sw, src_color, src_xform, [filter_radius]])
+    # the reference asserts finite points and thicknesses (render_pytorch.py:63, 68)
+    assert np.isfinite(points).all() and np.isfinite(thick_vals).all(), 'non-finite path points or thickness'
     thickness = np.zeros(n_points, np.float32)
     k = 0
     for off, t in src_thick:
         n = int(t.numel())
         thickness[off:off + n] = thick_vals[k:k + n]
         k += n
-    sw_arr = np.zeros(ns, np.float32)
-    k = 0
-    for s in range(ns):
-        sw_arr[s] = sw[k] if not (shape_type[s] == 2 and has_thick[s]) else 0.0
-        k += int(src_sw[s].numel())
-    ncp = (torch.cat([c.reshape(-1).to(torch.int32).cpu() for c in ncp_list]).numpy()
-           if ncp_list else np.zeros(0, np.int32))
-    gids = (torch.cat([c.reshape(-1).to(torch.int32).cpu() for c in ids_list]).numpy()
-            if ids_list else np.zeros(0, np.int32))
+    # stroke widths are scalars in practice; the first element of each is used
+    sw_sizes = np.fromiter((int(t.numel()) for t in src_sw), np.int64, count=ns)
+    sw_first = np.concatenate([[0], np.cumsum(sw_sizes)[:-1]]).astype(np.int64)
+    sw_arr = np.ascontiguousarray(sw[sw_first] if ns else np.zeros(0), np.float32)
+    sw_arr[(shape_type == 2) & (has_thick != 0)] = 0.0
+    ncp = _cat_int(ncp_list)
+    gids = _cat_int(ids_list)
     f.desc = dict(
         canvas_width=int(canvas_width), canvas_height=int(canvas_height), num_shapes=ns,
         shape_type=shape_type, stroke_width=sw_arr, shape_param_offset=param_off,
@@ -272,19 +304,19 @@ def ctypes_stream(s):
     return s.cuda_stream
 
 
-def _grad_tensors(flat, grads, args, num_args):
-    """Gradient list in argument order (render_pytorch.py:713-868)."""
+def _grad_tensors(flat, block, layout, args, num_args):
+    """Gradient list in argument order (render_pytorch.py:713-868): views of one
+    gradient block, moved once to each input device that needs it."""
     d = [None] * num_args
-    dev_cache = {}
+    per_device = {block.device: block}
     for (ai, kind, off, cnt, shape) in flat.grad_map:
         src = args[ai]
-        arr = grads[kind][off:off + cnt]
-        key = (kind, src.device)
-        t = torch.from_numpy(np.ascontiguousarray(arr)).reshape(shape)
-        if src.device.type != 'cpu':
-            t = t.to(src.device, non_blocking=False)
-        d[ai] = t.to(src.dtype) if src.dtype != torch.float32 else t
-        dev_cache[key] = True
+        blk = per_device.get(src.device)
+        if blk is None:
+            blk = per_device[src.device] = block.to(src.device)
+        base = layout[kind] + off
+        t = blk[base:base + cnt].view(shape)
+        d[ai] = t if src.dtype == torch.float32 else t.to(src.dtype)
     return d
 
 
@@ -313,10 +345,8 @@ class RenderFunction(torch.autograd.Function):
                 args += [diffvg.ShapeType.ellipse, shape.radius, shape.center]
             elif isinstance(shape, Path):
                 assert shape.points.shape[1] == 2
-                assert torch.isfinite(shape.points).all()
                 args += [diffvg.ShapeType.path, shape.num_control_points.to(torch.int32), shape.points]
                 if len(shape.stroke_width.shape) > 0 and shape.stroke_width.shape[0] > 1:
-                    assert torch.isfinite(shape.stroke_width).all()
                     use_thickness = True
                     args.append(shape.stroke_width)
                 else:
@@ -471,11 +501,10 @@ class RenderFunction(torch.autograd.Function):
                      use_prefiltering=flat.use_prefiltering, eval_positions=ctx.eval_positions,
                      num_eval_positions=0 if ctx.eval_positions is None else ctx.eval_positions.shape[0],
                      stream=_stream())
-        grads = scene.grads()
+        block, layout = scene.grads_device(grad_img.device, _stream())
         if print_timing:
+            torch.cuda.synchronize()
             print('Backward pass, time: %.5f s' % (time.time() - start))
-        d_scene = _grad_tensors(flat, grads, ctx.args, flat.num_args)
-        for t in d_scene:
-            if t is not None:
-                assert torch.isfinite(t).all()
+        assert bool(torch.isfinite(block).all())
+        d_scene = _grad_tensors(flat, block, layout, ctx.args, flat.num_args)
         return tuple([None, None, None, None, None, d_background_image] + d_scene)

@@ -132,10 +132,24 @@ int dvg_render(dvg_scene *scene, const dvg_render_args *args);
 /* Copies the accumulated gradients to host memory (synchronises the scene's last
  * stream first). */
 int dvg_scene_get_grads(dvg_scene *scene, dvg_grads *out);
+/* Layout of the scene's gradient block on the device: float offsets of the seven
+ * arrays of dvg_grads (in that order) and the block size in floats. */
+int dvg_scene_grad_layout(dvg_scene *scene, int64_t *offsets, int64_t *total_floats);
+/* Copies the gradient block to device memory `dst` (total_floats floats), stream-ordered
+ * on `stream`: the device-resident counterpart of dvg_scene_get_grads. */
+int dvg_scene_copy_grads(dvg_scene *scene, float *dst, void *stream);
 /* Clears the gradient buffers (stream-ordered on `stream`). */
 int dvg_scene_zero_grads(dvg_scene *scene, void *stream);
 /* Selects DVG_MODE_TILED (default) or DVG_MODE_PLAIN for later renders. */
 int dvg_scene_set_mode(dvg_scene *scene, int mode);
+
+/* Kernel timing: when enabled, each kernel launch is bracketed by HIP events recorded
+ * on its stream.  dvg_profile_read waits for the events and returns the number of
+ * kernels, their names (newline separated) and summed milliseconds and launch counts,
+ * then clears the records.  No reference counterpart (the reference only has
+ * wall-clock print_timing, render_pytorch.py:8-12). */
+int dvg_profile_enable(int on);
+int dvg_profile_read(char *names, int cap, double *ms, int *counts, int max_kernels);
 
 /* Last error message of the calling thread ("" when none). */
 const char *dvg_last_error(void);

@@ -116,7 +116,12 @@ const char *orc_last_error(void);
 void orc_set_num_threads(int n);
 int orc_get_num_threads(void);
 void orc_set_counting(int on);
-void orc_get_counters(orc_counters *out);
+/* Counters of the last render; phase 0 = weight and render kernels, 1 = boundary
+ * sampling and edge kernel. */
+void orc_get_counters(int phase, orc_counters *out);
+/* Evaluate only every stride-th sample in each pass (for cost counting on a sample;
+ * the images and gradients are then incomplete). */
+void orc_set_sample_stride(int stride);
 
 /* BVH inspection: level 0 = top level (groups), 1 = group `index`, 2 = path `index`.
  * Writes up to max_nodes nodes as (child0, child1, minx, miny, maxx, maxy, max_radius);

@@ -495,6 +495,7 @@ inline bool closest_point_path(const Scene &S, int s, V2 pt, float max_radius, P
             V2 cp{0, 0};
             float t_root = 0.f;
             bump(kCntClosest);
+            bump(k == 0 ? kCntLine : (k == 1 ? kCntQuad : kCntCubicDist));
             if (k == 0) {
                 V2 p0 = p[0], p1 = p[1];
                 float t = dot(pt - p0, p1 - p0) / dot(p1 - p0, p1 - p0);

@@ -27,7 +27,9 @@ thread_local Counters *g_tc = nullptr;
 static thread_local std::string g_err;
 static int g_num_threads = 0;
 static bool g_counting = false;
-static Counters g_counters;
+static Counters g_counters[2];  // phase 0: weight + render kernels, 1: boundary sampling + edges
+static int g_phase = 0;
+static int g_stride = 1;        // evaluate only every g_stride-th sample (cost counting)
 static std::mutex g_counter_mu;
 
 static bool g_multi = false;  // atomics needed only when several threads run
@@ -1112,11 +1114,12 @@ static void parallel_for(F f, int64_t count) {
             int64_t task = next.fetch_add(1);
             if (task >= ntasks) break;
             int64_t beg = task * chunk, end = std::min(beg + chunk, count);
-            for (int64_t i = beg; i < end; i++) f((int)i);
+            for (int64_t i = beg; i < end; i++)
+                if (g_stride <= 1 || i % g_stride == 0) f((int)i);
         }
         if (g_counting) {
             std::lock_guard<std::mutex> lk(g_counter_mu);
-            for (int i = 0; i < 12; i++) g_counters.c[i] += local.c[i];
+            for (int i = 0; i < 12; i++) g_counters[g_phase].c[i] += local.c[i];
             g_tc = nullptr;
         }
     };
@@ -1145,6 +1148,7 @@ static int render(Scene &S, const orc_render_args &a) {
             std::memcpy(a.render_image, a.background_image, sizeof(float) * 4 * W * H);
         return 0;
     }
+    g_phase = 0;
     if (a.render_image || a.d_render_image || a.render_sdf || a.d_render_sdf) {
         if (!eval_mode) {
             // weight_kernel (diffvg.cpp:1115-1158); use_prefiltering is never set there.
@@ -1247,6 +1251,7 @@ static int render(Scene &S, const orc_render_args &a) {
             num_samples);
     }
     if (!pre && a.d_render_image) {
+        g_phase = 1;
         int64_t num_samples = (int64_t)W * H * nsx * nsy;
         std::vector<BSample> bs(num_samples);
         // sample_boundary_kernel (diffvg.cpp:1325-1386)
@@ -1310,6 +1315,7 @@ static int render(Scene &S, const orc_render_args &a) {
             },
             num_samples);
     }
+    g_phase = 0;
     if (overflow) {
         g_err = "fragment list overflow (more overlapping shapes than the reference supports)";
         return 1;
@@ -1356,7 +1362,7 @@ int orc_render(orc_scene *scene, const orc_render_args *args) {
         return 1;
     }
     t_overflow = false;
-    if (g_counting) g_counters = Counters{};
+    if (g_counting) g_counters[0] = g_counters[1] = Counters{};
     return render(scene->s, *args);
 }
 
@@ -1392,9 +1398,10 @@ int orc_get_num_threads(void) {
     return g_num_threads > 0 ? g_num_threads : (int)std::thread::hardware_concurrency();
 }
 void orc_set_counting(int on) { g_counting = on != 0; }
-void orc_get_counters(orc_counters *out) {
+void orc_set_sample_stride(int stride) { g_stride = stride < 1 ? 1 : stride; }
+void orc_get_counters(int phase, orc_counters *out) {
     int64_t *o = (int64_t *)out;
-    for (int i = 0; i < 12; i++) o[i] = g_counters.c[i];
+    for (int i = 0; i < 12; i++) o[i] = g_counters[phase & 1].c[i];
 }
 
 int orc_scene_bvh(const orc_scene *scene, int level, int index, int32_t *children, float *boxes, float *max_radius,

@@ -259,6 +259,8 @@ def load_oracle():
         lib.orc_morton2d.restype = ctypes.c_uint32
         lib.orc_sample_boundary.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, F32P]
         lib.orc_set_counting.argtypes = [ctypes.c_int]
+        lib.orc_set_sample_stride.argtypes = [ctypes.c_int]
+        lib.orc_get_counters.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int64 * 12)]
         _oracle = lib
     return _oracle
 
