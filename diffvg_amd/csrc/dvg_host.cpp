@@ -231,8 +231,6 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
         return set_error("null group array in the scene description");
     if (d.filter_type < 0 || d.filter_type > 4) return set_error("unknown filter type");
     if (!(d.filter_radius > 0) || !std::isfinite(d.filter_radius)) return set_error("filter radius must be positive");
-    if (gpu_index >= 0) HIP_OK(hipSetDevice(gpu_index));
-    HIP_OK(hipGetDevice(&sc.device));
     sc.cw = d.canvas_width;
     sc.ch = d.canvas_height;
     sc.num_shapes = ns;
@@ -515,6 +513,10 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
                o_gb = P.add(g_box), o_gmr = P.add(g_maxr), o_top = P.add(topd), o_len = P.add(shape_length),
                o_cdf = P.add(cdf), o_pmf = P.add(pmf), o_sid = P.add(sid), o_gid = P.add(gid);
         sc.block_bytes = P.host.size();
+        // validation and host-side construction are done: only now is a device needed
+        if (gpu_index >= 0) HIP_OK(hipSetDevice(gpu_index));
+        HIP_OK(hipGetDevice(&sc.device));
+        sc.device_set = true;
         sc.block = cache_alloc(sc.device, sc.block_bytes, &sc.block_granted);
         if (!sc.block) return set_error("out of device memory for the scene");
         HIP_OK(hipMemcpy(sc.block, P.host.data(), sc.block_bytes, hipMemcpyHostToDevice));
@@ -602,6 +604,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
 }
 
 void free_scene(SceneHost &sc) {
+    if (!sc.device_set) return;
     int prev = -1;
     if (hipGetDevice(&prev) == hipSuccess) (void)hipSetDevice(sc.device);
     cache_free(sc.device, sc.block, sc.block_granted);

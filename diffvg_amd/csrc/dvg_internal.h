@@ -43,6 +43,7 @@ void release_workspace(int device, hipStream_t st);
 
 struct SceneHost {
     int device = 0;
+    bool device_set = false;      // a device was chosen and memory may be held on it
     int mode = DVG_MODE_TILED;
     int cw = 0, ch = 0, num_shapes = 0, num_groups = 0;
     int num_shape_params = 0, num_points_total = 0, num_color_params = 0;

@@ -666,7 +666,8 @@ __global__ __launch_bounds__(64) void kp_weight(DevScene S, Geom G, float *weigh
 
 template <int PRE>
 __global__ __launch_bounds__(64) void kp_render(DevScene S, Geom G, const float *bg, float *img, const float *weight,
-                                                const float *d_img, float *d_bg, float *d_translation, int *err) {
+                                                const float *d_img, float4 *dbg_rec, int *dbg_kind,
+                                                float *d_translation, int *err) {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= G.W * G.H * G.spp) return;
     int sx = idx % G.nsx, sy = (idx / G.nsx) % G.nsy;
@@ -699,10 +700,13 @@ __global__ __launch_bounds__(64) void kp_render(DevScene S, Geom G, const float 
     V4 color = comp.result(has_bg, bgc);
     if (d_img) {
         V4 d_color = gather_d_color(S.filter_type, S.filter_radius, d_img, weight, G.W, G.H, rpt);
+        // d_background: recorded per sample and folded per pixel in sample order by
+        // kp_fold_dbg (a sample without fragments assigns, one with fragments adds)
         if (comp.nf == 0) {
-            if (d_bg && has_bg)
-                *reinterpret_cast<float4 *>(d_bg + 4 * (y * G.W + x)) =
-                    make_float4(d_color.x, d_color.y, d_color.z, d_color.w);
+            if (dbg_rec) {
+                dbg_rec[idx] = make_float4(d_color.x, d_color.y, d_color.z, d_color.w);
+                dbg_kind[idx] = 0;
+            }
         } else {
             V4 d_back;
             V3 fc{color.x, color.y, color.z};
@@ -711,7 +715,10 @@ __global__ __launch_bounds__(64) void kp_render(DevScene S, Geom G, const float 
                                                        d_tr);
             else
                 d_back = composite_adjoint(S, frags, min(comp.nf, kMaxFrags), fc, color.w, d_color, pt, d_tr);
-            if (d_bg && has_bg) aadd4(d_bg + 4 * (y * G.W + x), d_back);
+            if (dbg_rec) {
+                dbg_rec[idx] = make_float4(d_back.x, d_back.y, d_back.z, d_back.w);
+                dbg_kind[idx] = 1;
+            }
         }
     }
     float radius = S.filter_radius;
@@ -733,6 +740,24 @@ __global__ __launch_bounds__(64) void kp_render(DevScene S, Geom G, const float 
             }
         }
     if (overflow) atomicOr(err, overflow);
+}
+
+// Per-pixel fold of the d_background records of its samples, in sample order
+// (diffvg.cpp:650-656, 700-704: assignment without fragments, addition with them).
+__global__ __launch_bounds__(64) void kp_fold_dbg(Geom G, const float4 *dbg_rec, const int *dbg_kind, float *d_bg) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= G.W * G.H) return;
+    float4 *dst = reinterpret_cast<float4 *>(d_bg) + p;
+    float4 acc = *dst;
+    for (int k = 0; k < G.spp; k++) {
+        size_t si = (size_t)p * G.spp + k;
+        float4 v = dbg_rec[si];
+        if (dbg_kind[si] == 0)
+            acc = v;
+        else
+            acc = make_float4(acc.x + v.x, acc.y + v.y, acc.z + v.z, acc.w + v.w);
+    }
+    *dst = acc;
 }
 
 __global__ __launch_bounds__(64) void kp_edge_sample(DevScene S, Geom G, int count, EdgeSample *edges, int *ids,
@@ -935,6 +960,17 @@ static int check_err_flag(SceneHost &sc, hipStream_t st) {
     return 0;
 }
 
+// A scene without shapes (and so without groups): every sample sees only the
+// background, which the box-normalised resolve returns unchanged (transparent black when
+// there is none); the SDF adds 0 (diffvg.cpp:741-743) and no gradient is produced.
+static int render_empty(const dvg_render_args &a, size_t npix, hipStream_t st) {
+    if (a.render_image && a.background_image)
+        DVG_CHECK(hipMemcpyAsync(a.render_image, a.background_image, npix * 16, hipMemcpyDeviceToDevice, st));
+    else if (a.render_image)
+        DVG_CHECK(hipMemsetAsync(a.render_image, 0, npix * 16, st));
+    return 0;
+}
+
 int render_tiled(SceneHost &sc, const dvg_render_args &a) {
     hipStream_t st = (hipStream_t)a.stream;
     Geom G = make_geom(sc, a);
@@ -946,11 +982,7 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
     if (eval_mode && (a.render_image || a.d_render_image))
         return set_error("eval_positions are supported for the SDF output only (as in the reference)");
     DVG_CHECK(hipMemsetAsync(sc.err, 0, sizeof(int), st));
-    if (sc.num_shapes == 0) {
-        if (a.render_image && a.background_image)
-            DVG_CHECK(hipMemcpyAsync(a.render_image, a.background_image, npix * 16, hipMemcpyDeviceToDevice, st));
-        return 0;
-    }
+    if (sc.num_shapes == 0) return render_empty(a, npix, st);
     const bool color_out = a.render_image || a.d_render_image;
     const bool sdf_out = a.render_sdf || a.d_render_sdf;
     int ntiles = G.tiles_x * G.tiles_y;
@@ -1038,8 +1070,7 @@ int render_plain(SceneHost &sc, const dvg_render_args &a) {
         return set_error("eval_positions are supported for the SDF output only (as in the reference)");
     DVG_CHECK(hipMemsetAsync(sc.err, 0, sizeof(int), st));
     if (sc.num_shapes == 0) {
-        if (a.render_image && a.background_image)
-            DVG_CHECK(hipMemcpyAsync(a.render_image, a.background_image, npix * 16, hipMemcpyDeviceToDevice, st));
+        if (int e = render_empty(a, npix, st)) return e;
         DVG_CHECK(hipStreamSynchronize(st));
         return 0;
     }
@@ -1054,16 +1085,28 @@ int render_plain(SceneHost &sc, const dvg_render_args &a) {
         DVG_PROF(st, "kp_weight");
         kp_weight<<<(unsigned)((nsamp + 63) / 64), 64, 0, st>>>(S, G, weight);
     }
+    float4 *dbg_rec = nullptr;
+    int *dbg_kind = nullptr;
+    if (color_out && a.d_render_image && a.d_background_image && a.background_image) {
+        DVG_CHECK(hipMalloc(&dbg_rec, nsamp * sizeof(float4)));
+        DVG_CHECK(hipMalloc(&dbg_kind, nsamp * sizeof(int)));
+    }
     if (color_out) {
         DVG_PROF(st, "kp_render");
         if (pre)
             kp_render<1><<<(unsigned)((nsamp + 63) / 64), 64, 0, st>>>(S, G, a.background_image, a.render_image,
-                                                                       weight, a.d_render_image, a.d_background_image,
+                                                                       weight, a.d_render_image, dbg_rec, dbg_kind,
                                                                        a.d_translation, sc.err);
         else
             kp_render<0><<<(unsigned)((nsamp + 63) / 64), 64, 0, st>>>(S, G, a.background_image, a.render_image,
-                                                                       weight, a.d_render_image, a.d_background_image,
+                                                                       weight, a.d_render_image, dbg_rec, dbg_kind,
                                                                        a.d_translation, sc.err);
+    }
+    if (dbg_rec) {
+        kp_fold_dbg<<<(unsigned)((npix + 63) / 64), 64, 0, st>>>(G, dbg_rec, dbg_kind, a.d_background_image);
+        DVG_CHECK(hipStreamSynchronize(st));
+        DVG_CHECK(hipFree(dbg_rec));
+        DVG_CHECK(hipFree(dbg_kind));
     }
     if (sdf_out) {
         int count = eval_mode ? a.num_eval_positions : (int)nsamp;

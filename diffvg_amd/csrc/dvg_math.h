@@ -13,6 +13,8 @@
 
 #include <cstdint>
 
+#include "dvg_crmath.h"
+
 namespace dvg {
 
 #define DVG_DEV __device__ __forceinline__
@@ -253,6 +255,9 @@ DVG_DEV int solve_cubic_f(float a, float b, float c, float d, float t[3]) {
         return 1;
     }
 }
+// solve_cubic<double> (solve.h:29-64).  CR selects the correctly rounded acos, cos and
+// pow; the default uses the device libm.
+template <bool CR = false>
 DVG_DEV int solve_cubic_d(double a, double b, double c, double d, double t[3]) {
     if (fabs(a) < 1e-6f) {
         if (solve_quadratic_d(b, c, d, &t[0], &t[1])) return 2;
@@ -264,18 +269,28 @@ DVG_DEV int solve_cubic_d(double a, double b, double c, double d, double t[3]) {
     double Q = (b * b - 3 * c) / 9.f;
     double R = (2 * b * b * b - 9 * b * c + 27 * d) / 54.f;
     if (R * R < Q * Q * Q) {
-        double theta = acos(R / sqrt(Q * Q * Q));
+        double x = R / sqrt(Q * Q * Q);
+        double theta = CR ? cr::acos_cr(x) : acos(x);
         double sq = sqrt(Q);
-        t[0] = -2.f * sq * cos(theta / 3.f) - b / 3.f;
-        t[1] = -2.f * sq * cos((theta + 2.f * kPi) / 3.f) - b / 3.f;
-        t[2] = -2.f * sq * cos((theta - 2.f * kPi) / 3.f) - b / 3.f;
+        double a0 = theta / 3.f, a1 = (theta + 2.f * kPi) / 3.f, a2 = (theta - 2.f * kPi) / 3.f;
+        t[0] = -2.f * sq * (CR ? cr::cos_cr(a0) : cos(a0)) - b / 3.f;
+        t[1] = -2.f * sq * (CR ? cr::cos_cr(a1) : cos(a1)) - b / 3.f;
+        t[2] = -2.f * sq * (CR ? cr::cos_cr(a2) : cos(a2)) - b / 3.f;
         return 3;
     } else {
-        double A = R > 0 ? -pow(R + sqrt(R * R - Q * Q * Q), 1. / 3.) : pow(-R + sqrt(R * R - Q * Q * Q), 1. / 3.);
+        double base = R > 0 ? R + sqrt(R * R - Q * Q * Q) : -R + sqrt(R * R - Q * Q * Q);
+        double root = CR ? cr::pow_cr(base, 1. / 3.) : pow(base, 1. / 3.);
+        double A = R > 0 ? -root : root;
         double B = fabs(A) > 1e-6f ? Q / A : 0.0;
         t[0] = (A + B) - b / 3.0;
         return 1;
     }
+}
+
+// Out of line, so that the rarely taken correctly rounded solve does not add to the
+// register budget of the kernels that inline the fast one.
+static __device__ inline int solve_cubic_d_cr(double a, double b, double c, double d, double *t) {
+    return solve_cubic_d<true>(a, b, c, d, t);
 }
 
 // ---- pixel filters (reference filter.h:22-106) and the Gaussian extension ----

@@ -77,8 +77,14 @@ DVG_DEV int segment_winding(int k, const V2 *p, V2 pt) {
     } else {
         V2 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3];
         double t[3];
-        int num_sol = solve_cubic_d((double)(-p0.y + 3 * p1.y - 3 * p2.y + p3.y), (double)(3 * p0.y - 6 * p1.y + 3 * p2.y),
-                                    (double)(-3 * p0.y + 3 * p1.y), (double)(p0.y - pt.y), t);
+        double ca = (double)(-p0.y + 3 * p1.y - 3 * p2.y + p3.y), cb = (double)(3 * p0.y - 6 * p1.y + 3 * p2.y),
+               cc = (double)(-3 * p0.y + 3 * p1.y), cd = (double)(p0.y - pt.y);
+        int num_sol = solve_cubic_d(ca, cb, cc, cd, t);
+        // a root at a segment end is counted by the side its last bit falls on: take
+        // that bit from correctly rounded libm functions, as the host libm gives it
+        bool edge = false;
+        for (int j = 0; j < num_sol; j++) edge |= fabs(t[j]) < 1e-9 || fabs(t[j] - 1.0) < 1e-9;
+        if (edge) num_sol = solve_cubic_d_cr(ca, cb, cc, cd, t);
         float ax = -p0.x + 3 * p1.x - 3 * p2.x + p3.x;
         float bx = 3 * p0.x - 6 * p1.x + 3 * p2.x;
         float cx = -3 * p0.x + 3 * p1.x;
