@@ -349,11 +349,15 @@ def load_ref():
 class RefRunner:
     """Runs the reference CPU build on one flattened scene (builds pybind objects)."""
 
-    def __init__(self, flat):
-        dv = load_ref()
+    def __init__(self, flat, dv=None, use_gpu=False, gpu_index=-1):
+        """dv: the module to drive (default: the reference build); the product's
+        drop-in shim diffvg_amd.diffvg is driven the same way with use_gpu=True."""
+        if dv is None:
+            dv = load_ref()
         if dv is None:
             raise RuntimeError('reference CPU build not available')
         self.dv = dv
+        self.use_gpu, self.gpu_index = use_gpu, gpu_index
         self.flat = flat
         f = flat
         keep = []
@@ -421,13 +425,15 @@ class RefRunner:
             groups.append(dv.ShapeGroup(dv.int_ptr(ids.ctypes.data), n, ft, fp, st_, sp_,
                                         bool(f['use_even_odd_rule'][g]), dv.float_ptr(m.ctypes.data)))
         ft_enum = [dv.FilterType.box, dv.FilterType.tent, dv.FilterType.parabolic, dv.FilterType.hann]
-        if f['filter_type'] >= 4:
+        if f['filter_type'] >= 4 and not hasattr(dv.FilterType, 'gaussian'):
             raise ValueError('the reference has no Gaussian filter')
+        if f['filter_type'] >= 4:
+            ft_enum.append(dv.FilterType.gaussian)
         self.filt = dv.Filter(ft_enum[f['filter_type']], float(f['filter_radius']))
         self.shapes = shapes
         self.groups = groups
         self._keep = keep
-        self.scene = dv.Scene(f['canvas_width'], f['canvas_height'], shapes, groups, self.filt, False, -1)
+        self.scene = dv.Scene(f['canvas_width'], f['canvas_height'], shapes, groups, self.filt, use_gpu, gpu_index)
 
     def render(self, width, height, nsx, nsy, seed, *, output='color', prefilter=False, background=None,
                d_image=None, d_sdf=None, translation=False, eval_positions=None, d_background=False):
@@ -453,7 +459,7 @@ class RefRunner:
         # A fresh scene per call so that the gradient buffers start at zero.
         if d_image is not None or d_sdf is not None:
             self.scene = dv.Scene(self.flat['canvas_width'], self.flat['canvas_height'], self.shapes, self.groups,
-                                  self.filt, False, -1)
+                                  self.filt, self.use_gpu, self.gpu_index)
         dv.render(self.scene, P(bg), P(img), P(sdf), width, height, nsx, nsy, seed, P(d_bg), P(di), P(ds),
                   P(d_tr), bool(prefilter), P(ev), eval_n)
         out = dict(image=img, sdf=sdf, d_translation=d_tr, d_background=d_bg)

@@ -89,10 +89,6 @@ def test_valid_scene_without_device_reports_hip_error():
     assert rc != 0 and 'hip' in err.lower()
 
 
-def _as_host(args):
-    return args
-
-
 @pytest.mark.parametrize('case', sorted(scenes.CASES))
 def test_flatten_args_matches_independent_flattening(case):
     sc = scenes.CASES[case]()
@@ -172,3 +168,27 @@ def test_bench_refuses_more_than_one_gpu():
     p = subprocess.run([sys.executable, os.path.join(ROOT, 'bench.py'), '--gpus', '2', '--no-cpu-baseline'],
                        capture_output=True, text=True, timeout=120)
     assert p.returncode != 0 and 'one GPU' in p.stderr
+
+
+@pytest.mark.parametrize('case', sorted(scenes.CASES))
+def test_drop_in_module_builds_the_same_scene(case, monkeypatch):
+    """The `diffvg` module shim, given pybind-style objects the way render_pytorch.py
+    builds them, produces the same dvg_scene_desc as the independent flattening."""
+    from diffvg_amd import diffvg as dv
+    from refdrive import RefRunner
+    captured = {}
+
+    class Capture:
+        def __init__(self, flat, gpu_index=-1, mode=0):
+            captured['flat'] = flat
+
+    monkeypatch.setattr(dv._lib, 'DeviceScene', Capture)
+    f = scenes.flat_case(case)
+    RefRunner(f, dv=dv, use_gpu=True, gpu_index=0)
+    got = captured['flat']
+    for k, v in f.items():
+        if isinstance(v, np.ndarray):
+            n = len(np.asarray(got[k]))
+            assert np.array_equal(np.asarray(got[k]), v[:n]), k
+        else:
+            assert got[k] == v, k

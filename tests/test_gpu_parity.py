@@ -18,7 +18,7 @@ import torch
 import diffvg_amd as pd
 from diffvg_amd import _lib, workloads
 import scenes
-from gpudrive import GpuRunner
+from gpudrive import GpuRunner, ShimRunner
 from refdrive import OracleRunner, flatten
 from test_oracle_pins import GRAD_KEYS, _rel, check_against_fixture, fixture_names, load_fixture
 
@@ -329,3 +329,13 @@ def test_gpu_edge_sampling_unbiased():
     r = GpuRunner(flatten(48, 48, [c], [g], 0, 0.5))
     gs = [r.render(48, 48, 4, 4, s, d_image=W)['grads']['shape_params'][1] for s in range(64)]
     assert np.mean(gs) == pytest.approx(2 * math.sqrt(100 - 9), rel=0.03)
+
+
+@pytest.mark.parametrize('name', ['gradients', 'thick_stroke', 'transformed', 'curve_fill', 'primitives'])
+def test_drop_in_diffvg_module_matches_reference_fixture(name):
+    """The `diffvg` module shim (Scene / render / get_d_shape / get_d_shape_group /
+    get_d_filter_radius), driven the way render_pytorch.py drives the reference's pybind11
+    module, against the reference's own outputs."""
+    print('shim', name, flush=True)
+    flat, z = load_fixture(name)
+    check_against_fixture(_runner_fn(ShimRunner(flat)), flat, z, 'shim/' + name)
