@@ -25,6 +25,7 @@ struct Workspace {
     int *tile_count = nullptr;
     int *tile_off = nullptr;
     int *tile_cursor = nullptr;
+    int *chunk_off = nullptr;  // [ntiles + 1] boundary-sample chunk offsets per tile
     int reserve(size_t npix, size_t nsamp, size_t ntiles);
     void release();
 };

@@ -449,31 +449,43 @@ __global__ __launch_bounds__(kBlock) void k_edge_sample(DevScene S, Geom G, int 
     edge_tile[idx] = tile;
 }
 
-// Exclusive scan of tile counts in one workgroup.
-__global__ __launch_bounds__(1024) void k_scan(const int *in, int *out, int *cursor, int n) {
+// Exclusive scans, in one workgroup, of the per-tile boundary-sample counts (-> tile
+// offsets) and of the per-tile chunk counts ceil(count / kBlock) (-> chunk offsets; entry
+// n holds the total), so that k_edge can run one workgroup per 256-sample chunk.
+__global__ __launch_bounds__(1024) void k_scan(const int *in, int *out, int *cursor, int *chunk_off, int n) {
     __shared__ int s[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
+    __shared__ int sc[1024];
+    __shared__ int carry, carry_c;
+    if (threadIdx.x == 0) carry = carry_c = 0;
     __syncthreads();
     for (int base = 0; base < n; base += 1024) {
         int i = base + threadIdx.x;
         int v = i < n ? in[i] : 0;
+        int vc = (v + kBlock - 1) / kBlock;
         s[threadIdx.x] = v;
+        sc[threadIdx.x] = vc;
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {
             int t = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+            int tc = threadIdx.x >= o ? sc[threadIdx.x - o] : 0;
             __syncthreads();
             s[threadIdx.x] += t;
+            sc[threadIdx.x] += tc;
             __syncthreads();
         }
         if (i < n) {
             out[i] = carry + s[threadIdx.x] - v;
+            chunk_off[i] = carry_c + sc[threadIdx.x] - vc;
             cursor[i] = 0;
         }
         __syncthreads();
-        if (threadIdx.x == 1023) carry += s[1023];
+        if (threadIdx.x == 1023) {
+            carry += s[1023];
+            carry_c += sc[1023];
+        }
         __syncthreads();
     }
+    if (threadIdx.x == 0) chunk_off[n] = carry_c;
 }
 
 __global__ __launch_bounds__(kBlock) void k_edge_bin(int count, const int *edge_tile, const int *tile_off,
@@ -486,20 +498,35 @@ __global__ __launch_bounds__(kBlock) void k_edge_bin(int count, const int *edge_
     sorted[pos] = idx;
 }
 
+// One workgroup per chunk of up to 256 boundary samples of one tile (the grid is an
+// upper bound on the chunk count; workgroups past the total exit at once), so a tile on a
+// busy boundary is spread over many workgroups instead of looping inside one.
 __global__ __launch_bounds__(kBlock) void k_edge(DevScene S, Geom G, const EdgeSample *edges, const int *tile_off,
-                                                 const int *tile_count, const int *sorted, const float *bg,
-                                                 const float *weight, const float *d_img, float *d_translation) {
+                                                 const int *tile_count, const int *chunk_off, int ntiles,
+                                                 const int *sorted, const float *bg, const float *weight,
+                                                 const float *d_img, float *d_translation) {
     __shared__ int s_list[kCandCap];
     __shared__ int s_w[kBlock / 64];
-    int tile = blockIdx.x;
-    int cnt = tile_count[tile];
-    if (cnt == 0) return;
+    const int chunk = blockIdx.x;
+    if (chunk >= chunk_off[ntiles]) return;
+    // tile = the last t with chunk_off[t] <= chunk (chunk_off is non-decreasing)
+    int lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (chunk_off[mid] <= chunk)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int tile = lo;
+    const int cnt = tile_count[tile];
     int tx = tile % G.tiles_x, ty = tile / G.tiles_x;
     float ext = 1e-4f * (float)max(S.cw, S.ch) * 1.01f;
     float4 tb = tile_box(S, G, tx, ty, ext);
     int off = tile_off[tile];
     bool has_bg = bg != nullptr;
-    for (int b0 = 0; b0 < cnt; b0 += kBlock) {
+    {
+        const int b0 = (chunk - chunk_off[tile]) * kBlock;
         int k = b0 + threadIdx.x;
         bool valid = k < cnt;
         EdgeSample es;
@@ -523,9 +550,9 @@ __global__ __launch_bounds__(kBlock) void k_edge(DevScene S, Geom G, const EdgeS
             }
             __syncthreads();
         }
-        if (!valid) continue;
+        if (!valid) return;
         bool in_hit = ci.edge_hit(hin), out_hit = co.edge_hit(hout);
-        if (!in_hit && !out_hit) continue;
+        if (!in_hit && !out_hit) return;
         V4 cin = ci.result(has_bg, bgc), cout = co.result(has_bg, bgc);
         if (!in_hit) {
             n = -n;
@@ -867,7 +894,8 @@ int Workspace::reserve(size_t npix, size_t nsamp, size_t ntiles) {
         al((void **)&edge_sorted, nsamp * sizeof(int)) != hipSuccess ||
         al((void **)&tile_count, ntiles * sizeof(int)) != hipSuccess ||
         al((void **)&tile_off, ntiles * sizeof(int)) != hipSuccess ||
-        al((void **)&tile_cursor, ntiles * sizeof(int)) != hipSuccess) {
+        al((void **)&tile_cursor, ntiles * sizeof(int)) != hipSuccess ||
+        al((void **)&chunk_off, (ntiles + 1) * sizeof(int)) != hipSuccess) {
         release();
         return set_error("out of device memory for the render workspace");
     }
@@ -910,13 +938,13 @@ void release_workspace(int device, hipStream_t st) {
 }
 
 void Workspace::release() {
-    void *ps[] = {weight, colors, edges, edge_tile, edge_sorted, tile_count, tile_off, tile_cursor};
+    void *ps[] = {weight, colors, edges, edge_tile, edge_sorted, tile_count, tile_off, tile_cursor, chunk_off};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     weight = nullptr;
     colors = nullptr;
     edges = nullptr;
-    edge_tile = edge_sorted = tile_count = tile_off = tile_cursor = nullptr;
+    edge_tile = edge_sorted = tile_count = tile_off = tile_cursor = chunk_off = nullptr;
     npix_cap = nsamp_cap = ntiles_cap = 0;
 }
 
@@ -1033,15 +1061,17 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
                 }
                 {
                     DVG_PROF(st, "k_edge_bin");
-                    k_scan<<<1, 1024, 0, st>>>(ws.tile_count, ws.tile_off, ws.tile_cursor, ntiles);
+                    k_scan<<<1, 1024, 0, st>>>(ws.tile_count, ws.tile_off, ws.tile_cursor, ws.chunk_off, ntiles);
                     k_edge_bin<<<(unsigned)((nsamp + kBlock - 1) / kBlock), kBlock, 0, st>>>(
                         (int)nsamp, ws.edge_tile, ws.tile_off, ws.tile_cursor, ws.edge_sorted);
                 }
                 {
                     DVG_PROF(st, "k_edge");
-                    k_edge<<<ntiles, kBlock, 0, st>>>(S, G, ws.edges, ws.tile_off, ws.tile_count,
-                                                      ws.edge_sorted, a.background_image, ws.weight,
-                                                      a.d_render_image, a.d_translation);
+                    // at most one partial chunk per tile plus nsamp / 256 full ones
+                    unsigned grid = (unsigned)(ntiles + (nsamp + kBlock - 1) / kBlock);
+                    k_edge<<<grid, kBlock, 0, st>>>(S, G, ws.edges, ws.tile_off, ws.tile_count, ws.chunk_off,
+                                                    ntiles, ws.edge_sorted, a.background_image, ws.weight,
+                                                    a.d_render_image, a.d_translation);
                 }
             }
         }
