@@ -128,6 +128,262 @@ def make_params(c, device):
     return params
 
 
+def _scene_params(shapes, groups):
+    import torch
+    ps = []
+    for sh in shapes:
+        for name in ('points', 'radius', 'center', 'p_min', 'p_max', 'stroke_width'):
+            t = getattr(sh, name, None)
+            if isinstance(t, torch.Tensor) and t.is_floating_point():
+                ps.append(t.requires_grad_(True))
+    for g in groups:
+        for col in (g.fill_color, g.stroke_color):
+            if isinstance(col, torch.Tensor):
+                ps.append(col.requires_grad_(True))
+            elif col is not None:
+                for name in ('begin', 'end', 'center', 'radius', 'offsets', 'stop_colors'):
+                    t = getattr(col, name, None)
+                    if isinstance(t, torch.Tensor):
+                        ps.append(t.requires_grad_(True))
+    return ps
+
+
+def bench_batched(a):
+    """256 uneven scenes (path counts log-uniform in [1, 512], 2-8 cubics per path, 10%
+    gradient fills, resolution in {28, 64, 128, 256}, 2x2 spp, box filter), one fwd+bwd
+    each with an L2 loss against a per-scene target.  Three measurements:
+      graph    -- scenes resident; forward batch, loss gradient and backward batch
+                  captured in one HIP graph and replayed (value);
+      batched  -- per step: marshal and build every scene, one batched forward and one
+                  batched backward, gradients handed to every parameter tensor;
+      plain    -- per step: one RenderFunction call per scene on the PLAIN schedule
+                  (the reference's one-scene-per-launch structure), loss, backward."""
+    import numpy as np
+    import torch
+    import diffvg_amd as pydiffvg
+    from diffvg_amd import workloads, _lib
+    from diffvg_amd.batch import Batch
+    from diffvg_amd.render_pytorch import flatten_args, _grad_tensors
+
+    dev = torch.device('cuda:0')
+    pydiffvg.set_use_gpu(True)
+    pydiffvg.set_device(dev)
+    items = workloads.config4_scenes(device=dev)
+    n = len(items)
+    params = [_scene_params(it['shapes'], it['groups']) for it in items]
+    sizes = [it['canvas'] for it in items]
+    npix = [w * h for w, h in sizes]
+    offs = np.concatenate([[0], np.cumsum(npix)]).astype(np.int64)
+    samples = int(sum(npix)) * 4
+    rng = np.random.default_rng(4)
+    target_flat = torch.from_numpy(rng.uniform(0, 1, (int(offs[-1]), 4)).astype(np.float32)).to(dev)
+    img_flat = torch.zeros(int(offs[-1]), 4, device=dev)
+    d_flat = torch.zeros(int(offs[-1]), 4, device=dev)
+
+    def view(flat, i):
+        w, h = sizes[i]
+        return flat[offs[i]:offs[i + 1]].view(h, w, 4)
+
+    # ---- graph: resident scenes, one captured fwd + loss + bwd
+    def build(i):
+        it = items[i]
+        w, h = sizes[i]
+        args = pydiffvg.RenderFunction.serialize_scene(w, h, it['shapes'], it['groups'], it['filter'])
+        fl = flatten_args(args)
+        return args, fl, _lib.DeviceScene(fl.desc, 0)
+
+    built = [build(i) for i in range(n)]
+    fjobs = [(ds, dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0, render_image=view(img_flat, i)))
+             for i, (_, _, ds) in enumerate(built)]
+    bjobs = [(ds, dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0, d_render_image=view(d_flat, i)))
+             for i, (_, _, ds) in enumerate(built)]
+    fb, bb = Batch(fjobs, zero_grads=False), Batch(bjobs, zero_grads=True)
+    stream = torch.cuda.Stream()
+
+    def graph_body(st):
+        img_flat.zero_()
+        fb.render(st)
+        torch.mul(img_flat - target_flat, 2.0, out=d_flat)
+        bb.render(st)
+
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        graph_body(stream.cuda_stream)
+    torch.cuda.synchronize()
+    fb.status()
+    bb.status()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        graph_body(torch.cuda.current_stream().cuda_stream)
+    for _ in range(a.warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    reps = max(a.steps, 5)
+    L = _lib.lib()
+    L.dvg_profile_enable(0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.replay()
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t0) / reps
+    fb.status()
+    bb.status()
+    # eager kernel breakdown of the same batches
+    names = ctypes.create_string_buffer(4096)
+    ms = (ctypes.c_double * 32)()
+    cnts = (ctypes.c_int * 32)()
+    L.dvg_profile_read(names, 4096, ms, cnts, 32)
+    L.dvg_profile_enable(1)
+    graph_body(None)
+    torch.cuda.synchronize()
+    L.dvg_profile_enable(0)
+    nk = L.dvg_profile_read(names, 4096, ms, cnts, 32)
+    kern = {nm: round(ms[i], 4) for i, nm in enumerate(names.value.decode().strip().split('\n')[:nk])}
+
+    # ---- batched end to end (marshalling and scene builds every step)
+    def batched_step():
+        for ps in params:
+            for p in ps:
+                p.grad = None
+        bl = [build(i) for i in range(n)]
+        f1 = Batch([(ds, dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0,
+                              render_image=view(img_flat, i))) for i, (_, _, ds) in enumerate(bl)], zero_grads=False)
+        img_flat.zero_()
+        f1.render(None)
+        torch.mul(img_flat - target_flat, 2.0, out=d_flat)
+        b1 = Batch([(ds, dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0,
+                              d_render_image=view(d_flat, i))) for i, (_, _, ds) in enumerate(bl)], zero_grads=True)
+        b1.render(None)
+        for args, fl, ds in bl:
+            block, layout = ds.grads_device(dev)
+            for t, gt in zip(args, _grad_tensors(fl, block, layout, args, fl.num_args)):
+                if gt is not None and isinstance(t, torch.Tensor) and t.requires_grad:
+                    t.grad = gt
+        b1.status()
+
+    for _ in range(2):
+        batched_step()
+    torch.cuda.synchronize()
+    nb = max(2, a.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(nb):
+        batched_step()
+    torch.cuda.synchronize()
+    t_batched = (time.perf_counter() - t0) / nb
+
+    # ---- plain: one RenderFunction call per scene (PLAIN schedule)
+    pydiffvg.set_render_mode('plain')
+
+    def plain_step():
+        for ps in params:
+            for p in ps:
+                p.grad = None
+        for i, it in enumerate(items):
+            w, h = sizes[i]
+            args = pydiffvg.RenderFunction.serialize_scene(w, h, it['shapes'], it['groups'], it['filter'])
+            img = pydiffvg.RenderFunction.apply(w, h, 2, 2, 0, None, *args)
+            ((img - view(target_flat, i)) ** 2).sum().backward()
+
+    plain_step()
+    torch.cuda.synchronize()
+    np_ = max(1, a.steps // 10)
+    t0 = time.perf_counter()
+    for _ in range(np_):
+        plain_step()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / np_
+    pydiffvg.set_render_mode('tiled')
+
+    out = {
+        'metric': 'fwd+bwd Msamples/s on 1 GPU; p50/p99 latency, batched/served vs plain',
+        'value': round(samples / t_graph / 1e6, 4), 'unit': 'Msamples/s', 'n_gpus': 1, 'steps': reps,
+        'warmup': a.warmup, 'ms_per_step': round(t_graph * 1e3, 4), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32 (f64 root solves)', 'data': 'synthetic',
+        'config': {'workload': 'config4_256_uneven_scenes', 'scenes': n, 'samples': samples,
+                   'paths': int(sum(len(it['shapes']) for it in items)), 'spp': [2, 2], 'filter': 'box r=0.5',
+                   'mode': 'batched, one HIP graph per fwd+bwd'},
+        'batched_end_to_end': {'ms_per_step': round(t_batched * 1e3, 3),
+                               'value': round(samples / t_batched / 1e6, 4)},
+        'plain': {'ms_per_step': round(t_plain * 1e3, 3), 'value': round(samples / t_plain / 1e6, 4)},
+        'speedup_graph_vs_plain': round(t_plain / t_graph, 2),
+        'speedup_batched_vs_plain': round(t_plain / t_batched, 2),
+        'kernels_ms_eager': kern,
+    }
+    print(json.dumps(out))
+
+
+def bench_serving(a):
+    """Serving stream: Poisson arrivals of render + gradient requests on resident scenes
+    (path counts log-uniform in [1, 4096], resolution in {128, 256, 512, 1024}, colour or
+    SDF, prefiltering on or off).  The arrival rate is 0.8x the plain path's sustained
+    request rate measured in the same run (one request at a time, PLAIN schedule); the
+    same arrival sequence is then served by the continuous-batching server and by the
+    plain path, and p50/p99 latency (arrival to gradients ready) and throughput are
+    reported for both."""
+    import numpy as np
+    import torch
+    import diffvg_amd as pydiffvg
+    from diffvg_amd import workloads, _lib, serve
+    from diffvg_amd.render_pytorch import flatten_args
+
+    dev = torch.device('cuda:0')
+    pydiffvg.set_use_gpu(True)
+    pydiffvg.set_device(dev)
+    rng = np.random.default_rng(5)
+    n_req = a.requests
+    specs = []
+    for i in range(n_req):
+        r = workloads.config5_request(rng, device='cpu')
+        w, h = r['canvas']
+        args = pydiffvg.RenderFunction.serialize_scene(w, h, r['shapes'], r['groups'], r['filter'])
+        scene = _lib.DeviceScene(flatten_args(args).desc, 0)
+        ch = 1 if r['sdf'] else 4
+        target = torch.from_numpy(rng.uniform(0, 1, (h, w, ch)).astype(np.float32)).to(dev)
+        specs.append((scene, w, h, r['nsx'], r['nsy'], r['sdf'], r['prefilter'], target, len(r['shapes'])))
+
+    def make(i):
+        sc, w, h, nsx, nsy, sdf, pre, target, _ = specs[i]
+        return serve.Request(i, sc, w, h, nsx, nsy, 0, sdf, pre, target, 0.0)
+
+    # plain service rate, back to back
+    cal = [make(i) for i in range(n_req)]
+    serve.serve_plain(cal[:4], [0.0] * 4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    serve.serve_plain(cal, [0.0] * n_req)
+    t_plain_all = time.perf_counter() - t0
+    mu = n_req / t_plain_all
+    lam = 0.8 * mu
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
+    arrivals -= arrivals[0]
+
+    # batched server (warm-up on the first requests, then the timed stream)
+    srv = serve.Server(device=0, hbm_budget=a.hbm_budget << 30, max_batch=a.max_batch, streams=3)
+    srv.run([make(i) for i in range(min(8, n_req))], [0.0] * min(8, n_req))
+    torch.cuda.synchronize()
+    srv = serve.Server(device=0, hbm_budget=a.hbm_budget << 30, max_batch=a.max_batch, streams=3)
+    done = srv.run([make(i) for i in range(n_req)], list(arrivals))
+    st_b = serve.latency_stats(done)
+    plain_done = serve.serve_plain([make(i) for i in range(n_req)], list(arrivals))
+    st_p = serve.latency_stats(plain_done)
+    samples = sum(w * h * (1 if sdf else nsx * nsy) for (_, w, h, nsx, nsy, sdf, _, _, _) in specs)
+    span_b = max(r.done for r in done) - min(r.arrival for r in done)
+    out = {
+        'metric': 'fwd+bwd Msamples/s on 1 GPU; p50/p99 latency, batched/served vs plain',
+        'value': round(samples / span_b / 1e6, 4), 'unit': 'Msamples/s', 'n_gpus': 1, 'steps': n_req,
+        'warmup': 8, 'ms_per_step': round(span_b / n_req * 1e3, 4), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32 (f64 root solves)', 'data': 'synthetic',
+        'config': {'workload': 'config5_serving_stream', 'requests': n_req, 'samples': samples,
+                   'mean_paths': float(np.mean([s_[8] for s_ in specs])),
+                   'sdf_fraction': float(np.mean([s_[5] for s_ in specs])),
+                   'arrival_rate_rps': round(lam, 3), 'plain_service_rate_rps': round(mu, 3),
+                   'hbm_budget_gib': a.hbm_budget, 'max_batch': a.max_batch, 'streams': 3},
+        'served': {k: round(v, 3) for k, v in st_b.items()},
+        'plain': {k: round(v, 3) for k, v in st_p.items()},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -137,15 +393,24 @@ def main():
     ap.add_argument('--mode', default='tiled', choices=['tiled', 'plain'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-baseline-only', action='store_true')
+    ap.add_argument('--requests', type=int, default=120, help='serving stream length (config 5)')
+    ap.add_argument('--hbm-budget', type=int, default=64, help='GiB that requests in flight may hold (config 5)')
+    ap.add_argument('--max-batch', type=int, default=32, help='requests per batch (config 5)')
     a = ap.parse_args()
     if a.cpu_baseline_only:
         cpu_baseline_main(a.config)
         return
     if a.gpus != 1:
-        print('bench.py measures one GPU: every box in this setup has one MI355X, so --gpus %d is not run '
-              '(the multi-GPU sharding design is described in DESIGN.md)' % a.gpus, file=sys.stderr)
+        print('bench.py measures one GPU: the render path is built and measured on one MI355X, so --gpus %d '
+              'is not run (how it would shard is a design note, not a measured path)' % a.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if a.config == 4:
+        bench_batched(a)
+        return
+    if a.config == 5:
+        bench_serving(a)
+        return
     cpu = None if a.no_cpu_baseline else run_cpu_baseline(a.config)
 
     import numpy as np

@@ -80,6 +80,11 @@ def lib():
     L.dvg_scene_grad_layout.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64 * 7),
                                         ctypes.POINTER(ctypes.c_int64)]
     L.dvg_scene_copy_grads.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.dvg_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(RenderArgs), ctypes.c_int,
+                                   ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    L.dvg_batch_render.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.dvg_batch_status.argtypes = [ctypes.c_void_p]
+    L.dvg_batch_destroy.argtypes = [ctypes.c_void_p]
     L.dvg_profile_enable.argtypes = [ctypes.c_int]
     L.dvg_profile_read.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_int), ctypes.c_int]
@@ -130,13 +135,17 @@ class DeviceScene:
     def set_mode(self, mode):
         check(self._lib.dvg_scene_set_mode(self.handle, int(mode)))
 
-    def render(self, width, height, nsx, nsy, seed, background=None, render_image=None, render_sdf=None,
-               d_background=None, d_render_image=None, d_render_sdf=None, d_translation=None,
-               use_prefiltering=False, eval_positions=None, num_eval_positions=0, stream=None):
-        a = RenderArgs(ptr(background), ptr(render_image), ptr(render_sdf), int(width), int(height), int(nsx),
-                       int(nsy), int(seed) & 0xFFFFFFFFFFFFFFFF, ptr(d_background), ptr(d_render_image),
-                       ptr(d_render_sdf), ptr(d_translation), 1 if use_prefiltering else 0, ptr(eval_positions),
-                       int(num_eval_positions), stream)
+    @staticmethod
+    def make_args(width, height, nsx, nsy, seed, background=None, render_image=None, render_sdf=None,
+                  d_background=None, d_render_image=None, d_render_sdf=None, d_translation=None,
+                  use_prefiltering=False, eval_positions=None, num_eval_positions=0, stream=None):
+        return RenderArgs(ptr(background), ptr(render_image), ptr(render_sdf), int(width), int(height), int(nsx),
+                          int(nsy), int(seed) & 0xFFFFFFFFFFFFFFFF, ptr(d_background), ptr(d_render_image),
+                          ptr(d_render_sdf), ptr(d_translation), 1 if use_prefiltering else 0, ptr(eval_positions),
+                          int(num_eval_positions), stream)
+
+    def render(self, width, height, nsx, nsy, seed, **kw):
+        a = self.make_args(width, height, nsx, nsy, seed, **kw)
         check(self._lib.dvg_render(self.handle, ctypes.byref(a)))
 
     def zero_grads(self, stream=None):

@@ -59,6 +59,36 @@ void cache_free(int device, void *p, size_t bytes) {
     g_cache[{device, size_class(bytes)}].push_back(p);
 }
 
+// A non-blocking stream per device for scene and batch uploads: a copy on it waits only
+// for itself, never for kernels that other streams have in flight (the null stream
+// would order it after all of them).
+hipStream_t upload_stream(int device) {
+    static std::mutex m;
+    static std::map<int, hipStream_t> streams;
+    std::lock_guard<std::mutex> lk(m);
+    hipStream_t &s = streams[device];
+    if (!s && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    return s;
+}
+
+int upload(int device, void *dst, const void *src, size_t bytes) {
+    hipStream_t s = upload_stream(device);
+    if (!s) return set_error("cannot create the upload stream");
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return set_error(std::string("upload: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int clear_device(int device, void *dst, size_t bytes) {
+    hipStream_t s = upload_stream(device);
+    if (!s) return set_error("cannot create the upload stream");
+    hipError_t e = hipMemsetAsync(dst, 0, bytes, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return set_error(std::string("clear: ") + hipGetErrorString(e));
+    return 0;
+}
+
 int set_error(const std::string &msg) {
     t_error = msg;
     return 1;
@@ -394,6 +424,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     std::vector<float> s2c(9 * (size_t)ng), c2s(9 * (size_t)ng);
     std::vector<float4> g_lbox(ng), g_box(ng);
     std::vector<float> g_maxr(ng);
+    std::vector<float4> g_sdf(ng);
     std::vector<HNode> top(ng > 0 ? 2 * (size_t)ng - 1 : 1, HNode{-1, -1, hbox_empty(), 0.f});
     int total = 0;
     for (int g = 0; g < ng; g++) {
@@ -489,6 +520,40 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
             float r = stroke_type[g] < 0 ? 0.f : maxr;
             g_box[g] = make_float4(tb.minx, tb.miny, tb.maxx, tb.maxy);
             g_maxr[g] = r;
+            // SDF bound (sample_distance, diffvg.cpp:709-775): a boundary point q of a
+            // member whose computed distance is always <= |pt - q|: the first point of a
+            // path (its first segment's t = 0 end is always a candidate of the segment's
+            // closest-point search, and a pruned segment is farther than the running
+            // minimum).  Valid when the transform preserves distance ratios and no member
+            // uses the approximate quadratic distance.
+            bool similarity = m[6] == 0.f && m[7] == 0.f && m[8] == 1.f &&
+                              ((m[0] == m[4] && m[1] == -m[3]) || (m[0] == -m[4] && m[1] == m[3]));
+            bool findable = false, approx_quad = false;
+            HV2 anchor{0, 0};
+            bool have_anchor = false;
+            for (int i = 0; i < n; i++) {
+                int s = d.group_shape_ids[g_off[g] + i];
+                if (shape_type[s] == 2 && nsegs[s] > 0) {
+                    findable = true;
+                    if (approx[s])
+                        for (int k = 0; k < nsegs[s]; k++) approx_quad |= seg_kind[seg_begin[s] + k] == 1;
+                    if (!have_anchor) {
+                        anchor = HV2{points[pt_off[s]].x, points[pt_off[s]].y};
+                        have_anchor = true;
+                    }
+                } else if (shape_type[s] == 3) {
+                    // a rect reports one of its corners, chosen by its nearest side, so
+                    // no single point bounds its distance: findable, never an anchor
+                    findable = true;
+                }
+            }
+            float qnan = std::numeric_limits<float>::quiet_NaN();
+            if (findable && have_anchor && similarity && !approx_quad) {
+                HV2 q = xform(m, anchor);
+                g_sdf[g] = make_float4(q.x, q.y, 1.f, 0.f);
+            } else {
+                g_sdf[g] = make_float4(qnan, qnan, findable ? 1.f : 0.f, 0.f);
+            }
             top[g] = HNode{g, -1, tb, r};
         }
         if (ng > 0) build_bvh(top.data(), ng, true, sc.cw, sc.ch);
@@ -510,7 +575,8 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
                o_fn = P.add(fill_ns), o_st = P.add(stroke_type), o_so = P.add(stroke_off), o_sn = P.add(stroke_ns),
                o_color = P.add(std::vector<float>(d.color_params, d.color_params + d.num_color_params)),
                o_eo = P.add(even_odd), o_s2c = P.add(s2c), o_c2s = P.add(c2s), o_glb = P.add(g_lbox),
-               o_gb = P.add(g_box), o_gmr = P.add(g_maxr), o_top = P.add(topd), o_len = P.add(shape_length),
+               o_gb = P.add(g_box), o_gmr = P.add(g_maxr), o_gsdf = P.add(g_sdf), o_top = P.add(topd),
+               o_len = P.add(shape_length),
                o_cdf = P.add(cdf), o_pmf = P.add(pmf), o_sid = P.add(sid), o_gid = P.add(gid);
         sc.block_bytes = P.host.size();
         // validation and host-side construction are done: only now is a device needed
@@ -519,7 +585,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
         sc.device_set = true;
         sc.block = cache_alloc(sc.device, sc.block_bytes, &sc.block_granted);
         if (!sc.block) return set_error("out of device memory for the scene");
-        HIP_OK(hipMemcpy(sc.block, P.host.data(), sc.block_bytes, hipMemcpyHostToDevice));
+        if (int e = upload(sc.device, sc.block, P.host.data(), sc.block_bytes)) return e;
         uint8_t *B = (uint8_t *)sc.block;
         DevScene &D = sc.dev;
         D.shape_type = (const int *)(B + o_shape_type);
@@ -560,6 +626,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
         D.g_lbox = (const float4 *)(B + o_glb);
         D.g_box = (const float4 *)(B + o_gb);
         D.g_maxr = (const float *)(B + o_gmr);
+        D.g_sdf = (const float4 *)(B + o_gsdf);
         D.top = (const BNode *)(B + o_top);
         D.shape_length = (const float *)(B + o_len);
         D.shapes_cdf = (const float *)(B + o_cdf);
@@ -588,7 +655,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     sc.grad_floats = sc.g_filter + 64;
     sc.grad = (float *)cache_alloc(sc.device, sc.grad_floats * sizeof(float), &sc.grad_granted);
     if (!sc.grad) return set_error("out of device memory for the gradients");
-    HIP_OK(hipMemset(sc.grad, 0, sc.grad_floats * sizeof(float)));
+    if (int e = clear_device(sc.device, sc.grad, sc.grad_floats * sizeof(float))) return e;
     D.d_params = sc.grad + sc.g_params;
     D.d_points = sc.grad + sc.g_points;
     D.d_thickness = sc.grad + sc.g_thick;
@@ -599,7 +666,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     size_t eg = 0;
     sc.err = (int *)cache_alloc(sc.device, sizeof(int), &eg);
     if (!sc.err) return set_error("out of device memory");
-    HIP_OK(hipMemset(sc.err, 0, sizeof(int)));
+    if (int e = clear_device(sc.device, sc.err, sizeof(int))) return e;
     return 0;
 }
 
@@ -681,6 +748,43 @@ int dvg_render(dvg_scene *scene, const dvg_render_args *args) {
     scene->h.last_stream = (hipStream_t)a.stream;
     if (prev >= 0 && prev != scene->h.device) (void)hipSetDevice(prev);
     return rc;
+}
+
+struct dvg_batch {
+    dvg::BatchHost *b;
+};
+
+int dvg_batch_create(dvg_scene *const *scenes, const dvg_render_args *args, int n, int flags, dvg_batch **out) {
+    if (!scenes || !args || !out) return set_error("null argument");
+    t_error.clear();
+    std::vector<SceneHost *> hs(n > 0 ? n : 0);
+    for (int i = 0; i < n; i++) {
+        if (!scenes[i]) return set_error("null scene in batch");
+        hs[i] = &scenes[i]->h;
+    }
+    dvg::BatchHost *b = nullptr;
+    if (int rc = dvg::batch_create(hs.data(), args, n, (flags & DVG_BATCH_ZERO_GRADS) ? 1 : 0, &b)) return rc;
+    *out = new dvg_batch{b};
+    return 0;
+}
+
+int dvg_batch_render(dvg_batch *batch, void *stream) {
+    if (!batch) return set_error("null batch");
+    int rc = dvg::batch_render(*batch->b, (hipStream_t)stream);
+    for (auto *sc : dvg::batch_scenes(*batch->b)) sc->last_stream = (hipStream_t)stream;
+    return rc;
+}
+
+int dvg_batch_status(dvg_batch *batch) {
+    if (!batch) return set_error("null batch");
+    return dvg::batch_status(*batch->b);
+}
+
+int dvg_batch_destroy(dvg_batch *batch) {
+    if (!batch) return 0;
+    dvg::batch_destroy(batch->b);
+    delete batch;
+    return 0;
 }
 
 int dvg_scene_zero_grads(dvg_scene *scene, void *stream) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/diffvg_amd.h"
 #include "dvg_scene.h"
@@ -13,6 +14,11 @@ namespace dvg {
 struct EdgeSample;
 
 int set_error(const std::string &msg);  // records the message, returns 1
+
+// Host-to-device copy / clear on the device's non-blocking upload stream (synchronous
+// for the caller, independent of work in flight on other streams).
+int upload(int device, void *dst, const void *src, size_t bytes);
+int clear_device(int device, void *dst, size_t bytes);
 
 // Per-scene scratch reused across renders of the same or smaller size.
 struct Workspace {
@@ -57,6 +63,14 @@ struct SceneHost {
     hipStream_t last_stream = nullptr;
     DevScene dev{};
 };
+
+// Batched renders (one kernel per stage over the jobs of many scenes).
+struct BatchHost;
+int batch_create(SceneHost *const *scenes, const dvg_render_args *args, int n, int zero_grads, BatchHost **out);
+int batch_render(BatchHost &b, hipStream_t st);
+int batch_status(BatchHost &b);
+void batch_destroy(BatchHost *b);
+const std::vector<SceneHost *> &batch_scenes(const BatchHost &b);
 
 int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc);
 void free_scene(SceneHost &sc);

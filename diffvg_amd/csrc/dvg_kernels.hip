@@ -20,6 +20,7 @@
 
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <mutex>
 #include <vector>
@@ -232,8 +233,7 @@ DVG_DEV V4 load_bg(const float *bg, int pix) {
 }
 
 // ------------------------------------------------------------------ TILED kernels
-__global__ __launch_bounds__(kBlock) void k_weight(DevScene S, Geom G, float *weight) {
-    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+DVG_DEV void weight_pixel(const DevScene &S, const Geom &G, int pix, float *weight) {
     if (pix >= G.W * G.H) return;
     int px = pix % G.W, py = pix / G.W;
     float radius = S.filter_radius;
@@ -254,13 +254,16 @@ __global__ __launch_bounds__(kBlock) void k_weight(DevScene S, Geom G, float *we
     }
     weight[pix] = acc;
 }
+__global__ __launch_bounds__(kBlock) void k_weight(DevScene S, Geom G, float *weight) {
+    weight_pixel(S, G, blockIdx.x * blockDim.x + threadIdx.x, weight);
+}
 
 template <int PRE>
-__global__ __launch_bounds__(kBlock) void k_color(DevScene S, Geom G, const float *bg, float4 *colors, int *err) {
+DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float *bg, float4 *colors, int *err) {
     constexpr int prefilter = PRE;
     __shared__ int s_list[kCandCap];
     __shared__ int s_w[kBlock / 64];
-    int tx = blockIdx.x % G.tiles_x, ty = blockIdx.x / G.tiles_x;
+    int tx = tile % G.tiles_x, ty = tile / G.tiles_x;
     int x, y, sx, sy, idx;
     bool valid = tile_sample(G, tx, ty, threadIdx.x, x, y, sx, sy, idx);
     V2 pt{0, 0};
@@ -289,10 +292,13 @@ __global__ __launch_bounds__(kBlock) void k_color(DevScene S, Geom G, const floa
     }
     if (overflow == 2) atomicOr(err, 2);
 }
+template <int PRE>
+__global__ __launch_bounds__(kBlock) void k_color(DevScene S, Geom G, const float *bg, float4 *colors, int *err) {
+    color_tile<PRE>(S, G, blockIdx.x, bg, colors, err);
+}
 
-__global__ __launch_bounds__(kBlock) void k_resolve(DevScene S, Geom G, int prefilter, const float *weight,
-                                                    const float4 *colors, float *out) {
-    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+DVG_DEV void resolve_pixel(const DevScene &S, const Geom &G, int pix, int prefilter, const float *weight,
+                           const float4 *colors, float *out) {
     if (pix >= G.W * G.H) return;
     float ws = weight[pix];
     if (!(ws > 0)) return;
@@ -319,18 +325,21 @@ __global__ __launch_bounds__(kBlock) void k_resolve(DevScene S, Geom G, int pref
     float4 prev = *o;
     *o = make_float4(prev.x + acc.x, prev.y + acc.y, prev.z + acc.z, prev.w + acc.w);
 }
+__global__ __launch_bounds__(kBlock) void k_resolve(DevScene S, Geom G, int prefilter, const float *weight,
+                                                    const float4 *colors, float *out) {
+    resolve_pixel(S, G, blockIdx.x * blockDim.x + threadIdx.x, prefilter, weight, colors, out);
+}
 
 template <int PRE>
-__global__ __launch_bounds__(kBlock) void k_color_bwd(DevScene S, Geom G, const float *bg, const float *weight,
-                                                      const float *d_img, float *d_bg, float *d_translation,
-                                                      int *err) {
+DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const float *bg, const float *weight,
+                            const float *d_img, float *d_bg, float *d_translation, int *err) {
     constexpr int prefilter = PRE;
     __shared__ int s_list[kCandCap];
     __shared__ int s_w[kBlock / 64];
     __shared__ float s_red[kBlock / 64];
     __shared__ float4 s_dbg[kBlock];
     __shared__ int s_dbg_kind[kBlock];
-    int tx = blockIdx.x % G.tiles_x, ty = blockIdx.x / G.tiles_x;
+    int tx = tile % G.tiles_x, ty = tile / G.tiles_x;
     int x, y, sx, sy, idx;
     bool valid = tile_sample(G, tx, ty, threadIdx.x, x, y, sx, sy, idx);
     V2 rpt{0, 0}, pt{0, 0};
@@ -431,14 +440,19 @@ __global__ __launch_bounds__(kBlock) void k_color_bwd(DevScene S, Geom G, const 
     }
     if (overflow) atomicOr(err, overflow);
 }
+template <int PRE>
+__global__ __launch_bounds__(kBlock) void k_color_bwd(DevScene S, Geom G, const float *bg, const float *weight,
+                                                      const float *d_img, float *d_bg, float *d_translation,
+                                                      int *err) {
+    color_bwd_tile<PRE>(S, G, blockIdx.x, bg, weight, d_img, d_bg, d_translation, err);
+}
 
-__global__ __launch_bounds__(kBlock) void k_edge_sample(DevScene S, Geom G, int count, EdgeSample *edges,
-                                                        int *edge_tile, int *tile_count) {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+DVG_DEV void edge_sample_one(const DevScene &S, const Geom &G, int idx, int count, EdgeSample *edges,
+                             int *edge_tile, int *tile_count) {
     if (idx >= count) return;
-    EdgeSample es;
+    EdgeSample es{};
     int tile = -1;
-    if (make_edge_sample(S, idx, G.seed, es)) {
+    if (S.num_total_shapes > 0 && make_edge_sample(S, idx, G.seed, es)) {
         int bx = (int)(es.px * G.W), by = (int)(es.py * G.H);
         if (bx >= 0 && bx < G.W && by >= 0 && by < G.H) {
             tile = (by / G.th) * G.tiles_x + (bx / G.tw);
@@ -448,11 +462,15 @@ __global__ __launch_bounds__(kBlock) void k_edge_sample(DevScene S, Geom G, int 
     edges[idx] = es;
     edge_tile[idx] = tile;
 }
+__global__ __launch_bounds__(kBlock) void k_edge_sample(DevScene S, Geom G, int count, EdgeSample *edges,
+                                                        int *edge_tile, int *tile_count) {
+    edge_sample_one(S, G, blockIdx.x * blockDim.x + threadIdx.x, count, edges, edge_tile, tile_count);
+}
 
 // Exclusive scans, in one workgroup, of the per-tile boundary-sample counts (-> tile
 // offsets) and of the per-tile chunk counts ceil(count / kBlock) (-> chunk offsets; entry
 // n holds the total), so that k_edge can run one workgroup per 256-sample chunk.
-__global__ __launch_bounds__(1024) void k_scan(const int *in, int *out, int *cursor, int *chunk_off, int n) {
+DVG_DEV void scan_tiles(const int *in, int *out, int *cursor, int *chunk_off, int n) {
     __shared__ int s[1024];
     __shared__ int sc[1024];
     __shared__ int carry, carry_c;
@@ -487,27 +505,30 @@ __global__ __launch_bounds__(1024) void k_scan(const int *in, int *out, int *cur
     }
     if (threadIdx.x == 0) chunk_off[n] = carry_c;
 }
+__global__ __launch_bounds__(1024) void k_scan(const int *in, int *out, int *cursor, int *chunk_off, int n) {
+    scan_tiles(in, out, cursor, chunk_off, n);
+}
 
-__global__ __launch_bounds__(kBlock) void k_edge_bin(int count, const int *edge_tile, const int *tile_off,
-                                                     int *cursor, int *sorted) {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+DVG_DEV void edge_bin_one(int idx, int count, const int *edge_tile, const int *tile_off, int *cursor, int *sorted) {
     if (idx >= count) return;
     int t = edge_tile[idx];
     if (t < 0) return;
     int pos = tile_off[t] + atomicAdd(cursor + t, 1);
     sorted[pos] = idx;
 }
+__global__ __launch_bounds__(kBlock) void k_edge_bin(int count, const int *edge_tile, const int *tile_off,
+                                                     int *cursor, int *sorted) {
+    edge_bin_one(blockIdx.x * blockDim.x + threadIdx.x, count, edge_tile, tile_off, cursor, sorted);
+}
 
 // One workgroup per chunk of up to 256 boundary samples of one tile (the grid is an
 // upper bound on the chunk count; workgroups past the total exit at once), so a tile on a
 // busy boundary is spread over many workgroups instead of looping inside one.
-__global__ __launch_bounds__(kBlock) void k_edge(DevScene S, Geom G, const EdgeSample *edges, const int *tile_off,
-                                                 const int *tile_count, const int *chunk_off, int ntiles,
-                                                 const int *sorted, const float *bg, const float *weight,
-                                                 const float *d_img, float *d_translation) {
+DVG_DEV void edge_chunk(const DevScene &S, const Geom &G, int chunk, const EdgeSample *edges, const int *tile_off,
+                        const int *tile_count, const int *chunk_off, int ntiles, const int *sorted, const float *bg,
+                        const float *weight, const float *d_img, float *d_translation) {
     __shared__ int s_list[kCandCap];
     __shared__ int s_w[kBlock / 64];
-    const int chunk = blockIdx.x;
     if (chunk >= chunk_off[ntiles]) return;
     // tile = the last t with chunk_off[t] <= chunk (chunk_off is non-decreasing)
     int lo = 0, hi = ntiles - 1;
@@ -571,12 +592,18 @@ __global__ __launch_bounds__(kBlock) void k_edge(DevScene S, Geom G, const EdgeS
         }
     }
 }
+__global__ __launch_bounds__(kBlock) void k_edge(DevScene S, Geom G, const EdgeSample *edges, const int *tile_off,
+                                                 const int *tile_count, const int *chunk_off, int ntiles,
+                                                 const int *sorted, const float *bg, const float *weight,
+                                                 const float *d_img, float *d_translation) {
+    edge_chunk(S, G, blockIdx.x, edges, tile_off, tile_count, chunk_off, ntiles, sorted, bg, weight, d_img,
+               d_translation);
+}
 
 // SDF (sample_distance, diffvg.cpp:709-775) -- a linear scan over groups, as the
 // reference does; one thread per sample.
-__global__ __launch_bounds__(kBlock) void k_sdf(DevScene S, Geom G, const float *eval_positions, int num_eval,
-                                                float *sdf, const float *d_sdf, float *d_translation) {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+DVG_DEV void sdf_sample(const DevScene &S, const Geom &G, int idx, const float *eval_positions, int num_eval,
+                        float *sdf, const float *d_sdf, float *d_translation) {
     int count = eval_positions ? num_eval : G.W * G.H * G.spp;
     if (idx >= count) return;
     V2 rpt;
@@ -634,6 +661,124 @@ __global__ __launch_bounds__(kBlock) void k_sdf(DevScene S, Geom G, const float 
         }
     }
     if (sdf) aadd(eval_positions ? sdf + idx : sdf + (y * G.W + x), dist_out);
+}
+// The SDF of one pixel tile (sample_distance, diffvg.cpp:709-775), with the groups that
+// cannot be nearest culled per tile.  For a group g, lb_g = distance from the tile box to
+// g's canvas box bounds its computed distance from below at every sample of the tile;
+// for an anchored group (dvg_scene.h, g_sdf), ub_g = the largest distance from a tile
+// corner to its anchor bounds it from above.  With U = min_g ub_g, a group with lb_g > U
+// is strictly farther than the group attaining U at every sample, so it can neither win
+// nor tie; the rest are visited in the reference's order (descending group id, first
+// strict minimum kept), which gives the reference's result.  Both bounds carry a margin
+// of 1e-3 + 1e-5 * |coordinate| for rounding.
+DVG_DEV float box_dist(float4 b, float4 t) {
+    float dx = fmaxf(0.f, fmaxf(b.x - t.z, t.x - b.z));
+    float dy = fmaxf(0.f, fmaxf(b.y - t.w, t.y - b.w));
+    return sqrtf(dx * dx + dy * dy);
+}
+
+DVG_DEV void sdf_tile(const DevScene &S, const Geom &G, int tile, float *sdf, const float *d_sdf, float *d_translation) {
+    __shared__ int s_list[kCandCap];
+    __shared__ int s_w[kBlock / 64];
+    __shared__ float s_red[kBlock / 64];
+    const int tx = tile % G.tiles_x, ty = tile / G.tiles_x;
+    int x, y, sx, sy, idx;
+    bool valid = tile_sample(G, tx, ty, threadIdx.x, x, y, sx, sy, idx);
+    float4 tb = tile_box(S, G, tx, ty, 0.f);
+    const float margin = 1e-3f + 1e-5f * fmaxf(fmaxf(fabsf(tb.x), fabsf(tb.z)), fmaxf(fabsf(tb.y), fabsf(tb.w)));
+    // U = min over anchored groups of the farthest tile corner's distance to the anchor
+    float u = __builtin_huge_valf();
+    for (int g = threadIdx.x; g < S.ng; g += blockDim.x) {
+        float4 a = S.g_sdf[g];
+        if (a.z > 0.f && a.x == a.x) {
+            float ex = fmaxf(fabsf(tb.x - a.x), fabsf(tb.z - a.x)), ey = fmaxf(fabsf(tb.y - a.y), fabsf(tb.w - a.y));
+            u = fminf(u, sqrtf(ex * ex + ey * ey) + margin);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) u = fminf(u, __shfl_down(u, o, 64));
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = u;
+    __syncthreads();
+    float U = s_red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) U = fminf(U, s_red[w]);
+    V2 pt{0, 0};
+    if (valid) pt = to_canvas(S, G, sample_pos(G, x, y, sx, sy, idx, false));
+    const float inf = __builtin_huge_valf();
+    int min_group = -1, min_shape = -1;
+    float min_distance = 0.f;
+    V2 closest{0, 0};
+    PathInfo min_info{-1, -1, 0.f};
+    // candidate ranges from the top, so that the visit order stays descending
+    for (int c1 = S.ng; c1 > 0; c1 -= kCandCap) {
+        int c0 = max(0, c1 - kCandCap);
+        int total = 0;
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (int base = c0; base < c1; base += blockDim.x) {
+            int g = base + threadIdx.x;
+            bool hit = false;
+            if (g < c1) {
+                float4 a = S.g_sdf[g];
+                hit = a.z > 0.f && box_dist(S.g_box[g], tb) - margin <= U;
+            }
+            uint64_t m = __ballot(hit);
+            int pre = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) s_w[wid] = __popcll(m);
+            __syncthreads();
+            int off = total, chunk = 0;
+            for (int w = 0; w < nw; w++) {
+                if (w < wid) off += s_w[w];
+                chunk += s_w[w];
+            }
+            if (hit) s_list[off + pre] = g;
+            total += chunk;
+            __syncthreads();
+        }
+        if (valid) {
+            for (int i = total - 1; i >= 0; i--) {
+                int g = s_list[i];
+                int sh = -1;
+                V2 p{0, 0};
+                PathInfo li{-1, -1, 0.f};
+                float d = inf;
+                if (compute_distance(S, g, pt, inf, sh, p, li, d)) {
+                    if (min_group == -1 || d < min_distance) {
+                        min_distance = d;
+                        min_group = g;
+                        min_shape = sh;
+                        closest = p;
+                        min_info = li;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!valid) return;
+    float weight = 1.f / G.spp;
+    float dist_out = min_distance;
+    if (min_group >= 0) {
+        dist_out = min_distance * weight;
+        bool in = false;
+        if (S.fill_type[min_group] >= 0) {
+            M3 c2s = m3_load(S.c2s + 9 * min_group);
+            in = is_inside_local(S, min_group, xform_pt(c2s, pt), nullptr);
+            if (in) dist_out = -dist_out;
+        }
+        if (d_sdf) {
+            float dd = d_sdf[y * G.W + x];
+            float d_abs = in ? -dd : dd;
+            float *d_tr = d_translation ? d_translation + 2 * (y * G.W + x) : nullptr;
+            d_compute_distance(S, min_group, min_shape, pt, closest, min_info, d_abs, d_tr);
+        }
+    }
+    if (sdf) aadd(sdf + (y * G.W + x), dist_out);
+}
+__global__ __launch_bounds__(kBlock) void k_sdf_tiled(DevScene S, Geom G, float *sdf, const float *d_sdf,
+                                                      float *d_translation) {
+    sdf_tile(S, G, blockIdx.x, sdf, d_sdf, d_translation);
+}
+__global__ __launch_bounds__(kBlock) void k_sdf(DevScene S, Geom G, const float *eval_positions, int num_eval,
+                                                float *sdf, const float *d_sdf, float *d_translation) {
+    sdf_sample(S, G, blockIdx.x * blockDim.x + threadIdx.x, eval_positions, num_eval, sdf, d_sdf, d_translation);
 }
 
 // ------------------------------------------------------------------ PLAIN kernels
@@ -864,6 +1009,8 @@ struct Scope {
     Rec r{};
     bool on;
     Scope(hipStream_t s, const char *name) : st(s), on(enabled) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (on && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) on = false;
         if (!on) return;
         r.name = name;
         (void)hipEventCreate(&r.a);
@@ -1077,11 +1224,14 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
         }
     }
     if (sdf_out) {
-        int count = eval_mode ? a.num_eval_positions : (int)nsamp;
         DVG_PROF(st, "k_sdf");
-        k_sdf<<<(count + kBlock - 1) / kBlock, kBlock, 0, st>>>(S, G, eval_mode ? a.eval_positions : nullptr,
-                                                                 a.num_eval_positions, a.render_sdf, a.d_render_sdf,
-                                                                 a.d_translation);
+        if (eval_mode) {
+            int count = a.num_eval_positions;
+            k_sdf<<<(count + kBlock - 1) / kBlock, kBlock, 0, st>>>(S, G, a.eval_positions, a.num_eval_positions,
+                                                                     a.render_sdf, a.d_render_sdf, a.d_translation);
+        } else {
+            k_sdf_tiled<<<ntiles, kBlock, 0, st>>>(S, G, a.render_sdf, a.d_render_sdf, a.d_translation);
+        }
     }
     DVG_CHECK(hipGetLastError());
     if (a.d_render_image || pre) return check_err_flag(sc, st);
@@ -1175,6 +1325,414 @@ int render_plain(SceneHost &sc, const dvg_render_args &a) {
     DVG_CHECK(hipStreamSynchronize(st));
     if (weight) DVG_CHECK(hipFree(weight));
     return check_err_flag(sc, st);
+}
+
+
+// ------------------------------------------------------------------ BATCHED launches
+// Many (scene, render args) jobs rendered with one kernel per stage over the work items
+// of all jobs.  The job table lives in HBM; each stage has a prefix array of its
+// workgroup counts per job, and a workgroup finds its job by binary search (uniform per
+// workgroup, so the job's fields are read with scalar loads).  The workspace is carved
+// when the batch is created, so dvg_batch_render allocates nothing, waits for nothing
+// and can be captured into a HIP graph.
+struct Job {
+    DevScene S;
+    Geom G;
+    const float *bg;
+    float *img;
+    const float *d_img;
+    float *d_bg;
+    float *d_tr;
+    float *sdf;
+    const float *d_sdf;
+    const float *eval;
+    int num_eval;
+    int pre;
+    float *weight;
+    float4 *colors;
+    EdgeSample *edges;
+    int *edge_tile, *edge_sorted, *tile_count, *tile_off, *tile_cursor, *chunk_off;
+    int *err;
+    float *grad;
+    int grad_n;
+    int ntiles, npix, nsamp;
+};
+
+enum { kStPix = 0, kStTile, kStSamp, kStChunk, kStSdf, kStZero, kNumStages };
+
+// prefix arrays: pref[stage * (nj + 1) + k] = first workgroup of job k in that stage
+DVG_DEV int find_job(const int *pref, int nj, int blk) {
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (pref[mid] <= blk)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void kb_zero(const Job *jobs, const int *pref, int nj, int zero_grads) {
+    int j = find_job(pref, nj, blockIdx.x);
+    const Job &J = jobs[j];
+    int b = blockIdx.x - pref[j], nb = pref[j + 1] - pref[j];
+    if (b == 0 && threadIdx.x == 0) *J.err = 0;
+    if (J.d_img && J.tile_count)
+        for (int i = b * kBlock + threadIdx.x; i < J.ntiles; i += nb * kBlock) J.tile_count[i] = 0;
+    if (zero_grads && (J.d_img || J.d_sdf))
+        for (int i = b * kBlock + threadIdx.x; i < J.grad_n; i += nb * kBlock) J.grad[i] = 0.f;
+}
+
+__global__ __launch_bounds__(kBlock) void kb_weight(const Job *jobs, const int *pref, int nj) {
+    int j = find_job(pref, nj, blockIdx.x);
+    const Job &J = jobs[j];
+    weight_pixel(J.S, J.G, (blockIdx.x - pref[j]) * kBlock + threadIdx.x, J.weight);
+}
+
+template <int PRE>
+__global__ __launch_bounds__(kBlock) void kb_color(const Job *jobs, const int *pref, int nj) {
+    int j = find_job(pref, nj, blockIdx.x);
+    const Job &J = jobs[j];
+    if (!J.img) return;
+    color_tile<PRE>(J.S, J.G, blockIdx.x - pref[j], J.bg, J.colors, J.err);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_resolve(const Job *jobs, const int *pref, int nj) {
+    int j = find_job(pref, nj, blockIdx.x);
+    const Job &J = jobs[j];
+    if (!J.img) return;
+    resolve_pixel(J.S, J.G, (blockIdx.x - pref[j]) * kBlock + threadIdx.x, J.pre, J.weight, J.colors, J.img);
+}
+
+template <int PRE>
+__global__ __launch_bounds__(kBlock) void kb_color_bwd(const Job *jobs, const int *pref, int nj) {
+    int j = find_job(pref, nj, blockIdx.x);
+    const Job &J = jobs[j];
+    if (!J.d_img) return;
+    color_bwd_tile<PRE>(J.S, J.G, blockIdx.x - pref[j], J.bg, J.weight, J.d_img, J.d_bg, J.d_tr, J.err);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_edge_sample(const Job *jobs, const int *pref, int nj) {
+    int j = find_job(pref, nj, blockIdx.x);
+    const Job &J = jobs[j];
+    if (!J.d_img) return;
+    edge_sample_one(J.S, J.G, (blockIdx.x - pref[j]) * kBlock + threadIdx.x, J.nsamp, J.edges, J.edge_tile,
+                    J.tile_count);
+}
+
+__global__ __launch_bounds__(1024) void kb_scan(const Job *jobs, int j0) {
+    const Job &J = jobs[j0 + blockIdx.x];
+    if (!J.d_img) return;
+    scan_tiles(J.tile_count, J.tile_off, J.tile_cursor, J.chunk_off, J.ntiles);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_edge_bin(const Job *jobs, const int *pref, int nj) {
+    int j = find_job(pref, nj, blockIdx.x);
+    const Job &J = jobs[j];
+    if (!J.d_img) return;
+    edge_bin_one((blockIdx.x - pref[j]) * kBlock + threadIdx.x, J.nsamp, J.edge_tile, J.tile_off, J.tile_cursor,
+                 J.edge_sorted);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_edge(const Job *jobs, const int *pref, int nj) {
+    int j = find_job(pref, nj, blockIdx.x);
+    const Job &J = jobs[j];
+    if (!J.d_img) return;
+    edge_chunk(J.S, J.G, blockIdx.x - pref[j], J.edges, J.tile_off, J.tile_count, J.chunk_off, J.ntiles,
+               J.edge_sorted, J.bg, J.weight, J.d_img, J.d_tr);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_sdf(const Job *jobs, const int *pref, int nj) {
+    int j = find_job(pref, nj, blockIdx.x);
+    const Job &J = jobs[j];
+    if (J.eval)
+        sdf_sample(J.S, J.G, (blockIdx.x - pref[j]) * kBlock + threadIdx.x, J.eval, J.num_eval, J.sdf, J.d_sdf,
+                   J.d_tr);
+    else
+        sdf_tile(J.S, J.G, blockIdx.x - pref[j], J.sdf, J.d_sdf, J.d_tr);
+}
+
+struct BatchHost {
+    int device = 0;
+    int n = 0;
+    int zero_grads = 0;
+    std::vector<SceneHost *> scenes;
+    std::vector<Job> jobs;         // host copy, in class order
+    std::vector<int> order;        // jobs[k] renders item order[k]
+    // classes: [0] colour without prefiltering, [1] colour prefiltered, [2] SDF
+    int cls_begin[4] = {0, 0, 0, 0};
+    bool any_img[3] = {false, false, false}, any_dimg[3] = {false, false, false};
+    std::vector<int> pref;         // per class, per stage: (njobs_in_class + 1) entries
+    std::vector<size_t> pref_off;  // offset into pref of (class, stage)
+    std::vector<int> total;        // workgroups of (class, stage)
+    void *dev_block = nullptr;     // job table + prefix arrays
+    size_t dev_bytes = 0, dev_granted = 0;
+    void *ws = nullptr;            // per-job scratch
+    size_t ws_bytes = 0, ws_granted = 0;
+    hipStream_t last_stream = nullptr;
+};
+
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int batch_create(SceneHost *const *scenes, const dvg_render_args *args, int n, int zero_grads, BatchHost **out) {
+    if (n <= 0) return set_error("a batch needs at least one job");
+    auto *B = new (std::nothrow) BatchHost();
+    if (!B) return set_error("out of host memory");
+    std::unique_ptr<BatchHost> guard(B);
+    B->n = n;
+    B->zero_grads = zero_grads;
+    B->device = scenes[0]->device;
+    // classify
+    std::vector<int> cls(n);
+    for (int i = 0; i < n; i++) {
+        const dvg_render_args &a = args[i];
+        if (!scenes[i]) return set_error("null scene in batch");
+        if (scenes[i]->device != B->device) return set_error("all scenes of a batch must live on one GPU");
+        if (a.width <= 0 || a.height <= 0 || a.num_samples_x <= 0 || a.num_samples_y <= 0)
+            return set_error("width, height and sample counts must be positive");
+        if (a.num_samples_x * a.num_samples_y > kBlock)
+            return set_error("batched renders support at most 256 samples per pixel");
+        if ((int64_t)a.width * a.height * a.num_samples_x * a.num_samples_y > (int64_t)0x7fffffff)
+            return set_error("too many samples for one render");
+        bool color = a.render_image || a.d_render_image;
+        bool sdf = a.render_sdf || a.d_render_sdf;
+        if (color == sdf) return set_error("a batch job renders either colour or SDF");
+        bool eval = a.eval_positions && a.num_eval_positions > 0;
+        if (color && eval) return set_error("eval_positions are supported for the SDF output only");
+        cls[i] = sdf ? 2 : (a.use_prefiltering ? 1 : 0);
+    }
+    for (int c = 0; c < 3; c++)
+        for (int i = 0; i < n; i++)
+            if (cls[i] == c) B->order.push_back(i);
+    for (int c = 0, k = 0; c < 3; c++) {
+        B->cls_begin[c] = k;
+        for (int i = 0; i < n; i++) k += cls[i] == c;
+        B->cls_begin[c + 1] = k;
+    }
+    // workspace layout
+    std::vector<size_t> o_weight(n), o_colors(n), o_edges(n), o_etile(n), o_esort(n), o_tcount(n), o_toff(n),
+        o_tcur(n), o_chunk(n);
+    size_t ws = 0;
+    B->jobs.resize(n);
+    for (int k = 0; k < n; k++) {
+        int i = B->order[k];
+        SceneHost &sc = *scenes[i];
+        const dvg_render_args &a = args[i];
+        Job &J = B->jobs[k];
+        std::memset(&J, 0, sizeof(Job));
+        J.S = sc.dev;
+        J.G = make_geom(sc, a);
+        J.bg = a.background_image;
+        J.img = a.render_image;
+        J.d_img = a.d_render_image;
+        J.d_bg = a.d_background_image;
+        J.d_tr = a.d_translation;
+        J.sdf = a.render_sdf;
+        J.d_sdf = a.d_render_sdf;
+        bool eval = a.eval_positions && a.num_eval_positions > 0;
+        J.eval = eval ? a.eval_positions : nullptr;
+        J.num_eval = eval ? a.num_eval_positions : 0;
+        J.pre = a.use_prefiltering ? 1 : 0;
+        J.err = sc.err;
+        J.grad = sc.grad;
+        J.grad_n = (int)sc.grad_floats;
+        J.npix = J.G.W * J.G.H;
+        J.nsamp = J.npix * J.G.spp;
+        J.ntiles = J.G.tiles_x * J.G.tiles_y;
+        if (cls[i] == 2) continue;
+        o_weight[k] = ws, ws += al256((size_t)J.npix * 4);
+        if (J.img) o_colors[k] = ws, ws += al256((size_t)J.nsamp * 16);
+        if (J.d_img && !J.pre) {
+            o_edges[k] = ws, ws += al256((size_t)J.nsamp * sizeof(EdgeSample));
+            o_etile[k] = ws, ws += al256((size_t)J.nsamp * 4);
+            o_esort[k] = ws, ws += al256((size_t)J.nsamp * 4);
+            o_tcount[k] = ws, ws += al256((size_t)J.ntiles * 4);
+            o_toff[k] = ws, ws += al256((size_t)J.ntiles * 4);
+            o_tcur[k] = ws, ws += al256((size_t)J.ntiles * 4);
+            o_chunk[k] = ws, ws += al256((size_t)(J.ntiles + 1) * 4);
+        }
+    }
+    // prefix arrays per class and stage
+    // workgroups of a job in each stage (0 when the job has no work there)
+    auto blocks = [](const Job &J, int stage) -> int {
+        bool color = J.img || J.d_img, edges = J.d_img && !J.pre, sdf = J.sdf || J.d_sdf;
+        switch (stage) {
+            case kStPix: return color ? (J.npix + kBlock - 1) / kBlock : 0;
+            case kStTile: return color ? J.ntiles : 0;
+            case kStSamp: return edges ? (J.nsamp + kBlock - 1) / kBlock : 0;
+            case kStChunk: return edges ? J.ntiles + (J.nsamp + kBlock - 1) / kBlock : 0;
+            case kStSdf: return sdf ? (J.eval ? (J.num_eval + kBlock - 1) / kBlock : J.ntiles) : 0;
+            default: return std::max(1, std::min(64, (std::max(J.grad_n, J.ntiles) + kBlock - 1) / kBlock));
+        }
+    };
+    for (int c = 0; c < 3; c++)
+        for (int k = B->cls_begin[c]; k < B->cls_begin[c + 1]; k++) {
+            B->any_img[c] |= B->jobs[k].img != nullptr;
+            B->any_dimg[c] |= B->jobs[k].d_img != nullptr;
+        }
+    B->pref_off.assign(3 * kNumStages, 0);
+    B->total.assign(3 * kNumStages, 0);
+    for (int c = 0; c < 3; c++) {
+        int j0 = B->cls_begin[c], nj = B->cls_begin[c + 1] - j0;
+        for (int st = 0; st < kNumStages; st++) {
+            B->pref_off[c * kNumStages + st] = B->pref.size();
+            int acc = 0;
+            for (int k = 0; k < nj; k++) {
+                B->pref.push_back(acc);
+                acc += blocks(B->jobs[j0 + k], st);
+            }
+            B->pref.push_back(acc);
+            B->total[c * kNumStages + st] = acc;
+        }
+    }
+    B->scenes.assign(scenes, scenes + n);
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    if (prev != B->device) (void)hipSetDevice(B->device);
+    auto restore = [&]() {
+        if (prev >= 0 && prev != B->device) (void)hipSetDevice(prev);
+    };
+    // both blocks come from the per-device cache, so batches created and destroyed in a
+    // serving loop neither allocate nor free device memory (hipFree would wait for the
+    // whole device)
+    size_t jbytes = al256(sizeof(Job) * n);
+    B->dev_bytes = jbytes + al256(B->pref.size() * sizeof(int));
+    B->dev_block = cache_alloc(B->device, B->dev_bytes, &B->dev_granted);
+    if (!B->dev_block) {
+        restore();
+        return set_error("out of device memory for the batch table");
+    }
+    if (ws) {
+        B->ws = cache_alloc(B->device, ws, &B->ws_granted);
+        if (!B->ws) {
+            cache_free(B->device, B->dev_block, B->dev_granted);
+            restore();
+            return set_error("out of device memory for the batch workspace");
+        }
+    }
+    B->ws_bytes = ws;
+    uint8_t *W = (uint8_t *)B->ws;
+    for (int k = 0; k < n; k++) {
+        Job &J = B->jobs[k];
+        if (cls[B->order[k]] == 2) continue;
+        J.weight = (float *)(W + o_weight[k]);
+        if (J.img) J.colors = (float4 *)(W + o_colors[k]);
+        if (J.d_img && !J.pre) {
+            J.edges = (EdgeSample *)(W + o_edges[k]);
+            J.edge_tile = (int *)(W + o_etile[k]);
+            J.edge_sorted = (int *)(W + o_esort[k]);
+            J.tile_count = (int *)(W + o_tcount[k]);
+            J.tile_off = (int *)(W + o_toff[k]);
+            J.tile_cursor = (int *)(W + o_tcur[k]);
+            J.chunk_off = (int *)(W + o_chunk[k]);
+        }
+    }
+    std::vector<uint8_t> host(B->dev_bytes, 0);
+    std::memcpy(host.data(), B->jobs.data(), sizeof(Job) * n);
+    std::memcpy(host.data() + jbytes, B->pref.data(), B->pref.size() * sizeof(int));
+    int rc = upload(B->device, B->dev_block, host.data(), B->dev_bytes);
+    restore();
+    if (rc) {
+        cache_free(B->device, B->dev_block, B->dev_granted);
+        if (B->ws) cache_free(B->device, B->ws, B->ws_granted);
+        return rc;
+    }
+    *out = guard.release();
+    return 0;
+}
+
+int batch_render(BatchHost &B, hipStream_t st) {
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    if (prev != B.device) (void)hipSetDevice(B.device);
+    const Job *jobs = (const Job *)B.dev_block;
+    const int *pref_base = (const int *)((uint8_t *)B.dev_block + al256(sizeof(Job) * B.n));
+    for (int c = 0; c < 3; c++) {
+        int j0 = B.cls_begin[c], nj = B.cls_begin[c + 1] - j0;
+        if (nj == 0) continue;
+        const Job *J = jobs + j0;
+        auto P = [&](int stage) { return pref_base + B.pref_off[c * kNumStages + stage]; };
+        auto T = [&](int stage) { return (unsigned)B.total[c * kNumStages + stage]; };
+        kb_zero<<<T(kStZero), kBlock, 0, st>>>(J, P(kStZero), nj, B.zero_grads);
+        if (c == 2) {
+            DVG_PROF(st, "kb_sdf");
+            kb_sdf<<<T(kStSdf), kBlock, 0, st>>>(J, P(kStSdf), nj);
+            continue;
+        }
+        {
+            DVG_PROF(st, "kb_weight");
+            kb_weight<<<T(kStPix), kBlock, 0, st>>>(J, P(kStPix), nj);
+        }
+        if (B.any_img[c]) {
+            {
+                DVG_PROF(st, "kb_color");
+                if (c == 1)
+                    kb_color<1><<<T(kStTile), kBlock, 0, st>>>(J, P(kStTile), nj);
+                else
+                    kb_color<0><<<T(kStTile), kBlock, 0, st>>>(J, P(kStTile), nj);
+            }
+            kb_resolve<<<T(kStPix), kBlock, 0, st>>>(J, P(kStPix), nj);
+        }
+        if (B.any_dimg[c]) {
+            {
+                DVG_PROF(st, "kb_color_bwd");
+                if (c == 1)
+                    kb_color_bwd<1><<<T(kStTile), kBlock, 0, st>>>(J, P(kStTile), nj);
+                else
+                    kb_color_bwd<0><<<T(kStTile), kBlock, 0, st>>>(J, P(kStTile), nj);
+            }
+            if (c == 0) {
+                {
+                    DVG_PROF(st, "kb_edge_sample");
+                    kb_edge_sample<<<T(kStSamp), kBlock, 0, st>>>(J, P(kStSamp), nj);
+                    kb_scan<<<(unsigned)nj, 1024, 0, st>>>(jobs, j0);
+                    kb_edge_bin<<<T(kStSamp), kBlock, 0, st>>>(J, P(kStSamp), nj);
+                }
+                {
+                    DVG_PROF(st, "kb_edge");
+                    kb_edge<<<T(kStChunk), kBlock, 0, st>>>(J, P(kStChunk), nj);
+                }
+            }
+        }
+    }
+    hipError_t e = hipGetLastError();
+    B.last_stream = st;
+    if (prev >= 0 && prev != B.device) (void)hipSetDevice(prev);
+    if (e != hipSuccess) return set_error(std::string("batch launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int batch_status(BatchHost &B) {
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    if (prev != B.device) (void)hipSetDevice(B.device);
+    DVG_CHECK(hipStreamSynchronize(B.last_stream));
+    int flags = 0;
+    for (auto *sc : B.scenes) {
+        int h = 0;
+        DVG_CHECK(hipMemcpyAsync(&h, sc->err, sizeof(int), hipMemcpyDeviceToHost, B.last_stream));
+        DVG_CHECK(hipStreamSynchronize(B.last_stream));
+        flags |= h;
+    }
+    if (prev >= 0 && prev != B.device) (void)hipSetDevice(prev);
+    if (flags & 2) return set_error("prefiltered stroke of a group without a path or rect shape (the reference asserts)");
+    if (flags & 1) return set_error("fragment list overflow: more overlapping fragments than the reference supports");
+    return 0;
+}
+
+const std::vector<SceneHost *> &batch_scenes(const BatchHost &b) { return b.scenes; }
+
+void batch_destroy(BatchHost *B) {
+    if (!B) return;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(B->device);
+    (void)hipStreamSynchronize(B->last_stream);
+    if (B->dev_block) cache_free(B->device, B->dev_block, B->dev_granted);
+    if (B->ws) cache_free(B->device, B->ws, B->ws_granted);
+    if (prev >= 0) (void)hipSetDevice(prev);
+    delete B;
 }
 
 }  // namespace dvg

@@ -74,6 +74,11 @@ struct DevScene {
     const float4 *g_lbox;      // local root box of the group BVH
     const float4 *g_box;       // canvas-space top-level leaf box
     const float *g_maxr;       // top-level leaf max_radius
+    // SDF culling per group: (x, y) = a canvas point of the boundary whose distance
+    // bounds the group's computed distance from above (NaN when no such bound holds:
+    // non-similarity transform or approximate quadratic distances), z = 1 when the group
+    // can report a distance at all (it has a path with segments or a rect)
+    const float4 *g_sdf;
     // top-level BVH [2 ng - 1]
     const BNode *top;
     // edge sampling

@@ -143,6 +143,29 @@ int dvg_scene_zero_grads(dvg_scene *scene, void *stream);
 /* Selects DVG_MODE_TILED (default) or DVG_MODE_PLAIN for later renders. */
 int dvg_scene_set_mode(dvg_scene *scene, int mode);
 
+/* Batched render: many (scene, render args) jobs -- scenes of any size, resolution,
+ * sample count, output (colour or SDF) and prefiltering -- rendered with one kernel
+ * per stage over the work items of all jobs.  It replaces the reference's loop of one
+ * render(...) call per scene (diffvg.cpp:1477-1649 called per item, as
+ * apps/generative_models/rendering.py:262-290 does); each job has the semantics of
+ * dvg_render(scene, &args[i]) in DVG_MODE_TILED (at most 256 samples per pixel).
+ *
+ * dvg_batch_create validates the jobs, uploads the job table and reserves the
+ * workspace; the scenes and every buffer named in `args` must stay alive and in place
+ * until the batch is destroyed.  dvg_batch_render enqueues the kernels on `stream`: it
+ * allocates nothing and waits for nothing, so it may be captured into a HIP graph and
+ * replayed; with DVG_BATCH_ZERO_GRADS it first clears the gradients of the scenes with
+ * backward jobs.  dvg_batch_status waits for the last render and reports the
+ * conditions the reference asserts on (fragment overflow, prefiltered circle strokes).
+ * Jobs of one batch that share a scene, or renders of the batch's scenes outside it,
+ * must not overlap in time. */
+#define DVG_BATCH_ZERO_GRADS 1
+typedef struct dvg_batch dvg_batch;
+int dvg_batch_create(dvg_scene *const *scenes, const dvg_render_args *args, int n, int flags, dvg_batch **out);
+int dvg_batch_render(dvg_batch *batch, void *stream);
+int dvg_batch_status(dvg_batch *batch);
+int dvg_batch_destroy(dvg_batch *batch);
+
 /* Kernel timing: when enabled, each kernel launch is bracketed by HIP events recorded
  * on its stream.  dvg_profile_read waits for the events and returns the number of
  * kernels, their names (newline separated) and summed milliseconds and launch counts,

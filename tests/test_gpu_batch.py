@@ -1,0 +1,212 @@
+"""Batched rendering (dvg_batch_*): many scenes of different sizes, sample counts,
+filters, outputs and options in one launch per stage, checked job by job against
+single-scene renders of the same scenes (same device code, so forward images are
+bit-identical and gradients agree up to atomic summation order) and against the oracle;
+and replayed from a captured graph."""
+import numpy as np
+import pytest
+import torch
+
+from diffvg_amd import _lib
+from diffvg_amd.batch import Batch
+import scenes
+from refdrive import OracleRunner
+from test_oracle_pins import GRAD_KEYS, _rel
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device('cuda:0')
+
+# (case, filter override, options)
+SPECS = [
+    ('single_circle_64', None, dict(nsx=2, nsy=2, seed=1)),
+    ('curve_fill', None, dict(nsx=3, nsy=2, seed=2, prefilter=True)),
+    ('strokes', ('gaussian', 1.5), dict(nsx=2, nsy=2, seed=3)),
+    ('overlap_eo', ('tent', 1.0), dict(nsx=1, nsy=1, seed=4, background=True)),
+    ('gradients', None, dict(nsx=2, nsy=2, seed=5, size=(40, 24))),
+    ('mixed_segments', None, dict(nsx=2, nsy=2, seed=6, prefilter=True, background=True)),
+    ('thick_stroke', None, dict(nsx=4, nsy=4, seed=7)),
+    ('transformed', None, dict(sdf=True)),
+    ('curve_fill', None, dict(sdf=True, eval=200)),
+    ('primitives', ('hann', 1.0), dict(nsx=2, nsy=3, seed=8, size=(70, 33))),
+]
+
+
+def _inputs(f, opt, k):
+    w, h = opt.get('size', (f['canvas_width'], f['canvas_height']))
+    rng = np.random.default_rng(100 + k)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(DEV)
+    bg = t(rng.uniform(0, 1, (h, w, 4))) if opt.get('background') else None
+    if opt.get('sdf'):
+        ev = t(rng.uniform(0, min(w, h), (opt['eval'], 2))) if opt.get('eval') else None
+        n = opt['eval'] if opt.get('eval') else None
+        d = t(rng.uniform(-1, 1, (n, 1) if n else (h, w, 1)))
+    else:
+        ev = None
+        d = t(rng.uniform(-1, 1, (h, w, 4)))
+    return w, h, bg, ev, d
+
+
+def _fwd_kw(f, opt, k, out):
+    w, h, bg, ev, d = _inputs(f, opt, k)
+    if opt.get('sdf'):
+        kw = dict(width=w, height=h, nsx=1, nsy=1, seed=0, render_sdf=out, eval_positions=ev,
+                  num_eval_positions=0 if ev is None else ev.shape[0])
+    else:
+        kw = dict(width=w, height=h, nsx=opt['nsx'], nsy=opt['nsy'], seed=opt['seed'], render_image=out,
+                  background=bg, use_prefiltering=opt.get('prefilter', False))
+    return kw, (bg, ev, d)
+
+
+def _bwd_kw(f, opt, k, d_tr, d_bg):
+    w, h, bg, ev, d = _inputs(f, opt, k)
+    if opt.get('sdf'):
+        kw = dict(width=w, height=h, nsx=1, nsy=1, seed=0, d_render_sdf=d, eval_positions=ev,
+                  num_eval_positions=0 if ev is None else ev.shape[0])
+    else:
+        kw = dict(width=w, height=h, nsx=opt['nsx'], nsy=opt['nsy'], seed=opt['seed'], d_render_image=d,
+                  background=bg, d_background=d_bg if bg is not None else None, d_translation=d_tr,
+                  use_prefiltering=opt.get('prefilter', False))
+    return kw, (bg, ev, d)
+
+
+def _outputs(f, opt):
+    w, h = opt.get('size', (f['canvas_width'], f['canvas_height']))
+    if opt.get('sdf'):
+        out = torch.zeros((opt['eval'], 1) if opt.get('eval') else (h, w, 1), device=DEV)
+        return out, None, None
+    return (torch.zeros(h, w, 4, device=DEV), torch.zeros(h, w, 2, device=DEV),
+            torch.zeros(h, w, 4, device=DEV))
+
+
+def _flats():
+    return [scenes.flat_case(c, fl) for c, fl, _ in SPECS]
+
+
+def _single(flats):
+    res = []
+    for k, (f, (_, _, opt)) in enumerate(zip(flats, SPECS)):
+        sc = _lib.DeviceScene(f, 0)
+        out, d_tr, d_bg = _outputs(f, opt)
+        kw, keep = _fwd_kw(f, opt, k, out)
+        sc.render(**kw)
+        sc.zero_grads(None)
+        kw, keep2 = _bwd_kw(f, opt, k, d_tr, d_bg)
+        sc.render(**kw)
+        torch.cuda.synchronize()
+        res.append(dict(out=out.cpu().numpy(), grads=sc.grads(),
+                        d_tr=None if d_tr is None else d_tr.cpu().numpy(),
+                        d_bg=None if d_bg is None else d_bg.cpu().numpy()))
+    return res
+
+
+def _batched(flats, graph=False, replays=1):
+    scs = [_lib.DeviceScene(f, 0) for f in flats]
+    outs, fj, bj, keep = [], [], [], []
+    for k, (f, (_, _, opt)) in enumerate(zip(flats, SPECS)):
+        out, d_tr, d_bg = _outputs(f, opt)
+        kw, kp = _fwd_kw(f, opt, k, out)
+        fj.append((scs[k], kw))
+        kw2, kp2 = _bwd_kw(f, opt, k, d_tr, d_bg)
+        bj.append((scs[k], kw2))
+        outs.append((out, d_tr, d_bg))
+        keep += [kp, kp2]
+    fb, bb = Batch(fj, zero_grads=False), Batch(bj, zero_grads=True)
+
+    def run(stream):
+        for out, d_tr, d_bg in outs:
+            out.zero_()
+            if d_tr is not None:
+                d_tr.zero_()
+                d_bg.zero_()
+        fb.render(stream)
+        bb.render(stream)
+
+    results = []
+    if not graph:
+        run(None)
+        fb.status()
+        bb.status()
+        replays = 1
+    else:
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            run(s.cuda_stream)  # warm-up outside the capture
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            run(torch.cuda.current_stream().cuda_stream)
+    for r in range(replays):
+        if graph:
+            junk = torch.randn(1 << 16, device=DEV).sum()  # eager work between replays
+            g.replay()
+            torch.cuda.synchronize()
+            fb.status()
+            bb.status()
+            del junk
+        results.append([dict(out=o.cpu().numpy(), grads=sc.grads(),
+                             d_tr=None if t is None else t.cpu().numpy(),
+                             d_bg=None if b is None else b.cpu().numpy())
+                        for sc, (o, t, b) in zip(scs, outs)])
+    return results
+
+
+def _same(a, b, label, exact=True):
+    if exact:
+        assert np.array_equal(a['out'], b['out']), label
+    else:
+        assert np.abs(a['out'] - b['out']).max() <= 1e-6, label
+    for k in GRAD_KEYS:
+        assert _rel(a['grads'][k], b['grads'][k]) <= 1e-5, (label, k)
+    for k in ('d_tr', 'd_bg'):
+        if b[k] is not None:
+            assert _rel(a[k], b[k]) <= 1e-5, (label, k)
+
+
+def test_batch_matches_single_scene_renders():
+    flats = _flats()
+    single = _single(flats)
+    batched = _batched(flats)[0]
+    for k, (a, b) in enumerate(zip(batched, single)):
+        print('job', k, SPECS[k][0], SPECS[k][2], flush=True)
+        _same(a, b, (k, SPECS[k][0]))
+
+
+def test_batch_matches_oracle():
+    flats = _flats()
+    batched = _batched(flats)[0]
+    for k, (f, (case, _, opt)) in enumerate(zip(flats, SPECS)):
+        if opt.get('sdf') or opt.get('background'):
+            continue
+        w, h = opt.get('size', (f['canvas_width'], f['canvas_height']))
+        o = OracleRunner(f)
+        ref = o.render(w, h, opt['nsx'], opt['nsy'], opt['seed'], prefilter=opt.get('prefilter', False))['image']
+        assert np.abs(batched[k]['out'] - ref).max() <= 1e-5, case
+        d = _inputs(f, opt, k)[4].cpu().numpy()
+        g = o.render(w, h, opt['nsx'], opt['nsy'], opt['seed'], prefilter=opt.get('prefilter', False), d_image=d,
+                     translation=True)
+        for key in GRAD_KEYS:
+            assert _rel(batched[k]['grads'][key], g['grads'][key]) <= 1e-3, (case, key)
+
+
+def test_batch_graph_replay():
+    """Forward and backward batches captured in one graph, replayed three times with
+    eager work in between; every replay equals the eager batch."""
+    flats = _flats()
+    eager = _batched(flats)[0]
+    reps = _batched(flats, graph=True, replays=3)
+    for r, res in enumerate(reps):
+        for k, (a, b) in enumerate(zip(res, eager)):
+            _same(a, b, ('replay', r, k), exact=True)
+
+
+def test_batch_rejects_invalid_jobs():
+    f = scenes.flat_case('single_circle_64')
+    sc = _lib.DeviceScene(f, 0)
+    out = torch.zeros(8, 8, 4, device=DEV)
+    with pytest.raises(RuntimeError, match='256 samples'):
+        Batch([(sc, dict(width=8, height=8, nsx=17, nsy=16, seed=0, render_image=out))])
+    with pytest.raises(RuntimeError, match='either colour or SDF'):
+        Batch([(sc, dict(width=8, height=8, nsx=1, nsy=1, seed=0))])

@@ -339,3 +339,25 @@ def test_drop_in_diffvg_module_matches_reference_fixture(name):
     print('shim', name, flush=True)
     flat, z = load_fixture(name)
     check_against_fixture(_runner_fn(ShimRunner(flat)), flat, z, 'shim/' + name)
+
+
+@pytest.mark.parametrize('case', ['sdf_many', 'primitives', 'transformed', 'quad_approx'])
+def test_gpu_sdf_tile_culling_vs_oracle(case):
+    """The tiled SDF culls groups per tile by distance bounds; the oracle scans every group
+    (sample_distance's linear loop).  Results and gradients must agree, including scenes
+    with rects, circles, a non-uniform transform and approximate quadratic distances."""
+    print('sdf', case, flush=True)
+    if case == 'sdf_many':
+        import diffvg_amd as pd_
+        rng = np.random.default_rng(21)
+        shapes, groups = workloads.closed_paths(rng, 300, 128.0, 2, 8, 8.0, 'cpu', stroke_frac=0.3)
+        f = flatten(128, 128, shapes, groups, 0, 0.5)
+    else:
+        f = scenes.flat_case(case)
+    w, h = f['canvas_width'], f['canvas_height']
+    o, g = OracleRunner(f), GpuRunner(f)
+    _compare(g.render(w, h, 1, 1, 0, output='sdf'), o.render(w, h, 1, 1, 0, output='sdf'), (case, 'sdf'))
+    d = np.random.default_rng(5).uniform(-1, 1, (h, w, 1)).astype(np.float32)
+    _compare(g.render(w, h, 1, 1, 0, output='sdf', d_sdf=d, translation=True),
+             o.render(w, h, 1, 1, 0, output='sdf', d_sdf=d, translation=True), (case, 'sdf bwd'))
+    _compare(g.render(w, h, 2, 2, 3, output='sdf'), o.render(w, h, 2, 2, 3, output='sdf'), (case, 'sdf 2x2'))
