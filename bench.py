@@ -193,6 +193,14 @@ def bench_batched(a):
         return args, fl, _lib.DeviceScene(fl.desc, 0)
 
     built = [build(i) for i in range(n)]
+    tuned = {}
+    if a.autotune:
+        # tile width per workload signature, timed once per signature on the first scene
+        from diffvg_amd.autotune import Tuner
+        tuner = Tuner()
+        for i, (_, _, ds) in enumerate(built):
+            tuner.tune(ds, sizes[i][0], sizes[i][1], 2, 2)
+        tuned = {str(k): v for k, v in tuner.best.items()}
     fjobs = [(ds, dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0, render_image=view(img_flat, i)))
              for i, (_, _, ds) in enumerate(built)]
     bjobs = [(ds, dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0, d_render_image=view(d_flat, i)))
@@ -308,6 +316,7 @@ def bench_batched(a):
         'speedup_graph_vs_plain': round(t_plain / t_graph, 2),
         'speedup_batched_vs_plain': round(t_plain / t_batched, 2),
         'kernels_ms_eager': kern,
+        'autotuned_tile_widths': tuned,
     }
     print(json.dumps(out))
 
@@ -424,6 +433,7 @@ def main():
     ap.add_argument('--cpu-baseline-only', action='store_true')
     ap.add_argument('--copy-calibration', action='store_true', help='run a plain copy kernel of known size')
     ap.add_argument('--requests', type=int, default=120, help='serving stream length (config 5)')
+    ap.add_argument('--autotune', action='store_true', help='pick tile widths per workload (config 4)')
     ap.add_argument('--hbm-budget', type=int, default=64, help='GiB that requests in flight may hold (config 5)')
     ap.add_argument('--max-batch', type=int, default=32, help='requests per batch (config 5)')
     a = ap.parse_args()

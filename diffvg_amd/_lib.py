@@ -77,6 +77,7 @@ def lib():
     L.dvg_scene_get_grads.argtypes = [ctypes.c_void_p, ctypes.POINTER(Grads)]
     L.dvg_scene_zero_grads.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.dvg_scene_set_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dvg_scene_set_tiling.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.dvg_scene_grad_layout.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64 * 7),
                                         ctypes.POINTER(ctypes.c_int64)]
     L.dvg_scene_copy_grads.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -134,6 +135,9 @@ class DeviceScene:
 
     def set_mode(self, mode):
         check(self._lib.dvg_scene_set_mode(self.handle, int(mode)))
+
+    def set_tiling(self, tile_width):
+        check(self._lib.dvg_scene_set_tiling(self.handle, int(tile_width)))
 
     @staticmethod
     def make_args(width, height, nsx, nsy, seed, background=None, render_image=None, render_sdf=None,

@@ -724,6 +724,14 @@ int dvg_scene_destroy(dvg_scene *scene) {
     return 0;
 }
 
+int dvg_scene_set_tiling(dvg_scene *scene, int tile_width) {
+    if (!scene) return set_error("null scene");
+    if (tile_width < 0 || tile_width > 256 || (tile_width & (tile_width - 1)))
+        return set_error("tile width must be 0 (default) or a power of two up to 256");
+    scene->h.tile_w = tile_width;
+    return 0;
+}
+
 int dvg_scene_set_mode(dvg_scene *scene, int mode) {
     if (!scene) return set_error("null scene");
     if (mode != DVG_MODE_TILED && mode != DVG_MODE_PLAIN) return set_error("unknown render mode");

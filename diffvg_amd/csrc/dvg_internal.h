@@ -52,6 +52,7 @@ struct SceneHost {
     int device = 0;
     bool device_set = false;      // a device was chosen and memory may be held on it
     int mode = DVG_MODE_TILED;
+    int tile_w = 0;               // pixel tile width of the TILED schedule (0 = square default)
     int cw = 0, ch = 0, num_shapes = 0, num_groups = 0;
     int num_shape_params = 0, num_points_total = 0, num_color_params = 0;
     void *block = nullptr;        // scene arrays

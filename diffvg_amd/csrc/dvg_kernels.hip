@@ -1128,6 +1128,8 @@ static Geom make_geom(const SceneHost &sc, const dvg_render_args &a) {
     if (px < 1) px = 1;
     int tw = 1;
     while ((tw * 2) * (tw * 2) <= px) tw *= 2;
+    // a tile width chosen by the caller (the autotuner) overrides the square default
+    if (sc.tile_w > 0 && sc.tile_w <= px) tw = sc.tile_w;
     int th = px / tw;
     if (th < 1) th = 1;
     G.tw = tw;

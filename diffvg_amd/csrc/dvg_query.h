@@ -14,6 +14,7 @@
 #pragma once
 
 #include "dvg_math.h"
+#include "dvg_winding_fast.h"
 #include "dvg_scene.h"
 
 namespace dvg {
@@ -76,6 +77,10 @@ DVG_DEV int segment_winding(int k, const V2 *p, V2 pt) {
         }
     } else {
         V2 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3];
+        // decided in float whenever every decision of the reference is certain
+        // (dvg_winding_fast.h); otherwise the reference's double solve below
+        int wfast = 0;
+        if (wf::cubic_winding_fast(p0.x, p0.y, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y, pt.x, pt.y, wfast)) return wfast;
         double t[3];
         double ca = (double)(-p0.y + 3 * p1.y - 3 * p2.y + p3.y), cb = (double)(3 * p0.y - 6 * p1.y + 3 * p2.y),
                cc = (double)(-3 * p0.y + 3 * p1.y), cd = (double)(p0.y - pt.y);

@@ -1,0 +1,152 @@
+// dvg_winding_fast.h -- decision-exact fast path of the cubic winding test.
+//
+// The winding contribution of a cubic segment (winding_number.h:119-157) is the sum,
+// over the roots t of y(t) = pt.y that the double cubic solver (solve.h:29-64) returns,
+// of [0 <= t <= 1] [x(t) - pt.x > 0] sign(y'(t)).  The reference evaluates this on
+// power-basis coefficients rounded to float, so its polynomial differs from the exact
+// Bezier curve by at most a few units in the last place of the coordinates.
+//
+// This file decides the same sum in float without the cubic solve whenever every one of
+// those decisions is certain by a margin that covers that difference:
+//   * the curve's y(t) - pt.y is split at its critical points; every breakpoint value
+//     (ends and extrema) must be at least `ey` away from zero, so each piece crosses
+//     the ray at most once, away from the piece ends (no root near t = 0 or 1, no
+//     tangency);
+//   * each crossing is bracketed by bisection to 2^-12, and |y'| must stay above a
+//     margin over the bracket, so the sign the reference computes at its root is the
+//     crossing direction;
+//   * x(t) - pt.x must be certainly positive or certainly negative over the bracket.
+// When any test is uncertain it returns false and the caller runs the reference's
+// solve.  Host and device compile the same code (no floating-point contraction).
+#pragma once
+
+#include <cmath>
+
+#if defined(__HIPCC__)
+#define DVG_WF __host__ __device__ inline
+#else
+#define DVG_WF inline
+#endif
+
+namespace dvg {
+namespace wf {
+
+DVG_WF float bern3(float c0, float c1, float c2, float c3, float t) {
+    float s = 1.f - t;
+    return (s * s * s) * c0 + (3.f * s * s * t) * c1 + (3.f * s * t * t) * c2 + (t * t * t) * c3;
+}
+// derivative / 3 of the cubic with control values c0..c3, from its differences
+DVG_WF float bern2(float d0, float d1, float d2, float t) {
+    float s = 1.f - t;
+    return (s * s) * d0 + (2.f * s * t) * d1 + (t * t) * d2;
+}
+
+DVG_WF float max4(float a, float b, float c, float d) { return fmaxf(fmaxf(a, b), fmaxf(c, d)); }
+
+// Returns true and sets `w` when the contribution is decided.
+DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float p3x,
+                               float p3y, float ptx, float pty, int &w) {
+    const float M = fmaxf(max4(fabsf(p0y), fabsf(p1y), fabsf(p2y), fabsf(p3y)), fabsf(pty)) + 1.f;
+    const float Mx = fmaxf(max4(fabsf(p0x), fabsf(p1x), fabsf(p2x), fabsf(p3x)), fabsf(ptx)) + 1.f;
+    // The reference's solver works on these float coefficients (winding_number.h:134-138).
+    // Its roots are accurate to about E in t: for a near-quadratic cubic (|b| >> |a|) the
+    // trigonometric branch loses precision as |b / a| grows; below |a| = 1e-6 it solves
+    // the quadratic instead.
+    const float ra = -p0y + 3 * p1y - 3 * p2y + p3y, rb = 3 * p0y - 6 * p1y + 3 * p2y, rc = -3 * p0y + 3 * p1y;
+    const float E = fabsf(ra) < 1e-6f ? 1e-9f : 2e-7f * (fabsf(rb) / fabsf(ra) + sqrtf(fabsf(rc) / fabsf(ra)) + 1.f);
+    if (!(E < 1e-3f)) return false;
+    const float ddy = fmaxf(fmaxf(fabsf(p1y - p0y), fabsf(p2y - p1y)), fabsf(p3y - p2y));
+    const float ddx = fmaxf(fmaxf(fabsf(p1x - p0x), fabsf(p2x - p1x)), fabsf(p3x - p2x));
+    // value margins: the reference's coefficients are within ~2e-6 M of the curve's, and
+    // its roots within E of the true ones
+    const float ey = 1e-5f * M + 3.f * E * ddy;
+    const float ex = 1e-5f * Mx + 3.f * E * ddx;
+    const float c0 = p0y - pty, c1 = p1y - pty, c2 = p2y - pty, c3 = p3y - pty;
+    // hull above or below the ray
+    if ((c0 > ey && c1 > ey && c2 > ey && c3 > ey) || (c0 < -ey && c1 < -ey && c2 < -ey && c3 < -ey)) {
+        w = 0;
+        return true;
+    }
+    // hull entirely left of pt: every root has x(t) - pt.x < 0
+    if (max4(p0x, p1x, p2x, p3x) < ptx - ex) {
+        w = 0;
+        return true;
+    }
+    // breakpoints: 0, critical points of y in (0, 1), 1
+    const float d0 = p1y - p0y, d1 = p2y - p1y, d2 = p3y - p2y;
+    const float qa = d0 - 2.f * d1 + d2, qb = 2.f * (d1 - d0), qc = d0;
+    float br[4];
+    int nb = 0;
+    br[nb++] = 0.f;
+    {
+        float r0 = -1.f, r1 = -1.f;
+        const float sc = fabsf(qa) + fabsf(qb) + fabsf(qc);
+        if (fabsf(qa) > 1e-6f * sc) {
+            float disc = qb * qb - 4.f * qa * qc;
+            if (disc >= 0.f) {
+                float sq = sqrtf(disc);
+                float q = -0.5f * (qb + (qb >= 0.f ? sq : -sq));
+                r0 = q / qa;
+                r1 = q != 0.f ? qc / q : -1.f;
+            }
+        } else if (fabsf(qb) > 1e-6f * sc) {
+            r0 = -qc / qb;
+        } else {
+            return false;  // y nearly constant: leave it to the solver
+        }
+        if (r0 > r1) {
+            float tmp = r0;
+            r0 = r1;
+            r1 = tmp;
+        }
+        if (r0 > 0.f && r0 < 1.f) br[nb++] = r0;
+        if (r1 > 0.f && r1 < 1.f && r1 != r0) br[nb++] = r1;
+    }
+    br[nb++] = 1.f;
+    float fv[4];
+    for (int i = 0; i < nb; i++) {
+        fv[i] = i == 0 ? c0 : (i == nb - 1 ? c3 : bern3(c0, c1, c2, c3, br[i]));
+        if (!(fabsf(fv[i]) > ey)) return false;
+    }
+    // bounds for the bracket tests
+    const float dmax = fmaxf(fmaxf(fabsf(d0), fabsf(d1)), fabsf(d2));       // |y'| / 3 <= dmax
+    const float ddmax = 2.f * fmaxf(fabsf(d1 - d0), fabsf(d2 - d1));        // |y''| / 3 <= ddmax
+    const float dx0 = p1x - p0x, dx1 = p2x - p1x, dx2 = p3x - p2x;
+    const float xdmax = 3.f * fmaxf(fmaxf(fabsf(dx0), fabsf(dx1)), fabsf(dx2));  // |x'| bound
+    const float ed = 1e-4f * M * 3.f + 1e-6f + 6.f * E * ddmax;              // derivative margin
+    (void)dmax;
+    int acc = 0;
+    for (int i = 0; i + 1 < nb; i++) {
+        const float fa = fv[i], fb = fv[i + 1];
+        if ((fa > 0.f) == (fb > 0.f)) continue;  // no crossing on this monotone piece
+        // bisection to a bracket of width (b - a) / 4096
+        float lo = br[i], hi = br[i + 1];
+        const bool rising = fb > 0.f;
+        for (int it = 0; it < 12; it++) {
+            float mid = 0.5f * (lo + hi);
+            float fm = bern3(c0, c1, c2, c3, mid);
+            if ((fm > 0.f) == rising)
+                hi = mid;
+            else
+                lo = mid;
+        }
+        const float width = hi - lo;
+        // |y'| over the bracket: value at the ends minus the largest change across it
+        float ylo = 3.f * bern2(d0, d1, d2, lo), yhi = 3.f * bern2(d0, d1, d2, hi);
+        float var = 3.f * ddmax * width;
+        if ((ylo > 0.f) != rising || (yhi > 0.f) != rising) return false;
+        if (!(fminf(fabsf(ylo), fabsf(yhi)) - var > ed)) return false;
+        // x(t) - pt.x over the bracket
+        float xlo = bern3(p0x, p1x, p2x, p3x, lo) - ptx, xhi = bern3(p0x, p1x, p2x, p3x, hi) - ptx;
+        float xvar = xdmax * width + ex;
+        if (fminf(xlo, xhi) - xvar > 0.f)
+            acc += rising ? 1 : -1;
+        else if (!(fmaxf(xlo, xhi) + xvar < 0.f))
+            return false;
+    }
+    w = acc;
+    return true;
+}
+
+}  // namespace wf
+}  // namespace dvg

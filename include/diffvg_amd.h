@@ -142,6 +142,11 @@ int dvg_scene_copy_grads(dvg_scene *scene, float *dst, void *stream);
 int dvg_scene_zero_grads(dvg_scene *scene, void *stream);
 /* Selects DVG_MODE_TILED (default) or DVG_MODE_PLAIN for later renders. */
 int dvg_scene_set_mode(dvg_scene *scene, int mode);
+/* Pixel tile width of the TILED schedule for later renders and batches of this scene
+ * (a power of two; the tile height follows from 256 samples per tile; 0 restores the
+ * square default).  Picked by the autotuner; results do not depend on it beyond float
+ * summation order.  No reference counterpart (the reference has no tiles). */
+int dvg_scene_set_tiling(dvg_scene *scene, int tile_width);
 
 /* Batched render: many (scene, render args) jobs -- scenes of any size, resolution,
  * sample count, output (colour or SDF) and prefiltering -- rendered with one kernel

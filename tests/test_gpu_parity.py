@@ -361,3 +361,32 @@ def test_gpu_sdf_tile_culling_vs_oracle(case):
     _compare(g.render(w, h, 1, 1, 0, output='sdf', d_sdf=d, translation=True),
              o.render(w, h, 1, 1, 0, output='sdf', d_sdf=d, translation=True), (case, 'sdf bwd'))
     _compare(g.render(w, h, 2, 2, 3, output='sdf'), o.render(w, h, 2, 2, 3, output='sdf'), (case, 'sdf 2x2'))
+
+
+@pytest.mark.parametrize('case', ['strokes', 'overlap_nz', 'gradients'])
+def test_gpu_tile_shape_does_not_change_results(case):
+    """The autotuner's knob (tile width) changes only float summation order: forward
+    images are identical and gradients agree to atomic-order noise."""
+    f = scenes.flat_case(case)
+    w, h = f['canvas_width'], f['canvas_height']
+    d = np.random.default_rng(6).uniform(-1, 1, (h, w, 4)).astype(np.float32)
+    base = GpuRunner(f)
+    ref_img = base.render(w, h, 2, 2, 3)['image']
+    ref_g = base.render(w, h, 2, 2, 3, d_image=d)['grads']
+    for tw in (1, 2, 4, 16, 32, 64):
+        g = GpuRunner(f)
+        g.scene.set_tiling(tw)
+        assert np.array_equal(g.render(w, h, 2, 2, 3)['image'], ref_img), tw
+        gg = g.render(w, h, 2, 2, 3, d_image=d)['grads']
+        for k in GRAD_KEYS:
+            assert _rel(gg[k], ref_g[k]) <= 1e-4, (tw, k)
+
+
+def test_autotuner_picks_a_candidate():
+    from diffvg_amd.autotune import Tuner
+    f = scenes.flat_case('overlap_eo')
+    sc = _lib.DeviceScene(f, 0)
+    t = Tuner(reps=1)
+    best = t.tune(sc, 64, 64, 2, 2)
+    assert best in t.candidates(4)
+    assert t.apply(_lib.DeviceScene(f, 0), 64, 64, 2, 2) == best
