@@ -298,6 +298,11 @@ DVG_DEV int segment_within(const DevScene &S, int s, int seg, V2 pt, float r) {
     }
     if (dist_sq(p0, pt) < r0 * r0) return 1;  // eval(0) == p0 exactly
     if (dist_sq(ceval(p0, p1, p2, p3, 1), pt) < r3 * r3) return 1;
+    // every curve point certainly farther than the largest radius magnitude (the radius
+    // at t is a convex combination of r0..r3): the quintic search below would find nothing
+    if (wf::cubic_certainly_farther(p0.x, p0.y, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y, pt.x, pt.y,
+                                    fmaxf(fmaxf(fabsf(r0), fabsf(r1)), fmaxf(fabsf(r2), fabsf(r3)))))
+        return 0;
     Quintic q = cubic_quintic(p0, p1, p2, p3, pt);
     float iv[4];
     int niv = quintic_intervals(q, iv);

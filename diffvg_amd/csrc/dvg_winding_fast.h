@@ -1,4 +1,5 @@
-// dvg_winding_fast.h -- decision-exact fast path of the cubic winding test.
+// dvg_winding_fast.h -- decision-exact fast paths of the cubic winding and cubic stroke
+// coverage tests.
 //
 // The winding contribution of a cubic segment (winding_number.h:119-157) is the sum,
 // over the roots t of y(t) = pt.y that the double cubic solver (solve.h:29-64) returns,
@@ -145,6 +146,42 @@ DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float
             return false;
     }
     w = acc;
+    return true;
+}
+
+// Stroke coverage of a cubic segment (within_distance.h:119-272) accepts only when one
+// of the curve points it evaluates (the ends and the stationary points of the distance)
+// is closer than the stroke radius.  This returns true when every point of the curve is
+// at least rmax + a rounding margin away from pt, so that test certainly rejects: the
+// curve lies within (h^2 / 8) max|B''| of its chords over 8 equal steps of t, with
+// |B''| <= 6 max|second differences of the control points| per coordinate.
+DVG_WF float seg_dist(float ax, float ay, float bx, float by, float px, float py) {
+    float dx = bx - ax, dy = by - ay;
+    float l2 = dx * dx + dy * dy;
+    float t = l2 > 0.f ? ((px - ax) * dx + (py - ay) * dy) / l2 : 0.f;
+    t = fminf(fmaxf(t, 0.f), 1.f);
+    float qx = ax + t * dx - px, qy = ay + t * dy - py;
+    return sqrtf(qx * qx + qy * qy);
+}
+
+DVG_WF bool cubic_certainly_farther(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float p3x,
+                                    float p3y, float ptx, float pty, float rmax) {
+    const float M = fmaxf(max4(fabsf(p0x), fabsf(p1x), fabsf(p2x), fabsf(p3x)),
+                          fmaxf(max4(fabsf(p0y), fabsf(p1y), fabsf(p2y), fabsf(p3y)), fmaxf(fabsf(ptx), fabsf(pty)))) +
+                    1.f;
+    const float margin = 1e-4f * M;
+    const float sx = fmaxf(fabsf(p0x - 2.f * p1x + p2x), fabsf(p1x - 2.f * p2x + p3x));
+    const float sy = fmaxf(fabsf(p0y - 2.f * p1y + p2y), fabsf(p1y - 2.f * p2y + p3y));
+    const float dev = (6.f / 512.f) * sqrtf(sx * sx + sy * sy) * 1.001f;
+    const float need = rmax + margin + dev;
+    float ax = p0x, ay = p0y;
+    for (int k = 1; k <= 8; k++) {
+        float t = 0.125f * (float)k;
+        float bx = bern3(p0x, p1x, p2x, p3x, t), by = bern3(p0y, p1y, p2y, p3y, t);
+        if (!(seg_dist(ax, ay, bx, by, ptx, pty) > need)) return false;
+        ax = bx;
+        ay = by;
+    }
     return true;
 }
 

@@ -1,9 +1,9 @@
-"""Shape containers with the constructor signatures of pydiffvg/shape.py:5-61.
-
-SVG path parsing (``from_svg_path``, pydiffvg/shape.py:63-172) is not part of the
-render path and is not provided.
+"""Shape containers with the constructor signatures of pydiffvg/shape.py:5-61, and
+``from_svg_path`` (pydiffvg/shape.py:63-172) over diffvg_amd.svg_path.
 """
 import torch
+
+from . import svg_path as _svg_path
 
 
 class Circle:
@@ -68,3 +68,14 @@ class ShapeGroup:
         self.stroke_color = stroke_color
         self.shape_to_canvas = shape_to_canvas
         self.id = id
+
+
+def from_svg_path(path_str, shape_to_canvas=torch.eye(3), force_close=False):
+    """SVG path data -> list of Path, one per continuous subpath, with points mapped
+    through `shape_to_canvas` (pydiffvg/shape.py:63-172).  Near-closed subpaths (ends
+    within 1e-5) are snapped shut; with `force_close` open subpaths get a closing line;
+    arcs become cubics."""
+    segs = _svg_path.parse_path(path_str)
+    if len(segs) == 0:
+        return []
+    return _svg_path.segments_to_paths(segs, shape_to_canvas, force_close, Path)

@@ -74,3 +74,96 @@ void wf_fuzz(int mode, int64_t n, uint64_t seed, int64_t *out, float *bad) {
     out[2] = n;
 }
 }
+
+#include "../../oracle/oracle.h"
+
+extern "C" {
+// Stroke coverage: whenever the fast rejection fires, the oracle's within-distance test
+// of a one-segment open path with that stroke width must reject.  out as wf_fuzz.
+void wf_within_fuzz(int mode, int64_t n, uint64_t seed, int64_t *out, float *bad) {
+    std::mt19937_64 g(seed);
+    std::uniform_real_distribution<float> U(0.f, 1.f);
+    std::normal_distribution<float> N(0.f, 1.f);
+    int64_t decided = 0, mism = 0;
+    for (int64_t i = 0; i < n; i++) {
+        float cx = 100.f + U(g) * 300.f, cy = 100.f + U(g) * 300.f;
+        float sp = 2.f + 20.f * U(g);
+        float pts[8];
+        for (int k = 0; k < 4; k++) {
+            pts[2 * k] = cx + sp * N(g);
+            pts[2 * k + 1] = cy + sp * N(g);
+        }
+        float r = mode == 1 ? 0.05f + 0.2f * U(g) : 0.5f + 3.5f * U(g);
+        // mode 2: per-point thickness of either sign (the radius the test sees is its magnitude)
+        float th[4];
+        float rmax = fabsf(r);
+        for (int k = 0; k < 4; k++) {
+            th[k] = mode == 2 ? (U(g) * 8.f - 4.f) : r;
+            rmax = mode == 2 ? (k == 0 ? fabsf(th[k]) : fmaxf(rmax, fabsf(th[k]))) : rmax;
+        }
+        float ptx = cx + 1.5f * sp * N(g), pty = cy + 1.5f * sp * N(g);
+        if (!dvg::wf::cubic_certainly_farther(pts[0], pts[1], pts[2], pts[3], pts[4], pts[5], pts[6], pts[7], ptx,
+                                              pty, rmax))
+            continue;
+        decided++;
+        int32_t st = 2, poff = -1, ctrl_off = 0, nb = 1, pto = 0, npt = 4, ncp = 2, goff = 0, gn = 1, gid = 0;
+        uint8_t no = 0;
+        int32_t ft = -1, fo = 0, fns = 0, sty = 0, so = 0, sns = 0;
+        float col[4] = {1, 1, 1, 1}, xf[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, sw = r, zero = 0.f;
+        orc_scene_desc d;
+        std::memset(&d, 0, sizeof(d));
+        d.canvas_width = d.canvas_height = 512;
+        d.num_shapes = 1;
+        d.shape_type = &st;
+        d.stroke_width = &sw;
+        d.shape_param_offset = &poff;
+        d.shape_params = &zero;
+        d.num_shape_params = 0;
+        d.path_ctrl_offset = &ctrl_off;
+        d.path_num_base_points = &nb;
+        d.path_point_offset = &pto;
+        d.path_num_points = &npt;
+        uint8_t has_th = mode == 2 ? 1 : 0;
+        d.path_has_thickness = &has_th;
+        d.path_is_closed = &no;
+        d.path_use_distance_approx = &no;
+        d.num_control_points = &ncp;
+        d.num_ctrl_total = 1;
+        d.points = pts;
+        d.thickness = mode == 2 ? th : nullptr;
+        d.num_points_total = 4;
+        d.num_groups = 1;
+        d.group_shape_offset = &goff;
+        d.group_num_shapes = &gn;
+        d.group_shape_ids = &gid;
+        d.num_group_shape_ids = 1;
+        d.fill_type = &ft;
+        d.fill_offset = &fo;
+        d.fill_num_stops = &fns;
+        d.stroke_type = &sty;
+        d.stroke_offset = &so;
+        d.stroke_num_stops = &sns;
+        d.color_params = col;
+        d.num_color_params = 4;
+        d.use_even_odd_rule = &no;
+        d.shape_to_canvas = xf;
+        d.filter_type = 0;
+        d.filter_radius = 0.5f;
+        orc_scene *sc = nullptr;
+        if (orc_scene_create(&d, &sc) != 0) continue;
+        int within = orc_within_distance(sc, 0, ptx, pty);
+        orc_scene_destroy(sc);
+        if (within) {
+            if (mism == 0) {
+                for (int k = 0; k < 8; k++) bad[k] = pts[k];
+                bad[8] = ptx;
+                bad[9] = pty;
+            }
+            mism++;
+        }
+    }
+    out[0] = decided;
+    out[1] = mism;
+    out[2] = n;
+}
+}

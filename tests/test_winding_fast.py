@@ -1,4 +1,4 @@
-"""The renderer's fast cubic winding path (diffvg_amd/csrc/dvg_winding_fast.h) against the
+"""The renderer's fast cubic winding and stroke-rejection paths (diffvg_amd/csrc/dvg_winding_fast.h) against the
 oracle's restatement of the reference's double-precision solve: on random and adversarial
 segments (rays through control-point y values, near-tangent rays, large coordinates,
 nearly flat curves, points inside the control box) every decision it takes must equal
@@ -21,5 +21,21 @@ def test_fast_cubic_winding_never_disagrees(mode, min_decided):
     out = (ctypes.c_int64 * 3)()
     bad = (ctypes.c_float * 10)()
     L.wf_fuzz(mode, 400000, 1000 + mode, out, bad)
+    assert out[1] == 0, ('mismatch', list(bad))
+    assert out[0] / out[2] >= min_decided
+
+
+@pytest.mark.parametrize('mode, min_decided', [(0, 0.8), (1, 0.9), (2, 0.6)])
+def test_fast_cubic_stroke_rejection_never_disagrees(mode, min_decided):
+    """Whenever the certain-rejection test of a cubic stroke fires, the oracle's
+    within-distance test (within_distance.h:119-272) rejects the point too."""
+    if not os.path.exists(SO):
+        pytest.skip('tests/csrc not built (make -C tests/csrc)')
+    L = ctypes.CDLL(SO)
+    L.wf_within_fuzz.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64),
+                                 ctypes.POINTER(ctypes.c_float)]
+    out = (ctypes.c_int64 * 3)()
+    bad = (ctypes.c_float * 10)()
+    L.wf_within_fuzz(mode, 200000, 77 + mode, out, bad)
     assert out[1] == 0, ('mismatch', list(bad))
     assert out[0] / out[2] >= min_decided
