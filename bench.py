@@ -375,6 +375,17 @@ def bench_serving(a):
     st_b = serve.latency_stats(done)
     plain_done = serve.serve_plain([make(i) for i in range(n_req)], list(arrivals))
     st_p = serve.latency_stats(plain_done)
+    # each request alone on an idle device (its service time without queueing), and the
+    # requests with the longest served latency
+    solo = []
+    for i in range(n_req):
+        one = serve.Server(device=0, hbm_budget=a.hbm_budget << 30, max_batch=1, streams=1)
+        r = one.run([make(i)], [0.0])[0]
+        solo.append((r.done - r.arrival) * 1e3)
+    slowest = sorted(done, key=lambda r: r.arrival - r.done)[:5]
+    tail = [{'request': r.rid, 'latency_ms': round((r.done - r.arrival) * 1e3, 2),
+             'alone_ms': round(solo[r.rid], 2), 'size': specs[r.rid][1], 'paths': specs[r.rid][8],
+             'sdf': bool(specs[r.rid][5]), 'prefilter': bool(specs[r.rid][6])} for r in slowest]
     samples = sum(w * h * (1 if sdf else nsx * nsy) for (_, w, h, nsx, nsy, sdf, _, _, _) in specs)
     span_b = max(r.done for r in done) - min(r.arrival for r in done)
     out = {
@@ -389,6 +400,9 @@ def bench_serving(a):
                    'hbm_budget_gib': a.hbm_budget, 'max_batch': a.max_batch, 'streams': 3},
         'served': {k: round(v, 3) for k, v in st_b.items()},
         'plain': {k: round(v, 3) for k, v in st_p.items()},
+        'alone': {'p50_ms': round(float(np.percentile(solo, 50)), 3),
+                  'p99_ms': round(float(np.percentile(solo, 99)), 3), 'max_ms': round(max(solo), 3)},
+        'slowest_served': tail,
     }
     print(json.dumps(out))
 

@@ -53,6 +53,12 @@ class RenderArgs(ctypes.Structure):
                 ('stream', ctypes.c_void_p)]
 
 
+class SceneParams(ctypes.Structure):
+    _fields_ = [('shape_params', ctypes.c_void_p), ('points', ctypes.c_void_p), ('thickness', ctypes.c_void_p),
+                ('stroke_width', ctypes.c_void_p), ('color_params', ctypes.c_void_p),
+                ('shape_to_canvas', ctypes.c_void_p)]
+
+
 class Grads(ctypes.Structure):
     _fields_ = [('d_shape_params', F32P), ('d_points', F32P), ('d_thickness', F32P),
                 ('d_stroke_width', F32P), ('d_color_params', F32P), ('d_shape_to_canvas', F32P),
@@ -81,6 +87,10 @@ def lib():
     L.dvg_scene_grad_layout.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64 * 7),
                                         ctypes.POINTER(ctypes.c_int64)]
     L.dvg_scene_copy_grads.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.dvg_scene_update.argtypes = [ctypes.c_void_p, ctypes.POINTER(SceneParams), ctypes.c_float, ctypes.c_void_p]
+    L.dvg_scene_status.argtypes = [ctypes.c_void_p]
+    L.dvg_scene_debug_block.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                        ctypes.POINTER(ctypes.c_int64)]
     L.dvg_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(RenderArgs), ctypes.c_int,
                                    ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
     L.dvg_batch_render.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -151,6 +161,28 @@ class DeviceScene:
     def render(self, width, height, nsx, nsy, seed, **kw):
         a = self.make_args(width, height, nsx, nsy, seed, **kw)
         check(self._lib.dvg_render(self.handle, ctypes.byref(a)))
+
+    def update(self, filter_radius, stream=None, shape_params=None, points=None, thickness=None, stroke_width=None,
+               color_params=None, shape_to_canvas=None):
+        """Rebuilds the scene on the device from new parameter values (float32 device
+        tensors in the description's layout; None keeps an array).  The tensors must stay
+        alive until the stream has run the copies (stream-ordered allocation does that)."""
+        p = SceneParams(ptr(shape_params), ptr(points), ptr(thickness), ptr(stroke_width), ptr(color_params),
+                        ptr(shape_to_canvas))
+        check(self._lib.dvg_scene_update(self.handle, ctypes.byref(p), float(filter_radius), stream))
+
+    def status(self):
+        """Waits for the scene's last stream; raises on the conditions its last update or
+        render found."""
+        check(self._lib.dvg_scene_status(self.handle))
+
+    def debug_block(self):
+        n = ctypes.c_int64()
+        check(self._lib.dvg_scene_debug_block(self.handle, None, 0, ctypes.byref(n)))
+        buf = np.zeros(int(n.value), np.uint8)
+        if n.value:
+            check(self._lib.dvg_scene_debug_block(self.handle, buf.ctypes.data, int(n.value), ctypes.byref(n)))
+        return buf
 
     def zero_grads(self, stream=None):
         check(self._lib.dvg_scene_zero_grads(self.handle, stream))

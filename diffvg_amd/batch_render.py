@@ -1,0 +1,104 @@
+"""Differentiable rendering of many scenes in one launch per stage.
+
+`BatchRenderFunction` is the batched counterpart of `RenderFunction`: it takes the
+serialize_scene argument lists of several scenes (any sizes, shape counts, sample
+counts) and renders all of them with one dvg_batch forward and one dvg_batch backward,
+where the reference renders such workloads with one RenderFunction call per scene
+(apps/generative_models/rendering.py:262-290 loops over the batch).  Each scene keeps
+the semantics of `RenderFunction.apply(width, height, nsx, nsy, seed, None, *args)`:
+same image, same gradients for its argument tensors.
+
+Scenes of a topology seen before are rebuilt on the device from their new values
+(dvg_scene_update) in a pool, so a training loop over same-shaped scenes builds nothing
+on the host after its first step.
+"""
+import torch
+
+from . import _lib
+from .batch import Batch
+from .device import get_device
+from .render_pytorch import (OutputType, _check_gpu, _device_scene_for, _grad_tensors, _stream, check_rebuild)
+
+__all__ = ['BatchRenderFunction', 'render_batch']
+
+
+class BatchRenderFunction(torch.autograd.Function):
+    """apply(spec, *args): spec is a tuple of (width, height, num_samples_x,
+    num_samples_y, seed, num_args) per scene; args are the scenes' serialize_scene lists
+    concatenated.  Returns one image tensor per scene."""
+
+    @staticmethod
+    def forward(ctx, spec, *args):
+        _check_gpu()
+        dev = get_device()
+        st = _stream()
+        scenes, flats, images, jobs, spans = [], [], [], [], []
+        k = 0
+        for (w, h, nsx, nsy, seed, n) in spec:
+            a = args[k:k + n]
+            spans.append((k, n))
+            k += n
+            flat, scene = _device_scene_for(a, dev, check=False)
+            if flat.output_type != OutputType.color or flat.eval_positions.shape[0] != 0:
+                raise NotImplementedError('BatchRenderFunction renders colour images (OutputType.color)')
+            if nsx * nsy > 256:
+                raise ValueError('a batched render takes at most 256 samples per pixel')
+            img = torch.zeros(h, w, 4, device=dev)
+            scenes.append(scene)
+            flats.append(flat)
+            images.append(img)
+            jobs.append((scene, dict(width=w, height=h, nsx=nsx, nsy=nsy, seed=seed, render_image=img,
+                                     use_prefiltering=flat.use_prefiltering)))
+        if k != len(args):
+            raise ValueError('spec does not account for every argument')
+        for sc in scenes:
+            check_rebuild(sc)
+        fwd = Batch(jobs, zero_grads=False)
+        fwd.render(st)
+        ctx.spec, ctx.spans, ctx.scenes, ctx.flats = spec, spans, scenes, flats
+        ctx.args = args
+        ctx.fwd = fwd
+        return tuple(images)
+
+    @staticmethod
+    def backward(ctx, *grad_imgs):
+        dev = get_device()
+        st = _stream()
+        jobs, keep = [], []
+        for i, ((w, h, nsx, nsy, seed, n), sc, flat) in enumerate(zip(ctx.spec, ctx.scenes, ctx.flats)):
+            g = grad_imgs[i]
+            g = torch.zeros(h, w, 4, device=dev) if g is None else g.to(torch.float32).contiguous()
+            keep.append(g)
+            jobs.append((sc, dict(width=w, height=h, nsx=nsx, nsy=nsy, seed=seed, d_render_image=g,
+                                  use_prefiltering=flat.use_prefiltering)))
+        bwd = Batch(jobs, zero_grads=True)
+        bwd.render(st)
+        out = [None] * len(ctx.args)
+        blocks = []
+        for sc, flat, (k, n) in zip(ctx.scenes, ctx.flats, ctx.spans):
+            block, layout = sc.grads_device(dev, st)
+            blocks.append(block)
+            for j, t in enumerate(_grad_tensors(flat, block, layout, ctx.args[k:k + n], flat.num_args)):
+                out[k + j] = t
+        bwd.status()
+        assert all(bool(torch.isfinite(b).all()) for b in blocks)
+        return (None,) + tuple(out)
+
+
+def render_batch(items, num_samples_x=2, num_samples_y=2, seeds=None):
+    """items: list of (canvas_width, canvas_height, shapes, shape_groups) or of
+    serialize_scene argument lists; returns the list of rendered images (differentiable
+    w.r.t. every parameter tensor of every scene)."""
+    from .render_pytorch import RenderFunction
+    spec, flat_args = [], []
+    for i, it in enumerate(items):
+        if isinstance(it, (list, tuple)) and len(it) == 4 and isinstance(it[2], (list, tuple)):
+            w, h, shapes, groups = it
+            a = RenderFunction.serialize_scene(w, h, shapes, groups)
+        else:
+            a = list(it)
+            w, h = int(a[0]), int(a[1])
+        seed = 0 if seeds is None else int(seeds[i])
+        spec.append((int(w), int(h), int(num_samples_x), int(num_samples_y), seed, len(a)))
+        flat_args += a
+    return list(BatchRenderFunction.apply(tuple(spec), *flat_args))

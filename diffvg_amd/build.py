@@ -7,7 +7,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libdiffvg_amd.so')
 SOURCES = ['dvg_host.cpp', 'dvg_kernels.hip']
-HEADERS = ['dvg_scene.h', 'dvg_math.h', 'dvg_query.h', 'dvg_shade.h', 'dvg_internal.h']
+HEADERS = ['dvg_scene.h', 'dvg_math.h', 'dvg_crmath.h', 'dvg_query.h', 'dvg_winding_fast.h', 'dvg_shade.h',
+           'dvg_internal.h', 'dvg_build.h', 'dvg_sort.h']
 
 # -ffp-contract=off keeps every multiply and add separately rounded, as the reference's
 # x86-64 CPU build computes them, so coverage decisions are taken on identical bits.

@@ -18,6 +18,7 @@
 #include <numeric>
 #include <vector>
 
+#include "dvg_build.h"
 #include "dvg_internal.h"
 
 namespace dvg {
@@ -96,28 +97,11 @@ int set_error(const std::string &msg) {
 
 namespace {
 
-struct HV2 {
-    float x, y;
-};
-inline HV2 operator+(HV2 a, HV2 b) { return {a.x + b.x, a.y + b.y}; }
-inline HV2 operator-(HV2 a, HV2 b) { return {a.x - b.x, a.y - b.y}; }
-inline HV2 operator*(float s, HV2 a) { return {s * a.x, s * a.y}; }
-inline float hlen(HV2 v) { return (float)std::sqrt((double)(v.x * v.x + v.y * v.y)); }
-inline float hdist(HV2 a, HV2 b) { return hlen(b - a); }
-
-struct HBox {
-    float minx, miny, maxx, maxy;
-};
-inline HBox hbox_empty() {
-    float inf = std::numeric_limits<float>::infinity();
-    return {inf, inf, -inf, -inf};
-}
-inline float mn(float a, float b) { return a < b ? a : b; }
-inline float mx(float a, float b) { return a > b ? a : b; }
-inline HBox hmerge(const HBox &b, HV2 p) { return {mn(p.x, b.minx), mn(p.y, b.miny), mx(p.x, b.maxx), mx(p.y, b.maxy)}; }
-inline HBox hmerge(const HBox &a, const HBox &b) {
-    return {mn(a.minx, b.minx), mn(a.miny, b.miny), mx(a.maxx, b.maxx), mx(a.maxy, b.maxy)};
-}
+using HV2 = bld::P2;
+using HBox = bld::Box;
+using bld::box_empty;
+using bld::merge;
+using bld::mx;
 
 struct HNode {
     int child0, child1;
@@ -125,33 +109,16 @@ struct HNode {
     float max_radius;
 };
 
-inline uint32_t expand10(uint32_t x) {
-    uint32_t r = 0;
-    for (uint32_t i = 0; i < 10; i++) r |= (x & (1u << i)) << i;
-    return r;
-}
-// float -> uint32 as an x86-64 compiler lowers it (64-bit truncation, low word kept)
-inline uint32_t f2u32(float f) {
-    if (!(f > -9.2233720368547758e18f && f < 9.2233720368547758e18f)) return 0u;
-    return (uint32_t)(uint64_t)(int64_t)f;
-}
-inline uint32_t morton(HV2 p, int w, int h) {
-    float px = p.x / (float)w, py = p.y / (float)h;
-    return (expand10(f2u32(px * 1023)) << 1u) | expand10(f2u32(py * 1023));
-}
-
 // pairwise bottom-up build with the odd-node carry (scene.cpp:439-494)
 void build_bvh(HNode *nodes, int n, bool sort, int cw, int ch) {
     int size = 2 * n - 1;
     if (size <= 1) return;
     if (sort) {
         std::sort(nodes, nodes + n, [&](const HNode &a, const HNode &b) {
-            HV2 ca = 0.5f * (HV2{a.box.minx, a.box.miny} + HV2{a.box.maxx, a.box.maxy});
-            HV2 cb = 0.5f * (HV2{b.box.minx, b.box.miny} + HV2{b.box.maxx, b.box.maxy});
-            return morton(ca, cw, ch) < morton(cb, cw, ch);
+            return bld::morton_of_box(a.box, cw, ch) < bld::morton_of_box(b.box, cw, ch);
         });
     }
-    for (int i = n; i < size; i++) nodes[i] = HNode{-1, -1, hbox_empty(), 0.f};
+    for (int i = n; i < size; i++) nodes[i] = HNode{-1, -1, box_empty(), 0.f};
     int beg = 0, end = n;
     int leftover = end % 2 == 0 ? -1 : end - 1;
     while (end - beg >= 1 || leftover != -1) {
@@ -165,7 +132,7 @@ void build_bvh(HNode *nodes, int n, bool sort, int cw, int ch) {
                 node.child1 = leftover;
                 leftover = -1;
             }
-            node.box = hmerge(nodes[node.child0].box, nodes[node.child1].box);
+            node.box = merge(nodes[node.child0].box, nodes[node.child1].box);
             node.max_radius = std::max(nodes[node.child0].max_radius, nodes[node.child1].max_radius);
             nodes[end + i] = node;
         }
@@ -194,36 +161,20 @@ void dfs_leaves(const HNode *nodes, int n, std::vector<int> &out) {
     }
 }
 
-// 3x3 inverse (matrix.h:211-231)
-void inverse3(const float *m, float *r) {
-    auto M = [&](int i, int j) { return m[3 * i + j]; };
-    float det = M(0, 0) * (M(1, 1) * M(2, 2) - M(2, 1) * M(1, 2)) - M(0, 1) * (M(1, 0) * M(2, 2) - M(1, 2) * M(2, 0)) +
-                M(0, 2) * (M(1, 0) * M(2, 1) - M(1, 1) * M(2, 0));
-    float invdet = 1 / det;
-    r[0] = (M(1, 1) * M(2, 2) - M(2, 1) * M(1, 2)) * invdet;
-    r[1] = (M(0, 2) * M(2, 1) - M(0, 1) * M(2, 2)) * invdet;
-    r[2] = (M(0, 1) * M(1, 2) - M(0, 2) * M(1, 1)) * invdet;
-    r[3] = (M(1, 2) * M(2, 0) - M(1, 0) * M(2, 2)) * invdet;
-    r[4] = (M(0, 0) * M(2, 2) - M(0, 2) * M(2, 0)) * invdet;
-    r[5] = (M(1, 0) * M(0, 2) - M(0, 0) * M(1, 2)) * invdet;
-    r[6] = (M(1, 0) * M(2, 1) - M(2, 0) * M(1, 1)) * invdet;
-    r[7] = (M(2, 0) * M(0, 1) - M(0, 0) * M(2, 1)) * invdet;
-    r[8] = (M(0, 0) * M(1, 1) - M(1, 0) * M(0, 1)) * invdet;
-}
-HV2 xform(const float *m, HV2 p) {
-    float t0 = m[0] * p.x + m[1] * p.y + m[2];
-    float t1 = m[3] * p.x + m[4] * p.y + m[5];
-    float t2 = m[6] * p.x + m[7] * p.y + m[8];
-    return {t0 / t2, t1 / t2};
-}
-
-HV2 qeval(HV2 p0, HV2 p1, HV2 p2, float t) {
-    float tt = 1 - t;
-    return (tt * tt) * p0 + (2 * tt * t) * p1 + (t * t) * p2;
-}
-HV2 ceval(HV2 p0, HV2 p1, HV2 p2, HV2 p3, float t) {
-    float tt = 1 - t;
-    return (tt * tt * tt) * p0 + (3 * tt * tt * t) * p1 + (3 * tt * t * t) * p2 + (t * t * t) * p3;
+// Leaf positions (0..n-1 in sorted order) in the order the traversal visits them: the
+// pairwise build's shape depends on n only, so this is topology the device rebuild reuses.
+const std::vector<int> &visit_positions(int n) {
+    static std::mutex mu;
+    static std::map<int, std::vector<int>> memo;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = memo.find(n);
+    if (it != memo.end()) return it->second;
+    std::vector<HNode> nodes(n > 0 ? 2 * n - 1 : 0, HNode{-1, -1, box_empty(), 0.f});
+    for (int i = 0; i < n; i++) nodes[i] = HNode{i, -1, box_empty(), 0.f};
+    build_bvh(nodes.data(), n, false, 1, 1);
+    std::vector<int> out;
+    dfs_leaves(nodes.data(), n, out);
+    return memo.emplace(n, std::move(out)).first->second;
 }
 
 // Packs host arrays into one device allocation with 256-byte aligned sub-arrays.
@@ -275,10 +226,10 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
                                                 d.shape_params ? d.shape_params + d.num_shape_params : nullptr);
     std::vector<uint8_t> has_thick(ns, 0), closed(ns, 0), approx(ns, 0);
     std::vector<float4> shape_box(ns);
-    std::vector<HBox> sbox(ns, hbox_empty());
+    std::vector<HBox> sbox(ns, box_empty());
     const int nseg = d.num_ctrl_total;
     std::vector<int4> seg_idx(nseg);
-    std::vector<int> seg_kind(nseg, 0), seg_pid(nseg, 0), seg_dfs(nseg, 0);
+    std::vector<int> seg_kind(nseg, 0), seg_pid(nseg, 0), seg_dfs(nseg, 0), seg_perm(nseg, 0);
     std::vector<float4> seg_box(nseg);
     std::vector<float> seg_maxr(nseg, 0.f), seg_cdf(nseg, 0.f), seg_pmf(nseg, 0.f);
     std::vector<float2> points(d.num_points_total);
@@ -299,19 +250,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
             int need = t == 0 ? 3 : 4;
             if (param_off[s] < 0 || param_off[s] + need > d.num_shape_params)
                 return set_error("shape parameter offset out of range");
-            const float *P = params.data() + param_off[s];
-            if (t == 0) {
-                sbox[s] = HBox{P[1] - P[0], P[2] - P[0], P[1] + P[0], P[2] + P[0]};
-                shape_length[s] = 0.f + (float)(2.f * 3.14159265358979323846) * P[0];
-            } else if (t == 1) {
-                sbox[s] = HBox{P[2] - P[0], P[3] - P[1], P[2] + P[0], P[3] + P[1]};
-                float a = P[0], b = P[1];
-                shape_length[s] = (float)(0.0 + (double)(float)3.14159265358979323846 *
-                                                     ((double)(3 * (a + b)) - std::sqrt((double)((3 * a + b) * (a + 3 * b)))));
-            } else {
-                sbox[s] = HBox{P[0], P[1], P[2], P[3]};
-                shape_length[s] = 0.f + 2 * (P[2] - P[0] + P[3] - P[1]);
-            }
+            shape_length[s] = bld::basic_shape(t, params.data() + param_off[s], sbox[s]);
         } else {
             param_off[s] = 0;
             int c0 = d.path_ctrl_offset[s], nb = d.path_num_base_points[s];
@@ -326,12 +265,8 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
             if (has_thick[s] && !d.thickness) return set_error("path with thickness but no thickness array");
             closed[s] = d.path_is_closed[s] ? 1 : 0;
             approx[s] = d.path_use_distance_approx[s] ? 1 : 0;
-            HBox box = hbox_empty();
-            {
-                HV2 q{points[p0].x, points[p0].y};
-                box = HBox{q.x, q.y, q.x, q.y};
-            }
-            for (int i = 1; i < np; i++) box = hmerge(box, HV2{points[p0 + i].x, points[p0 + i].y});
+            HBox box{points[p0].x, points[p0].y, points[p0].x, points[p0].y};
+            for (int i = 1; i < np; i++) box = merge(box, HV2{points[p0 + i].x, points[p0 + i].y});
             sbox[s] = box;
             float r = stroke_width[s];
             float length = 0.f;
@@ -361,10 +296,10 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
                 seg_idx[seg] = make_int4(p0 + li[0], p0 + li[1], k >= 1 ? p0 + li[2] : -1, k >= 2 ? p0 + li[3] : -1);
                 seg_kind[seg] = k;
                 seg_pid[seg] = pid;
-                HBox b = hbox_empty();
+                HBox b = box_empty();
                 float rr = -1;
                 for (int j = 0; j <= k + 1; j++) {
-                    b = hmerge(b, p[j]);
+                    b = merge(b, p[j]);
                     float rj = has_thick[s] ? thickness[p0 + li[j]] : r;
                     rr = j == 0 ? rj : mx(rr, rj);
                 }
@@ -372,18 +307,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
                 thick[i] = rr;
                 seg_box[seg] = make_float4(b.minx, b.miny, b.maxx, b.maxy);
                 seg_maxr[seg] = rr;
-                // compute_shape_length (scene.cpp:147-184)
-                float l;
-                if (k == 0) {
-                    l = hdist(p[1], p[0]);
-                } else if (k == 1) {
-                    HV2 v1 = qeval(p[0], p[1], p[2], 0.5f);
-                    l = hdist(v1, p[0]) + hdist(v1, p[2]);
-                } else {
-                    HV2 v1 = ceval(p[0], p[1], p[2], p[3], 1.f / 3.f);
-                    HV2 v2 = ceval(p[0], p[1], p[2], p[3], 2.f / 3.f);
-                    l = hdist(v1, p[0]) + hdist(v1, v2) + hdist(v2, p[3]);
-                }
+                float l = bld::segment_length(k, p);
                 seglen[i] = l;
                 length += l;
                 pid += k + 1;
@@ -400,16 +324,18 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
             std::vector<int> order(nb);
             std::iota(order.begin(), order.end(), 0);
             std::sort(order.begin(), order.end(), [&](int i0, int i1) {
-                float b0y = 0.5f * (boxes[i0].miny + boxes[i0].maxy);
-                float b1y = 0.5f * (boxes[i1].miny + boxes[i1].maxy);
-                return b0y < b1y;
+                return bld::box_ycentre(boxes[i0]) < bld::box_ycentre(boxes[i1]);
             });
-            pnodes.assign(2 * nb - 1, HNode{-1, -1, hbox_empty(), 0.f});
+            pnodes.assign(2 * nb - 1, HNode{-1, -1, box_empty(), 0.f});
             for (int i = 0; i < nb; i++) pnodes[i] = HNode{order[i], -1 - i, boxes[order[i]], thick[order[i]]};
             build_bvh(pnodes.data(), nb, false, sc.cw, sc.ch);
             std::vector<int> leaves;
             dfs_leaves(pnodes.data(), nb, leaves);
-            for (int i = 0; i < nb; i++) seg_dfs[c0 + i] = c0 + leaves[i];
+            const std::vector<int> &vp = visit_positions(nb);
+            for (int i = 0; i < nb; i++) {
+                seg_dfs[c0 + i] = c0 + leaves[i];
+                seg_perm[c0 + i] = vp[i];
+            }
             // the top-level radius of a thickness path reads leaf 0 of this BVH (scene.cpp:651-656)
             thick_leaf0[s] = pnodes[0].max_radius;
         }
@@ -418,14 +344,16 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     for (int s = 0; s < ns; s++) shape_box[s] = make_float4(sbox[s].minx, sbox[s].miny, sbox[s].maxx, sbox[s].maxy);
 
     // ---- groups
-    std::vector<int> g_off(ng), g_n(ng), g_ids(d.num_group_shape_ids);
+    std::vector<int> g_off(ng), g_n(ng), g_ids(d.num_group_shape_ids), g_ids_desc(d.num_group_shape_ids),
+        g_perm(d.num_group_shape_ids);
     std::vector<int> fill_type(ng), fill_off(ng), fill_ns(ng), stroke_type(ng), stroke_off(ng), stroke_ns(ng);
     std::vector<uint8_t> even_odd(ng);
     std::vector<float> s2c(9 * (size_t)ng), c2s(9 * (size_t)ng);
     std::vector<float4> g_lbox(ng), g_box(ng);
     std::vector<float> g_maxr(ng);
     std::vector<float4> g_sdf(ng);
-    std::vector<HNode> top(ng > 0 ? 2 * (size_t)ng - 1 : 1, HNode{-1, -1, hbox_empty(), 0.f});
+    std::vector<int> g_anchor(ng, -1), g_sdf_flags(ng, 0);
+    std::vector<HNode> top(ng > 0 ? 2 * (size_t)ng - 1 : 1, HNode{-1, -1, box_empty(), 0.f});
     int total = 0;
     for (int g = 0; g < ng; g++) {
         g_off[g] = d.group_shape_offset[g];
@@ -455,7 +383,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
             return set_error("paint description out of bounds");
         even_odd[g] = d.use_even_odd_rule[g] ? 1 : 0;
         std::memcpy(&s2c[9 * (size_t)g], d.shape_to_canvas + 9 * (size_t)g, 9 * sizeof(float));
-        inverse3(&s2c[9 * (size_t)g], &c2s[9 * (size_t)g]);
+        bld::inverse3(&s2c[9 * (size_t)g], &c2s[9 * (size_t)g]);
     }
     if (ns > 0) {
         // build_shape_cdfs (scene.cpp:207-246)
@@ -491,7 +419,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
         // group BVHs (Morton-sorted shape boxes) -> member visit order and root box
         for (int g = 0; g < ng; g++) {
             int n = g_n[g];
-            std::vector<HNode> nodes(2 * n - 1, HNode{-1, -1, hbox_empty(), 0.f});
+            std::vector<HNode> nodes(2 * n - 1, HNode{-1, -1, box_empty(), 0.f});
             for (int i = 0; i < n; i++) {
                 int s = d.group_shape_ids[g_off[g] + i];
                 float r = stroke_type[g] < 0 ? 0.f : stroke_width[s];
@@ -500,7 +428,12 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
             build_bvh(nodes.data(), n, true, sc.cw, sc.ch);
             std::vector<int> leaves;
             dfs_leaves(nodes.data(), n, leaves);
-            for (int i = 0; i < n; i++) g_ids[g_off[g] + i] = leaves[i];
+            const std::vector<int> &vp = visit_positions(n);
+            for (int i = 0; i < n; i++) {
+                g_ids[g_off[g] + i] = leaves[i];
+                g_ids_desc[g_off[g] + i] = d.group_shape_ids[g_off[g] + i];
+                g_perm[g_off[g] + i] = vp[i];
+            }
             HBox rb = nodes[2 * n - 2].box;
             g_lbox[g] = make_float4(rb.minx, rb.miny, rb.maxx, rb.maxy);
             // top-level leaf (scene.cpp:648-681)
@@ -512,11 +445,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
             float maxr = radius_of(d.group_shape_ids[g_off[g]]);
             for (int i = 1; i < n; i++) maxr = std::max(maxr, radius_of(d.group_shape_ids[g_off[g] + i]));
             const float *m = &s2c[9 * (size_t)g];
-            HBox tb = hbox_empty();
-            tb = hmerge(tb, xform(m, HV2{rb.minx, rb.miny}));
-            tb = hmerge(tb, xform(m, HV2{rb.minx, rb.maxy}));
-            tb = hmerge(tb, xform(m, HV2{rb.maxx, rb.miny}));
-            tb = hmerge(tb, xform(m, HV2{rb.maxx, rb.maxy}));
+            HBox tb = bld::canvas_box(m, rb);
             float r = stroke_type[g] < 0 ? 0.f : maxr;
             g_box[g] = make_float4(tb.minx, tb.miny, tb.maxx, tb.maxy);
             g_maxr[g] = r;
@@ -526,33 +455,30 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
             // closest-point search, and a pruned segment is farther than the running
             // minimum).  Valid when the transform preserves distance ratios and no member
             // uses the approximate quadratic distance.
-            bool similarity = m[6] == 0.f && m[7] == 0.f && m[8] == 1.f &&
-                              ((m[0] == m[4] && m[1] == -m[3]) || (m[0] == -m[4] && m[1] == m[3]));
-            bool findable = false, approx_quad = false;
-            HV2 anchor{0, 0};
-            bool have_anchor = false;
+            int flags = 0;
             for (int i = 0; i < n; i++) {
                 int s = d.group_shape_ids[g_off[g] + i];
                 if (shape_type[s] == 2 && nsegs[s] > 0) {
-                    findable = true;
+                    flags |= bld::kSdfFindable;
                     if (approx[s])
-                        for (int k = 0; k < nsegs[s]; k++) approx_quad |= seg_kind[seg_begin[s] + k] == 1;
-                    if (!have_anchor) {
-                        anchor = HV2{points[pt_off[s]].x, points[pt_off[s]].y};
-                        have_anchor = true;
-                    }
+                        for (int q = 0; q < nsegs[s]; q++)
+                            if (seg_kind[seg_begin[s] + q] == 1) flags |= bld::kSdfApproxQuad;
+                    if (g_anchor[g] < 0) g_anchor[g] = s;
                 } else if (shape_type[s] == 3) {
                     // a rect reports one of its corners, chosen by its nearest side, so
                     // no single point bounds its distance: findable, never an anchor
-                    findable = true;
+                    flags |= bld::kSdfFindable;
                 }
             }
+            g_sdf_flags[g] = flags;
             float qnan = std::numeric_limits<float>::quiet_NaN();
-            if (findable && have_anchor && similarity && !approx_quad) {
-                HV2 q = xform(m, anchor);
-                g_sdf[g] = make_float4(q.x, q.y, 1.f, 0.f);
+            float fz = (flags & bld::kSdfFindable) ? 1.f : 0.f;
+            if ((flags & bld::kSdfFindable) && g_anchor[g] >= 0 && bld::similarity(m) &&
+                !(flags & bld::kSdfApproxQuad)) {
+                HV2 q = bld::xform(m, HV2{points[pt_off[g_anchor[g]]].x, points[pt_off[g_anchor[g]]].y});
+                g_sdf[g] = make_float4(q.x, q.y, fz, 0.f);
             } else {
-                g_sdf[g] = make_float4(qnan, qnan, findable ? 1.f : 0.f, 0.f);
+                g_sdf[g] = make_float4(qnan, qnan, fz, 0.f);
             }
             top[g] = HNode{g, -1, tb, r};
         }
@@ -564,21 +490,30 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
         for (size_t i = 0; i < top.size(); i++)
             topd[i] = BNode{top[i].child0, top[i].child1, top[i].box.minx, top[i].box.miny, top[i].box.maxx,
                             top[i].box.maxy, top[i].max_radius, 0};
-        std::vector<int> g_ids_dfs = g_ids;
         size_t o_shape_type = P.add(shape_type), o_sw = P.add(stroke_width), o_poff = P.add(param_off),
                o_params = P.add(params), o_sb = P.add(seg_begin), o_nsegs = P.add(nsegs), o_ptoff = P.add(pt_off),
                o_npt = P.add(npt), o_thick_flag = P.add(has_thick), o_closed = P.add(closed), o_approx = P.add(approx),
                o_sbox = P.add(shape_box), o_sidx = P.add(seg_idx), o_skind = P.add(seg_kind), o_spid = P.add(seg_pid),
                o_sbx = P.add(seg_box), o_smr = P.add(seg_maxr), o_sdfs = P.add(seg_dfs), o_scdf = P.add(seg_cdf),
                o_spmf = P.add(seg_pmf), o_pts = P.add(points), o_th = P.add(thickness), o_goff = P.add(g_off),
-               o_gn = P.add(g_n), o_gids = P.add(g_ids_dfs), o_ft = P.add(fill_type), o_fo = P.add(fill_off),
+               o_gn = P.add(g_n), o_gids = P.add(g_ids), o_ft = P.add(fill_type), o_fo = P.add(fill_off),
                o_fn = P.add(fill_ns), o_st = P.add(stroke_type), o_so = P.add(stroke_off), o_sn = P.add(stroke_ns),
                o_color = P.add(std::vector<float>(d.color_params, d.color_params + d.num_color_params)),
                o_eo = P.add(even_odd), o_s2c = P.add(s2c), o_c2s = P.add(c2s), o_glb = P.add(g_lbox),
                o_gb = P.add(g_box), o_gmr = P.add(g_maxr), o_gsdf = P.add(g_sdf), o_top = P.add(topd),
-               o_len = P.add(shape_length),
-               o_cdf = P.add(cdf), o_pmf = P.add(pmf), o_sid = P.add(sid), o_gid = P.add(gid);
+               o_len = P.add(shape_length), o_cdf = P.add(cdf), o_pmf = P.add(pmf), o_sid = P.add(sid),
+               o_gid = P.add(gid);
+        // topology tables and scratch of the device rebuild (dvg_scene_update)
+        size_t o_sperm = P.add(seg_perm), o_gdesc = P.add(g_ids_desc), o_gperm = P.add(g_perm),
+               o_ganchor = P.add(g_anchor), o_gflags = P.add(g_sdf_flags), o_tl0 = P.add(thick_leaf0),
+               o_order = P.add(std::vector<int>(nseg, 0)), o_gorder = P.add(std::vector<int>(total, 0));
         sc.block_bytes = P.host.size();
+        sc.top_offset = o_top;
+        sc.top_bytes = topd.size() * sizeof(BNode);
+        sc.scratch_offset[0] = o_order;
+        sc.scratch_bytes[0] = (size_t)nseg * sizeof(int);
+        sc.scratch_offset[1] = o_gorder;
+        sc.scratch_bytes[1] = (size_t)total * sizeof(int);
         // validation and host-side construction are done: only now is a device needed
         if (gpu_index >= 0) HIP_OK(hipSetDevice(gpu_index));
         HIP_OK(hipGetDevice(&sc.device));
@@ -633,6 +568,15 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
         D.shapes_pmf = (const float *)(B + o_pmf);
         D.sample_shape_id = (const int *)(B + o_sid);
         D.sample_group_id = (const int *)(B + o_gid);
+        RefitTables &R = sc.refit;
+        R.seg_perm = (const int *)(B + o_sperm);
+        R.g_ids_desc = (const int *)(B + o_gdesc);
+        R.g_perm = (const int *)(B + o_gperm);
+        R.g_anchor = (const int *)(B + o_ganchor);
+        R.g_sdf_flags = (const int *)(B + o_gflags);
+        R.thick_leaf0 = (float *)(B + o_tl0);
+        R.order = (int *)(B + o_order);
+        R.gorder = (int *)(B + o_gorder);
     }
     DevScene &D = sc.dev;
     D.cw = sc.cw;
@@ -664,9 +608,9 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     D.d_xform = sc.grad + sc.g_xform;
     D.d_filter_radius = sc.grad + sc.g_filter;
     size_t eg = 0;
-    sc.err = (int *)cache_alloc(sc.device, sizeof(int), &eg);
+    sc.err = (int *)cache_alloc(sc.device, 2 * sizeof(int), &eg);
     if (!sc.err) return set_error("out of device memory");
-    if (int e = clear_device(sc.device, sc.err, sizeof(int))) return e;
+    if (int e = clear_device(sc.device, sc.err, 2 * sizeof(int))) return e;
     return 0;
 }
 
@@ -676,7 +620,7 @@ void free_scene(SceneHost &sc) {
     if (hipGetDevice(&prev) == hipSuccess) (void)hipSetDevice(sc.device);
     cache_free(sc.device, sc.block, sc.block_granted);
     cache_free(sc.device, sc.grad, sc.grad_granted);
-    cache_free(sc.device, sc.err, sizeof(int));
+    cache_free(sc.device, sc.err, 2 * sizeof(int));
     sc.block = nullptr;
     sc.grad = nullptr;
     sc.err = nullptr;
@@ -792,6 +736,43 @@ int dvg_batch_destroy(dvg_batch *batch) {
     if (!batch) return 0;
     dvg::batch_destroy(batch->b);
     delete batch;
+    return 0;
+}
+
+int dvg_scene_update(dvg_scene *scene, const dvg_scene_params *params, float filter_radius, void *stream) {
+    if (!scene || !params) return set_error("null argument");
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    if (scene->h.device_set && prev != scene->h.device) (void)hipSetDevice(scene->h.device);
+    int rc = update_scene(scene->h, *params, filter_radius, (hipStream_t)stream);
+    if (prev >= 0 && scene->h.device_set && prev != scene->h.device) (void)hipSetDevice(prev);
+    return rc;
+}
+
+int dvg_scene_status(dvg_scene *scene) {
+    if (!scene) return set_error("null scene");
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    if (scene->h.device_set && prev != scene->h.device) (void)hipSetDevice(scene->h.device);
+    int rc = scene_status(scene->h);
+    if (prev >= 0 && scene->h.device_set && prev != scene->h.device) (void)hipSetDevice(prev);
+    return rc;
+}
+
+int dvg_scene_debug_block(dvg_scene *scene, void *dst, int64_t cap, int64_t *bytes) {
+    if (!scene) return set_error("null scene");
+    SceneHost &h = scene->h;
+    if (bytes) *bytes = (int64_t)h.block_bytes;
+    if (!dst || !h.block || h.block_bytes == 0) return 0;
+    if (cap < (int64_t)h.block_bytes) return set_error("destination smaller than the scene block");
+    (void)hipSetDevice(h.device);
+    if (h.last_stream) HIP_OK(hipStreamSynchronize(h.last_stream));
+    HIP_OK(hipMemcpy(dst, h.block, h.block_bytes, hipMemcpyDeviceToHost));
+    // the PLAIN schedule's top-level BVH (not rebuilt by dvg_scene_update) and the
+    // rebuild's sort scratch are zeroed, so blocks of equal scenes compare equal
+    uint8_t *b = (uint8_t *)dst;
+    std::memset(b + h.top_offset, 0, h.top_bytes);
+    for (int i = 0; i < 2; i++) std::memset(b + h.scratch_offset[i], 0, h.scratch_bytes[i]);
     return 0;
 }
 

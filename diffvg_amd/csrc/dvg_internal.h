@@ -48,6 +48,19 @@ void cache_free(int device, void *p, size_t bytes);
 Workspace &acquire_workspace(int device, hipStream_t st);
 void release_workspace(int device, hipStream_t st);
 
+// Topology tables and scratch of the device rebuild (dvg_scene_update), in the scene
+// block next to the arrays it rewrites.
+struct RefitTables {
+    const int *seg_perm = nullptr;     // [nseg] per path: sorted-leaf positions in visit order
+    const int *g_ids_desc = nullptr;   // [n_ids] group members in description order
+    const int *g_perm = nullptr;       // [n_ids] per group: sorted-leaf positions in visit order
+    const int *g_anchor = nullptr;     // [ng] shape whose first point anchors the SDF bound, or -1
+    const int *g_sdf_flags = nullptr;  // [ng] bld::kSdfFindable | bld::kSdfApproxQuad
+    float *thick_leaf0 = nullptr;      // [ns] radius of leaf 0 of a thickness path's segment BVH
+    int *order = nullptr;              // [nseg] sort scratch
+    int *gorder = nullptr;             // [n_ids] sort scratch
+};
+
 struct SceneHost {
     int device = 0;
     bool device_set = false;      // a device was chosen and memory may be held on it
@@ -60,8 +73,12 @@ struct SceneHost {
     float *grad = nullptr;        // gradient arrays
     size_t grad_floats = 0, grad_granted = 0;
     size_t g_params = 0, g_points = 0, g_thick = 0, g_sw = 0, g_color = 0, g_xform = 0, g_filter = 0;
-    int *err = nullptr;
+    int *err = nullptr;            // [0] render conditions, [1] conditions found by dvg_scene_update
     hipStream_t last_stream = nullptr;
+    RefitTables refit;
+    size_t top_offset = 0, top_bytes = 0;          // top-level BVH in the block
+    size_t scratch_offset[2] = {0, 0}, scratch_bytes[2] = {0, 0};  // rebuild sort scratch in the block
+    bool top_stale = false;       // the top-level BVH (PLAIN schedule) predates a device update
     DevScene dev{};
 };
 
@@ -74,6 +91,8 @@ void batch_destroy(BatchHost *b);
 const std::vector<SceneHost *> &batch_scenes(const BatchHost &b);
 
 int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc);
+int update_scene(SceneHost &sc, const dvg_scene_params &p, float filter_radius, hipStream_t st);
+int scene_status(SceneHost &sc);
 void free_scene(SceneHost &sc);
 int render_tiled(SceneHost &sc, const dvg_render_args &a);
 int render_plain(SceneHost &sc, const dvg_render_args &a);

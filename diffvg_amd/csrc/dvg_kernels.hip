@@ -25,8 +25,10 @@
 #include <mutex>
 #include <vector>
 
+#include "dvg_build.h"
 #include "dvg_internal.h"
 #include "dvg_shade.h"
+#include "dvg_sort.h"
 
 namespace dvg {
 
@@ -376,16 +378,13 @@ DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const fl
         color = comp.result(has_bg, bgc);
     }
     if constexpr (PRE) {
+        // all lanes: paint, stroke-width and distance adjoints are combined across the wave
+        int nf = valid ? min(comp.nf, kMaxPFrags) : 0;
+        V4 d_back = composite_adjoint_prefiltered_wave(S, pfrags, nf, V3{color.x, color.y, color.z}, color.w,
+                                                       d_color, pt, d_tr);
         if (valid) {
-            if (comp.nf == 0) {
-                kind = 0;
-                dbg_val = d_color;
-            } else {
-                V3 fc{color.x, color.y, color.z};
-                dbg_val = composite_adjoint_prefiltered(S, pfrags, min(comp.nf, kMaxPFrags), fc, color.w, d_color,
-                                                        pt, d_tr);
-                kind = 1;
-            }
+            kind = comp.nf == 0 ? 0 : 1;
+            dbg_val = comp.nf == 0 ? d_color : d_back;
         }
     } else {
         // all lanes: the constant-paint adjoints are combined across the wave
@@ -773,25 +772,27 @@ DVG_DEV void sdf_tile(const DevScene &S, const Geom &G, int tile, float *sdf, co
         }
         __syncthreads();
     }
-    if (!valid) return;
     float weight = 1.f / G.spp;
     float dist_out = min_distance;
-    if (min_group >= 0) {
+    const bool found = valid && min_group >= 0;
+    bool in = false;
+    if (found) {
         dist_out = min_distance * weight;
-        bool in = false;
         if (S.fill_type[min_group] >= 0) {
             M3 c2s = m3_load(S.c2s + 9 * min_group);
             in = is_inside_local(S, min_group, xform_pt(c2s, pt), nullptr);
             if (in) dist_out = -dist_out;
         }
-        if (d_sdf) {
-            float dd = d_sdf[y * G.W + x];
-            float d_abs = in ? -dd : dd;
-            float *d_tr = d_translation ? d_translation + 2 * (y * G.W + x) : nullptr;
-            d_compute_distance(S, min_group, min_shape, pt, closest, min_info, d_abs, d_tr);
-        }
     }
-    if (sdf) aadd(sdf + (y * G.W + x), dist_out);
+    if (d_sdf) {
+        // every lane: the distance adjoints of the tile's samples are combined per wave
+        float dd = found ? d_sdf[y * G.W + x] : 0.f;
+        float d_abs = in ? -dd : dd;
+        float *d_tr = (found && d_translation) ? d_translation + 2 * (y * G.W + x) : nullptr;
+        d_compute_distance_wave(S, found, found ? min_group : 0, found ? min_shape : 0, pt, closest, min_info, d_abs,
+                                d_tr);
+    }
+    if (valid && sdf) aadd(sdf + (y * G.W + x), dist_out);
 }
 __global__ __launch_bounds__(kBlock) void k_sdf_tiled(DevScene S, Geom G, float *sdf, const float *d_sdf,
                                                       float *d_translation) {
@@ -1149,13 +1150,30 @@ static Geom make_geom(const SceneHost &sc, const dvg_render_args &a) {
         if (e_ != hipSuccess) return set_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-static int check_err_flag(SceneHost &sc, hipStream_t st) {
-    int h = 0;
-    DVG_CHECK(hipMemcpyAsync(&h, sc.err, sizeof(int), hipMemcpyDeviceToHost, st));
-    DVG_CHECK(hipStreamSynchronize(st));
-    if (h & 2) return set_error("prefiltered stroke of a group without a path or rect shape (the reference asserts)");
-    if (h & 1) return set_error("fragment list overflow: more overlapping fragments than the reference supports");
+// conditions found by dvg_scene_update's kernels (err[1])
+enum { kUpdNonFinitePoints = 1, kUpdNonFiniteLength = 2, kUpdTotalLength = 4 };
+
+static int report_flags(const int *h) {
+    if (h[1] & kUpdNonFinitePoints) return set_error("non-finite path points or thickness");
+    if (h[1] & kUpdNonFiniteLength) return set_error("non-finite shape boundary length");
+    if (h[1] & kUpdTotalLength)
+        return set_error("The total length of the shape boundaries in the scene is equal or less than 0, or not a "
+                         "number");
+    if (h[0] & 2) return set_error("prefiltered stroke of a group without a path or rect shape (the reference asserts)");
+    if (h[0] & 1) return set_error("fragment list overflow: more overlapping fragments than the reference supports");
     return 0;
+}
+
+static int check_err_flag(SceneHost &sc, hipStream_t st) {
+    int h[2] = {0, 0};
+    DVG_CHECK(hipMemcpyAsync(h, sc.err, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    DVG_CHECK(hipStreamSynchronize(st));
+    return report_flags(h);
+}
+
+int scene_status(SceneHost &sc) {
+    if (!sc.device_set || !sc.err) return 0;
+    return check_err_flag(sc, sc.last_stream);
 }
 
 // A scene without shapes (and so without groups): every sample sees only the
@@ -1271,6 +1289,9 @@ int render_plain(SceneHost &sc, const dvg_render_args &a) {
     DevScene S = sc.dev;
     if (eval_mode && (a.render_image || a.d_render_image))
         return set_error("eval_positions are supported for the SDF output only (as in the reference)");
+    if (sc.top_stale)
+        return set_error("the PLAIN schedule needs a scene built by dvg_scene_create: this scene was updated on the "
+                         "device, which does not rebuild the top-level BVH");
     DVG_CHECK(hipMemsetAsync(sc.err, 0, sizeof(int), st));
     if (sc.num_shapes == 0) {
         if (int e = render_empty(a, npix, st)) return e;
@@ -1731,17 +1752,16 @@ int batch_status(BatchHost &B) {
     (void)hipGetDevice(&prev);
     if (prev != B.device) (void)hipSetDevice(B.device);
     DVG_CHECK(hipStreamSynchronize(B.last_stream));
-    int flags = 0;
+    int flags[2] = {0, 0};
     for (auto *sc : B.scenes) {
-        int h = 0;
-        DVG_CHECK(hipMemcpyAsync(&h, sc->err, sizeof(int), hipMemcpyDeviceToHost, B.last_stream));
+        int h[2] = {0, 0};
+        DVG_CHECK(hipMemcpyAsync(h, sc->err, 2 * sizeof(int), hipMemcpyDeviceToHost, B.last_stream));
         DVG_CHECK(hipStreamSynchronize(B.last_stream));
-        flags |= h;
+        flags[0] |= h[0];
+        flags[1] |= h[1];
     }
     if (prev >= 0 && prev != B.device) (void)hipSetDevice(prev);
-    if (flags & 2) return set_error("prefiltered stroke of a group without a path or rect shape (the reference asserts)");
-    if (flags & 1) return set_error("fragment list overflow: more overlapping fragments than the reference supports");
-    return 0;
+    return report_flags(flags);
 }
 
 const std::vector<SceneHost *> &batch_scenes(const BatchHost &b) { return b.scenes; }
@@ -1756,6 +1776,199 @@ void batch_destroy(BatchHost *B) {
     if (B->ws) cache_free(B->device, B->ws, B->ws_granted);
     if (prev >= 0) (void)hipSetDevice(prev);
     delete B;
+}
+
+// ------------------------------------------------------------------ device scene rebuild
+// dvg_scene_update: new parameter values for an unchanged topology.  The arithmetic is
+// dvg_build.h's, which the host builder uses too, and the leaf orders come from the
+// reference's std::sort restated in dvg_sort.h, so every table equals the one
+// dvg_scene_create derives from the same values.
+struct RefitArgs {
+    DevScene S;
+    RefitTables R;
+    int *uerr;
+};
+
+__global__ void k_refit_clear(int *uerr) {
+    if (threadIdx.x == 0) uerr[0] = 0;
+}
+
+template <class T>
+DVG_DEV T *writable(const T *p) {
+    return const_cast<T *>(p);
+}
+DVG_DEV bld::Box to_box(float4 b) { return bld::Box{b.x, b.y, b.z, b.w}; }
+DVG_DEV float4 to_f4(const bld::Box &b) { return make_float4(b.minx, b.miny, b.maxx, b.maxy); }
+
+// Per shape (scene.cpp:113-333, 496-619): local box, boundary length; for paths the
+// segment boxes, radii, length CDF/PMF and the segment BVH's leaf visit order.
+__global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
+    const DevScene &S = A.S;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S.ns) return;
+    const int type = S.shape_type[s];
+    bld::Box box;
+    float len;
+    if (type != 2) {
+        len = bld::basic_shape(type, S.params + S.param_off[s], box);
+    } else {
+        const int c0 = S.seg_begin[s], nb = S.nsegs[s], p0 = S.pt_off[s], np = S.npt[s];
+        const bool thick = S.has_thick[s] != 0;
+        const float r = S.stroke_width[s];
+        bool finite = true;
+        box = bld::Box{S.points[p0].x, S.points[p0].y, S.points[p0].x, S.points[p0].y};
+        for (int i = 0; i < np; i++) {
+            float2 q = S.points[p0 + i];
+            finite = finite && isfinite(q.x) && isfinite(q.y) && (!thick || isfinite(S.thickness[p0 + i]));
+            if (i > 0) box = bld::merge(box, bld::P2{q.x, q.y});
+        }
+        float4 *seg_box = writable(S.seg_box);
+        float *seg_maxr = writable(S.seg_maxr), *seg_pmf = writable(S.seg_pmf), *seg_cdf = writable(S.seg_cdf);
+        float length = 0.f;
+        for (int i = 0; i < nb; i++) {
+            const int seg = c0 + i, k = S.seg_kind[seg];
+            const int4 ix = S.seg_idx[seg];
+            const int id[4] = {ix.x, ix.y, ix.z, ix.w};
+            bld::P2 p[4];
+            bld::Box b = bld::box_empty();
+            float rr = -1;
+            for (int j = 0; j <= k + 1; j++) {
+                float2 q = S.points[id[j]];
+                p[j] = bld::P2{q.x, q.y};
+                b = bld::merge(b, p[j]);
+                float rj = thick ? S.thickness[id[j]] : r;
+                rr = j == 0 ? rj : bld::mx(rr, rj);
+            }
+            seg_box[seg] = to_f4(b);
+            seg_maxr[seg] = rr;
+            float l = bld::segment_length(k, p);
+            seg_pmf[seg] = l;
+            length += l;
+        }
+        len = 0.f + length;
+        const float inv_length = 1.f / len;
+        for (int i = 0; i < nb; i++) {
+            float dd = seg_pmf[c0 + i] * inv_length;
+            seg_pmf[c0 + i] = dd;
+            seg_cdf[c0 + i] = i == 0 ? dd : dd + seg_cdf[c0 + i - 1];
+        }
+        // leaves sorted by box y-centre (scene.cpp:602-611), visited in the pairwise
+        // BVH's depth-first order
+        int *order = A.R.order + c0;
+        for (int i = 0; i < nb; i++) order[i] = i;
+        int stack[192];
+        stl::sort(
+            order, nb,
+            [&](int a, int b) {
+                return bld::box_ycentre(to_box(seg_box[c0 + a])) < bld::box_ycentre(to_box(seg_box[c0 + b]));
+            },
+            stack);
+        int *seg_dfs = writable(S.seg_dfs);
+        for (int i = 0; i < nb; i++) seg_dfs[c0 + i] = c0 + order[A.R.seg_perm[c0 + i]];
+        A.R.thick_leaf0[s] = seg_maxr[c0 + order[0]];
+        if (!finite) atomicOr(A.uerr, kUpdNonFinitePoints);
+    }
+    writable(S.shape_box)[s] = to_f4(box);
+    writable(S.shape_length)[s] = len;
+    if (!isfinite(len)) atomicOr(A.uerr, kUpdNonFiniteLength);
+}
+
+// Per group (scene.cpp:621-684, shape.h:123): canvas_to_shape, member visit order of the
+// Morton-sorted group BVH, local root box, canvas-space leaf box and radius, SDF anchor.
+__global__ __launch_bounds__(kBlock) void k_refit_groups(RefitArgs A) {
+    const DevScene &S = A.S;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= S.ng) return;
+    const float *m = S.s2c + 9 * (size_t)g;
+    bld::inverse3(m, writable(S.c2s) + 9 * (size_t)g);
+    const int off = S.g_off[g], n = S.g_n[g];
+    const int *desc = A.R.g_ids_desc + off;
+    int *ord = A.R.gorder + off;
+    for (int i = 0; i < n; i++) ord[i] = i;
+    int stack[192];
+    stl::sort(
+        ord, n,
+        [&](int a, int b) {
+            return bld::morton_of_box(to_box(S.shape_box[desc[a]]), S.cw, S.ch) <
+                   bld::morton_of_box(to_box(S.shape_box[desc[b]]), S.cw, S.ch);
+        },
+        stack);
+    int *g_ids = writable(S.g_ids);
+    for (int i = 0; i < n; i++) g_ids[off + i] = desc[ord[A.R.g_perm[off + i]]];
+    bld::Box lb = bld::box_empty();
+    float maxr = 0.f;
+    for (int i = 0; i < n; i++) {
+        const int sh = desc[i];
+        lb = bld::merge(lb, to_box(S.shape_box[sh]));
+        float r = (S.shape_type[sh] == 2 && S.has_thick[sh]) ? A.R.thick_leaf0[sh] : S.stroke_width[sh];
+        maxr = i == 0 ? r : (maxr < r ? r : maxr);
+    }
+    writable(S.g_lbox)[g] = to_f4(lb);
+    writable(S.g_box)[g] = to_f4(bld::canvas_box(m, lb));
+    writable(S.g_maxr)[g] = S.stroke_type[g] < 0 ? 0.f : maxr;
+    const int flags = A.R.g_sdf_flags[g], anchor = A.R.g_anchor[g];
+    const float fz = (flags & bld::kSdfFindable) ? 1.f : 0.f;
+    float4 sd = make_float4(__builtin_nanf(""), __builtin_nanf(""), fz, 0.f);
+    if ((flags & bld::kSdfFindable) && anchor >= 0 && bld::similarity(m) && !(flags & bld::kSdfApproxQuad)) {
+        float2 q0 = S.points[S.pt_off[anchor]];
+        bld::P2 q = bld::xform(m, bld::P2{q0.x, q0.y});
+        sd = make_float4(q.x, q.y, fz, 0.f);
+    }
+    writable(S.g_sdf)[g] = sd;
+}
+
+// build_shape_cdfs (scene.cpp:207-246): the running sum is sequential, as in the
+// reference, so the CDF has its rounding; then every entry is normalised.
+__global__ __launch_bounds__(kBlock) void k_refit_cdf(RefitArgs A) {
+    const DevScene &S = A.S;
+    const int total = S.num_total_shapes;
+    float *cdf = writable(S.shapes_cdf), *pmf = writable(S.shapes_pmf);
+    __shared__ float s_norm;
+    if (threadIdx.x == 0) {
+        float acc = 0.f;
+        for (int k = 0; k < total; k++) {
+            float l = S.shape_length[S.sample_shape_id[k]];
+            acc = k == 0 ? l : l + acc;
+            cdf[k] = acc;
+            pmf[k] = l;
+        }
+        s_norm = acc;
+        if (!(acc > 0.f) || !isfinite(acc)) atomicOr(A.uerr, kUpdTotalLength);
+    }
+    __syncthreads();
+    const float norm = s_norm;
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+        cdf[k] /= norm;
+        pmf[k] /= norm;
+    }
+}
+
+int update_scene(SceneHost &sc, const dvg_scene_params &p, float filter_radius, hipStream_t st) {
+    if (!(filter_radius > 0) || !std::isfinite(filter_radius)) return set_error("filter radius must be positive");
+    sc.dev.filter_radius = filter_radius;
+    if (!sc.device_set || sc.num_shapes == 0) return 0;
+    DevScene &D = sc.dev;
+    auto copy = [&](const void *dst, const float *src, size_t floats) -> int {
+        if (src && floats)
+            DVG_CHECK(hipMemcpyAsync(const_cast<void *>(dst), src, floats * sizeof(float), hipMemcpyDeviceToDevice, st));
+        return 0;
+    };
+    if (copy(D.params, p.shape_params, sc.num_shape_params) || copy(D.points, p.points, 2 * (size_t)D.npts) ||
+        copy(D.thickness, p.thickness, D.npts) || copy(D.stroke_width, p.stroke_width, D.ns) ||
+        copy(D.color, p.color_params, sc.num_color_params) || copy(D.s2c, p.shape_to_canvas, 9 * (size_t)D.ng))
+        return 1;
+    RefitArgs A{D, sc.refit, sc.err + 1};
+    {
+        DVG_PROF(st, "k_refit");
+        k_refit_clear<<<1, 64, 0, st>>>(A.uerr);
+        k_refit_shapes<<<(D.ns + kBlock - 1) / kBlock, kBlock, 0, st>>>(A);
+        if (D.ng > 0) k_refit_groups<<<(D.ng + kBlock - 1) / kBlock, kBlock, 0, st>>>(A);
+        k_refit_cdf<<<1, kBlock, 0, st>>>(A);
+    }
+    DVG_CHECK(hipGetLastError());
+    sc.top_stale = true;
+    sc.last_stream = st;
+    return 0;
 }
 
 }  // namespace dvg

@@ -577,12 +577,28 @@ DVG_DEV bool compute_distance(const DevScene &S, int g, V2 pt, float max_radius,
 // d_closest_point (compute_distance.h:442-897).  The quadratic interior branch of the
 // reference declares new d_p0..d_p2 that shadow the accumulated ones, so only d_pt
 // receives its adjoint there.
-DVG_DEV void d_closest_point_path(const DevScene &S, int s, V2 pt, V2 d_cp, const PathInfo &info, V2 &d_pt) {
+// Control-point adjoints of one segment: slot j adds d[j] to point idx[j] when use[j].
+struct SegPointAdj {
+    V2 d[4];
+    int idx[4];
+    bool use[4];
+};
+
+DVG_DEV int d_closest_point_path_vals(const DevScene &S, int s, V2 pt, V2 d_cp, const PathInfo &info, V2 &d_pt,
+                                      SegPointAdj &out) {
     int seg = S.seg_begin[s] + info.base_point_id;
     V2 p[4];
     int4 idx;
     int k = seg_load(S, seg, p, idx);
-    float *dpts = S.d_points;
+    out.idx[0] = idx.x;
+    out.idx[1] = idx.y;
+    out.idx[2] = idx.z;
+    out.idx[3] = idx.w;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        out.d[j] = V2{0, 0};
+        out.use[j] = false;
+    }
     if (k == 0) {
         V2 p0 = p[0], p1 = p[1];
         float t = dot(pt - p0, p1 - p0) / dot(p1 - p0, p1 - p0);
@@ -595,8 +611,9 @@ DVG_DEV void d_closest_point_path(const DevScene &S, int s, V2 pt, V2 d_cp, cons
             d_p0 += d_cp * (1 - t);
             d_p1 += d_cp * t;
         }
-        aadd2(dpts + 2 * idx.x, d_p0);
-        aadd2(dpts + 2 * idx.y, d_p1);
+        out.d[0] = d_p0;
+        out.d[1] = d_p1;
+        out.use[0] = out.use[1] = true;
     } else if (k == 1) {
         V2 p0 = p[0], p1 = p[1], p2 = p[2];
         V2 d_p0{0, 0}, d_p2{0, 0};
@@ -619,8 +636,9 @@ DVG_DEV void d_closest_point_path(const DevScene &S, int s, V2 pt, V2 d_cp, cons
                 d_pt += d_C * (-(p0 - 2 * p1 + p2)) + d_D * (-(-p0 + p1));
             }
         }
-        aadd2(dpts + 2 * idx.x, d_p0);
-        aadd2(dpts + 2 * idx.z, d_p2);
+        out.d[0] = d_p0;
+        out.d[2] = d_p2;
+        out.use[0] = out.use[2] = true;
     } else {
         V2 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3];
         V2 d_p0{0, 0}, d_p1{0, 0}, d_p2{0, 0}, d_p3{0, 0};
@@ -683,14 +701,25 @@ DVG_DEV void d_closest_point_path(const DevScene &S, int s, V2 pt, V2 d_cp, cons
                 d_pt += dmul(d_F, (-1) * c);
             }
         }
-        aadd2(dpts + 2 * idx.x, d_p0);
-        aadd2(dpts + 2 * idx.y, d_p1);
-        aadd2(dpts + 2 * idx.z, d_p2);
-        aadd2(dpts + 2 * idx.w, d_p3);
+        out.d[0] = d_p0;
+        out.d[1] = d_p1;
+        out.d[2] = d_p2;
+        out.d[3] = d_p3;
+        out.use[0] = out.use[1] = out.use[2] = out.use[3] = true;
     }
+    return seg;
 }
 
-DVG_DEV void d_closest_point_rect(const DevScene &S, int s, V2 pt, V2 d_cp, V2 &d_pt) {
+DVG_DEV void d_closest_point_path(const DevScene &S, int s, V2 pt, V2 d_cp, const PathInfo &info, V2 &d_pt) {
+    SegPointAdj a;
+    d_closest_point_path_vals(S, s, pt, d_cp, info, d_pt, a);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (a.use[j]) aadd2(S.d_points + 2 * a.idx[j], a.d[j]);
+}
+
+// d_pmin (v[0..1]) and d_pmax (v[2..3]) of a rect
+DVG_DEV void d_closest_point_rect_vals(const DevScene &S, int s, V2 pt, V2 d_cp, V2 &d_pt, float (&v)[4]) {
     const float *P = S.params + S.param_off[s];
     auto seg_dist = [&](V2 p0, V2 p1) -> float {
         float t = dot(pt - p0, p1 - p0) / dot(p1 - p0, p1 - p0);
@@ -749,24 +778,53 @@ DVG_DEV void d_closest_point_rect(const DevScene &S, int s, V2 pt, V2 d_cp, V2 &
     d_pmin.x += d_lb.x;
     d_pmax.y += d_lb.y;
     d_pmax += d_rb;
+    v[0] = d_pmin.x;
+    v[1] = d_pmin.y;
+    v[2] = d_pmax.x;
+    v[3] = d_pmax.y;
+}
+
+DVG_DEV void d_closest_point_rect(const DevScene &S, int s, V2 pt, V2 d_cp, V2 &d_pt) {
+    float v[4];
+    d_closest_point_rect_vals(S, s, pt, d_cp, d_pt, v);
     float *dp = S.d_params + S.param_off[s];
-    aadd2(dp, d_pmin);
-    aadd2(dp + 2, d_pmax);
+    aadd2(dp, V2{v[0], v[1]});
+    aadd2(dp + 2, V2{v[2], v[3]});
+}
+
+// d_radius (v[0]) and d_center (v[1..2]) of a circle
+DVG_DEV void d_closest_point_circle_vals(const DevScene &S, int s, V2 pt, V2 d_cp, float (&v)[4]) {
+    const float *P = S.params + S.param_off[s];
+    V2 c{P[1], P[2]};
+    V2 dn = d_normalize(pt - c, P[0] * d_cp);
+    V2 dc = d_cp * V2{1 + dn.x, 1 + dn.y};
+    v[0] = dot(d_cp, normalize(pt - c));
+    v[1] = dc.x;
+    v[2] = dc.y;
+    v[3] = 0.f;
 }
 
 DVG_DEV void d_closest_point_circle(const DevScene &S, int s, V2 pt, V2 d_cp) {
-    const float *P = S.params + S.param_off[s];
+    float v[4];
+    d_closest_point_circle_vals(S, s, pt, d_cp, v);
     float *dp = S.d_params + S.param_off[s];
-    V2 c{P[1], P[2]};
-    V2 dn = d_normalize(pt - c, P[0] * d_cp);
-    aadd2(dp + 1, d_cp * V2{1 + dn.x, 1 + dn.y});
-    aadd(dp, dot(d_cp, normalize(pt - c)));
+    aadd2(dp + 1, V2{v[1], v[2]});
+    aadd(dp, v[0]);
 }
 
-// d_compute_distance (compute_distance.h:899-949)
-DVG_DEV void d_compute_distance(const DevScene &S, int g, int s, V2 pt, V2 closest, const PathInfo &info, float d_dist,
-                                float *d_translation) {
-    if (dist_sq(pt, closest) < 1e-10f) return;
+// Adjoint values of one d_compute_distance call, before they are scattered.
+struct DistAdj {
+    float xform[9];   // shape_to_canvas of the group
+    SegPointAdj pts;  // path: control points
+    int seg;          // path: segment (the key of its points)
+    float par[4];     // circle (3) or rect (4) parameters
+    int npar;
+    V2 d_pt;          // canvas point (-> d_translation)
+};
+
+// d_compute_distance (compute_distance.h:899-949) up to the scatter
+DVG_DEV void d_compute_distance_vals(const DevScene &S, int g, int s, V2 pt, V2 closest, const PathInfo &info,
+                                     float d_dist, DistAdj &out) {
     M3 c2s = m3_load(S.c2s + 9 * g);
     M3 s2c = m3_load(S.s2c + 9 * g);
     V2 local = xform_pt(c2s, pt);
@@ -780,12 +838,19 @@ DVG_DEV void d_compute_distance(const DevScene &S, int g, int s, V2 pt, V2 close
     d_xform_pt(s2c, local_cp, d_cp, d_s2c, d_local_cp);
     V2 d_local{0, 0};
     int type = S.shape_type[s];
-    if (type == 2)
-        d_closest_point_path(S, s, local, d_local_cp, info, d_local);
-    else if (type == 3)
-        d_closest_point_rect(S, s, local, d_local_cp, d_local);
-    else if (type == 0)
-        d_closest_point_circle(S, s, local, d_local_cp);
+    out.npar = 0;
+    out.seg = -1;
+#pragma unroll
+    for (int j = 0; j < 4; j++) out.pts.use[j] = false;
+    if (type == 2) {
+        out.seg = d_closest_point_path_vals(S, s, local, d_local_cp, info, d_local, out.pts);
+    } else if (type == 3) {
+        d_closest_point_rect_vals(S, s, local, d_local_cp, d_local, out.par);
+        out.npar = 4;
+    } else if (type == 0) {
+        d_closest_point_circle_vals(S, s, local, d_local_cp, out.par);
+        out.npar = 3;
+    }
     M3 d_c2s;
 #pragma unroll
     for (int i = 0; i < 9; i++) d_c2s.m[i] = 0.f;
@@ -810,10 +875,31 @@ DVG_DEV void d_compute_distance(const DevScene &S, int g, int s, V2 pt, V2 close
             for (int k = 0; k < 3; k++) acc += A[3 * i + k] * c2s.m[3 * j + k];
             B[3 * i + j] = acc;
         }
+#pragma unroll
+    for (int i = 0; i < 9; i++) out.xform[i] = d_s2c.m[i] + B[i];
+    out.d_pt = d_pt;
+}
+
+DVG_DEV void d_compute_distance(const DevScene &S, int g, int s, V2 pt, V2 closest, const PathInfo &info, float d_dist,
+                                float *d_translation) {
+    if (dist_sq(pt, closest) < 1e-10f) return;
+    DistAdj a;
+    d_compute_distance_vals(S, g, s, pt, closest, info, d_dist, a);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (a.pts.use[j]) aadd2(S.d_points + 2 * a.pts.idx[j], a.pts.d[j]);
+    float *dp = S.d_params + (a.npar ? S.param_off[s] : 0);
+    if (a.npar == 4) {
+        aadd2(dp, V2{a.par[0], a.par[1]});
+        aadd2(dp + 2, V2{a.par[2], a.par[3]});
+    } else if (a.npar == 3) {
+        aadd2(dp + 1, V2{a.par[1], a.par[2]});
+        aadd(dp, a.par[0]);
+    }
     float *dx = S.d_xform + 9 * g;
 #pragma unroll
-    for (int i = 0; i < 9; i++) aadd(dx + i, d_s2c.m[i] + B[i]);
-    if (d_translation) aadd2(d_translation, -d_pt);
+    for (int i = 0; i < 9; i++) aadd(dx + i, a.xform[i]);
+    if (d_translation) aadd2(d_translation, -a.d_pt);
 }
 
 }  // namespace dvg

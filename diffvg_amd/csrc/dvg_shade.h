@@ -737,6 +737,133 @@ DVG_DEV void accumulate_boundary_gradient_wave(const DevScene &S, bool active, i
     }
 }
 
+// d_compute_distance for a whole wave (every lane calls it; lanes without a distance
+// adjoint pass active = false): the values of the per-lane version, with the point,
+// shape-parameter and transform adjoints of lanes that share a segment, shape or group
+// added together before the atomics.  A distance field or a prefiltered image of a few
+// shapes sends nearly every sample's adjoint to the same dozen addresses.
+DVG_DEV void d_compute_distance_wave(const DevScene &S, bool active, int g, int s, V2 pt, V2 closest,
+                                     const PathInfo &info, float d_dist, float *d_translation) {
+    const bool act = active && !(dist_sq(pt, closest) < 1e-10f);
+    DistAdj a;
+    if (act) {
+        d_compute_distance_vals(S, g, s, pt, closest, info, d_dist, a);
+    } else {
+        a.seg = -1;
+        a.npar = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            a.pts.use[j] = false;
+            a.pts.idx[j] = 0;
+            a.pts.d[j] = V2{0, 0};
+            a.par[j] = 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < 9; i++) a.xform[i] = 0.f;
+        a.d_pt = V2{0, 0};
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const bool on = act && a.pts.use[j];
+        float v[2] = {a.pts.d[j].x, a.pts.d[j].y};
+        wave_combine_add<2>(on, a.seg, v, S.d_points + (on ? 2 * a.pts.idx[j] : 0));
+    }
+    {
+        // circle: radius, centre; rect: p_min, p_max -- contiguous at the shape's offset
+        const bool on = act && a.npar > 0;
+        wave_combine_add<4>(on, s, a.par, S.d_params + (on ? S.param_off[s] : 0), a.npar);
+    }
+    wave_combine_add<9>(act, g, a.xform, S.d_xform + (act ? 9 * g : 0));
+    if (act && d_translation) aadd2(d_translation, -a.d_pt);
+}
+
+// composite_adjoint_prefiltered for a whole wave, lanes in lockstep by fragment position
+// (as composite_adjoint_wave): constant paints, stroke widths and distance adjoints are
+// combined across lanes.  Every lane must call it; lanes without fragments pass nf = 0.
+DVG_DEV V4 composite_adjoint_prefiltered_wave(const DevScene &S, const PFragRec *frags, int nf, V3 final_color,
+                                              float final_alpha, V4 d_color, V2 pt, float *d_translation) {
+    V3 d_final_color{d_color.x, d_color.y, d_color.z};
+    V3 d_curr_color = d_final_color;
+    float d_curr_alpha = d_color.w;
+    if (nf > 0 && final_alpha > 1e-6f) {
+        d_curr_color = vdiv(d_final_color, final_alpha);
+        d_curr_alpha -= vsum(d_final_color * final_color) / final_alpha;
+    }
+    int mx = nf;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    for (int i = mx - 1; i >= 0; i--) {
+        const bool on = i < nf;
+        PFragRec f{};
+        if (on) f = frags[i];
+        int g = 0, s = 0, type = -1, off = 0;
+        bool paint_on = false, dist_on = false, sw_on = false;
+        float d_d = 0.f, d_sw = 0.f;
+        V4 dci{0, 0, 0, 0};
+        if (on) {
+            float d_prev_alpha = d_curr_alpha * (1.f - f.alpha);
+            float d_alpha_i = d_curr_alpha * (1.f - f.prev_alpha);
+            d_alpha_i += vsum(d_curr_color * (f.color - f.prev_color));
+            V3 d_prev_color = d_curr_color * (1 - f.alpha);
+            V3 d_color_i = d_curr_color * f.alpha;
+            g = f.gid >> 1;
+            s = f.shape_id;
+            float d = f.distance;
+            int ns = 0;
+            if (f.gid & 1) {
+                type = S.stroke_type[g];
+                off = S.stroke_off[g];
+                ns = S.stroke_ns[g];
+                float sw = S.stroke_width[s];
+                float ad = (float)fabs((double)d);
+                float apw = ad + sw, amw = ad - sw;
+                float w = smoothstep(apw) - smoothstep(amw);
+                if (w != 0) {
+                    float d_w = w > 0 ? (f.alpha / w) * d_alpha_i : 0.f;
+                    d_alpha_i *= w;
+                    paint_on = true;
+                    dci = V4{d_color_i.x, d_color_i.y, d_color_i.z, d_alpha_i};
+                    float d_apw = d_smoothstep(apw, d_w);
+                    float d_amw = -d_smoothstep(amw, d_w);
+                    d_d = d_apw + d_amw;
+                    if (d < 0) d_d = -d_d;
+                    d_sw = d_apw - d_amw;
+                    dist_on = fabs((double)d_d) > 1e-10f;
+                    sw_on = true;
+                }
+            } else {
+                type = S.fill_type[g];
+                off = S.fill_off[g];
+                ns = S.fill_ns[g];
+                float w = smoothstep(d);
+                if (w != 0) {
+                    float d_w = w > 0 ? (f.alpha / w) * d_alpha_i : 0.f;
+                    d_alpha_i *= w;
+                    paint_on = true;
+                    dci = V4{d_color_i.x, d_color_i.y, d_color_i.z, d_alpha_i};
+                    d_d = d_smoothstep(d, d_w);
+                    if (d < 0) d_d = -d_d;
+                    dist_on = fabs((double)d_d) > 1e-10f && f.within;
+                }
+            }
+            if (paint_on && type != 0) d_sample_paint(S, type, off, ns, pt, dci, d_translation);
+            d_curr_color = d_prev_color;
+            d_curr_alpha = d_prev_alpha;
+        }
+        {
+            float v[4] = {dci.x, dci.y, dci.z, dci.w};
+            const bool konst = paint_on && type == 0;
+            wave_combine_add<4>(konst, on ? f.gid : 0, v, S.d_color + (konst ? off : 0));
+        }
+        d_compute_distance_wave(S, dist_on, g, s, pt, on ? f.closest : pt, f.info, d_d, d_translation);
+        {
+            float v[1] = {d_sw};
+            wave_combine_add<1>(sw_on, s, v, S.d_stroke_width + (sw_on ? s : 0));
+        }
+    }
+    return V4{d_curr_color.x, d_curr_color.y, d_curr_color.z, d_curr_alpha};
+}
+
 // composite_adjoint for a whole wave: lanes walk their fragment lists in lockstep by
 // position (each lane still goes from its last fragment to its first), so that lanes of
 // a tile that see the same paint at the same depth add their constant-paint adjoints

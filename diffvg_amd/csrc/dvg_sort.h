@@ -1,0 +1,185 @@
+// dvg_sort.h -- the comparison sort the reference's BVH build uses, for host and device.
+//
+// The reference orders BVH leaves with std::sort (scene.cpp:602-611 segment y-centres,
+// scene.cpp:632-645 and 676 Morton codes), built against GNU libstdc++.  std::sort is not
+// stable, so leaves with equal keys end up in an order fixed by the library's algorithm,
+// and that order decides which of two equidistant segments a closest-point query
+// reports.  To build BVHs on the device with the same leaf order, this file implements
+// that algorithm (Musser's introsort as the SGI STL / libstdc++ formulate it):
+//   * ranges longer than 16 elements are partitioned around the median of (second,
+//     middle, last) moved to the front, the right part first, with a depth budget of
+//     2 floor(log2 n) partitions, after which a range is heap-sorted;
+//   * a final insertion pass sorts the first 16 elements and inserts the rest one by one.
+// Only comparison outcomes steer it, so sorting element indices with a comparator on
+// their keys yields the permutation std::sort yields for the elements themselves.
+// tests/csrc/sort_check.cpp compares it with std::sort on inputs full of ties.
+#pragma once
+
+#if defined(__HIPCC__)
+#define DVG_SORT_HD __host__ __device__ inline
+#else
+#define DVG_SORT_HD inline
+#endif
+
+namespace dvg {
+namespace stl {
+
+constexpr int kInsertionLimit = 16;
+
+DVG_SORT_HD int floor_log2(int n) {
+    int k = 0;
+    while (n > 1) {
+        n >>= 1;
+        k++;
+    }
+    return k;
+}
+
+template <class Less>
+DVG_SORT_HD void sift_up(int *a, int hole, int top, int v, Less &less) {
+    int parent = (hole - 1) / 2;
+    while (hole > top && less(a[parent], v)) {
+        a[hole] = a[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    a[hole] = v;
+}
+
+// move the larger child up until the hole reaches the bottom, then sift v up from there
+template <class Less>
+DVG_SORT_HD void adjust_heap(int *a, int hole, int len, int v, Less &less) {
+    const int top = hole;
+    int child = hole;
+    while (child < (len - 1) / 2) {
+        child = 2 * (child + 1);
+        if (less(a[child], a[child - 1])) child--;
+        a[hole] = a[child];
+        hole = child;
+    }
+    if ((len & 1) == 0 && child == (len - 2) / 2) {
+        child = 2 * (child + 1);
+        a[hole] = a[child - 1];
+        hole = child - 1;
+    }
+    sift_up(a, hole, top, v, less);
+}
+
+template <class Less>
+DVG_SORT_HD void heap_sort(int *a, int n, Less &less) {
+    if (n < 2) return;
+    for (int parent = (n - 2) / 2;; parent--) {
+        adjust_heap(a, parent, n, a[parent], less);
+        if (parent == 0) break;
+    }
+    for (int last = n - 1; last > 0; last--) {
+        int v = a[last];
+        a[last] = a[0];
+        adjust_heap(a, 0, last, v, less);
+    }
+}
+
+DVG_SORT_HD void swap_at(int *a, int i, int j) {
+    int t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+}
+
+// median of a[i], a[j], a[k] swapped into a[r]
+template <class Less>
+DVG_SORT_HD void median_to(int *a, int r, int i, int j, int k, Less &less) {
+    int m;
+    if (less(a[i], a[j])) {
+        if (less(a[j], a[k]))
+            m = j;
+        else if (less(a[i], a[k]))
+            m = k;
+        else
+            m = i;
+    } else if (less(a[i], a[k])) {
+        m = i;
+    } else if (less(a[j], a[k])) {
+        m = k;
+    } else {
+        m = j;
+    }
+    swap_at(a, r, m);
+}
+
+// Hoare partition of [lo, hi) around the pivot at a[p] (p < lo); returns the cut
+template <class Less>
+DVG_SORT_HD int partition(int *a, int lo, int hi, int p, Less &less) {
+    for (;;) {
+        while (less(a[lo], a[p])) lo++;
+        hi--;
+        while (less(a[p], a[hi])) hi--;
+        if (!(lo < hi)) return lo;
+        swap_at(a, lo, hi);
+        lo++;
+    }
+}
+
+template <class Less>
+DVG_SORT_HD void linear_insert(int *a, int i, Less &less) {
+    int v = a[i];
+    int j = i;
+    while (j > 0 && less(v, a[j - 1])) {
+        a[j] = a[j - 1];
+        j--;
+    }
+    a[j] = v;
+}
+
+template <class Less>
+DVG_SORT_HD void insertion_sort(int *a, int lo, int hi, Less &less) {
+    for (int i = lo + 1; i < hi; i++) {
+        if (less(a[i], a[lo])) {
+            int v = a[i];
+            for (int j = i; j > lo; j--) a[j] = a[j - 1];
+            a[lo] = v;
+        } else {
+            linear_insert(a, i, less);
+        }
+    }
+}
+
+// Sorts a[0..n) with `less(x, y)` on element values; `stack` holds sort_stack_ints(n)
+// ints: the pending right parts of the partition loop, at most one per depth level.
+template <class Less>
+DVG_SORT_HD void sort(int *a, int n, Less less, int *stack) {
+    if (n < 2) return;
+    int top = 0;
+    int lo = 0, hi = n, depth = 2 * floor_log2(n);
+    for (;;) {
+        while (hi - lo > kInsertionLimit) {
+            if (depth == 0) {
+                heap_sort(a + lo, hi - lo, less);
+                break;
+            }
+            depth--;
+            median_to(a, lo, lo + 1, lo + (hi - lo) / 2, hi - 1, less);
+            int cut = partition(a, lo + 1, hi, lo, less);
+            // [cut, hi) is sorted with the same remaining depth; continue with [lo, cut)
+            stack[top++] = cut;
+            stack[top++] = hi;
+            stack[top++] = depth;
+            hi = cut;
+        }
+        if (top == 0) break;
+        depth = stack[--top];
+        hi = stack[--top];
+        lo = stack[--top];
+    }
+    if (n > kInsertionLimit) {
+        insertion_sort(a, 0, kInsertionLimit, less);
+        for (int i = kInsertionLimit; i < n; i++) linear_insert(a, i, less);
+    } else {
+        insertion_sort(a, 0, n, less);
+    }
+}
+
+// ints the `stack` argument of sort() needs for n elements
+DVG_SORT_HD int sort_stack_ints(int n) { return 3 * (2 * floor_log2(n > 1 ? n : 2) + 2); }
+
+}  // namespace stl
+}  // namespace dvg

@@ -72,6 +72,14 @@ def _cat_kind(tensors):
     thousands of them)."""
     if not tensors:
         return None
+    try:
+        return _cat_same_device(tensors)
+    except RuntimeError:
+        # tensors of one kind on several devices: gather them on the host
+        return _cat_same_device([t.detach().cpu() for t in tensors])
+
+
+def _cat_same_device(tensors):
     ranks = {t.dim() for t in tensors}
     if ranks == {0}:
         out = torch.stack(tensors)
@@ -133,8 +141,9 @@ def flatten_args(args):
     has_thick = np.zeros(ns, np.uint8)
     closed = np.zeros(ns, np.uint8)
     approx = np.zeros(ns, np.uint8)
-    # float sources in description order, per kind
+    # float sources in description order, per kind, and their argument positions
     src_params, src_points, src_thick, src_sw, src_color, src_xform = [], [], [], [], [], []
+    pos = {k: [] for k in ('params', 'points', 'thick', 'sw', 'color', 'xform', 'ncp', 'ids')}
     thick_fill = []  # (offset, count) of zero thickness ranges
     ncp_list = []
     grad_map = []  # (arg index, kind, offset, count, shape)
@@ -147,6 +156,7 @@ def flatten_args(args):
             shape_type[s] = 0
             param_off[s] = n_params
             src_params += [radius, center]
+            pos['params'] += [i, i + 1]
             grad_map.append((i, 'shape_params', n_params, 1, tuple(radius.shape)))
             grad_map.append((i + 1, 'shape_params', n_params + 1, 2, tuple(center.shape)))
             n_params += 3
@@ -156,6 +166,7 @@ def flatten_args(args):
             shape_type[s] = 1
             param_off[s] = n_params
             src_params += [radius, center]
+            pos['params'] += [i, i + 1]
             grad_map.append((i, 'shape_params', n_params, 2, tuple(radius.shape)))
             grad_map.append((i + 1, 'shape_params', n_params + 2, 2, tuple(center.shape)))
             n_params += 4
@@ -173,10 +184,13 @@ def flatten_args(args):
             approx[s] = 1 if use_approx else 0
             ncp_list.append(ncp)
             src_points.append(points)
+            pos['ncp'].append(i)
+            pos['points'].append(i + 1)
             grad_map.append((i + 1, 'points', 2 * n_points, 2 * n, tuple(points.shape)))
             if thickness is not None:
                 has_thick[s] = 1
                 src_thick.append((n_points, thickness))
+                pos['thick'].append(i + 2)
                 grad_map.append((i + 2, 'thickness', n_points, n, tuple(thickness.shape)))
             n_points += n
             n_ctrl += nb
@@ -186,6 +200,7 @@ def flatten_args(args):
             shape_type[s] = 3
             param_off[s] = n_params
             src_params += [p_min, p_max]
+            pos['params'] += [i, i + 1]
             grad_map.append((i, 'shape_params', n_params, 2, tuple(p_min.shape)))
             grad_map.append((i + 1, 'shape_params', n_params + 2, 2, tuple(p_max.shape)))
             n_params += 4
@@ -194,6 +209,7 @@ def flatten_args(args):
             raise ValueError('unknown shape type %r' % (t,))
         sw = args[i]
         src_sw.append(sw)
+        pos['sw'].append(i)
         if not (shape_type[s] == 2 and has_thick[s]):
             grad_map.append((i, 'stroke_width', s, 1, tuple(sw.shape)))
         i += 1
@@ -210,6 +226,7 @@ def flatten_args(args):
     n_ids = 0
     for g in range(ng):
         ids = args[i]
+        pos['ids'].append(i)
         i += 1
         ids_list.append(ids)
         g_off[g] = n_ids
@@ -225,6 +242,7 @@ def flatten_args(args):
                 c = args[i]
                 paint[kind + '_type'][g] = 0
                 src_color.append(c)
+                pos['color'].append(i)
                 grad_map.append((i, 'color_params', n_color, 4, tuple(c.shape)))
                 n_color += 4
                 i += 1
@@ -236,6 +254,7 @@ def flatten_args(args):
                 paint[kind + '_type'][g] = 1 if ct == diffvg.ColorType.linear_gradient else 2
                 paint[kind + '_num_stops'][g] = nstops
                 src_color += [a, b, offsets, colors]
+                pos['color'] += [i, i + 1, i + 2, i + 3]
                 grad_map.append((i, 'color_params', n_color, 2, tuple(a.shape)))
                 grad_map.append((i + 1, 'color_params', n_color + 2, 2, tuple(b.shape)))
                 grad_map.append((i + 2, 'color_params', n_color + 4, nstops, tuple(offsets.shape)))
@@ -245,6 +264,7 @@ def flatten_args(args):
         even_odd[g] = 1 if args[i] else 0
         xf = args[i + 1]
         src_xform.append(xf)
+        pos['xform'].append(i + 1)
         grad_map.append((i + 1, 'shape_to_canvas', 9 * g, 9, tuple(xf.shape)))
         i += 2
     filter_type = args[i]
@@ -287,7 +307,163 @@ def flatten_args(args):
         use_even_odd_rule=even_odd, shape_to_canvas=np.ascontiguousarray(xform, np.float32),
         filter_type=_filter_int(filter_type), filter_radius=float(frad[0]))
     f.grad_map = grad_map
+    f.pos = pos
+    f.thick_ranges = [(off, int(t.numel())) for off, t in src_thick]
+    f.sw_first = sw_first
+    f.sw_zero = (shape_type == 2) & (has_thick != 0)
+    f.int_bytes = ncp.tobytes() + b'|' + gids.tobytes()
     return f
+
+
+class _SceneTemplate:
+    """Topology of one serialize_scene argument list: everything but the float values
+    (shape kinds, segment counts, point counts, group members, paint kinds, flags,
+    canvas and filter type), the argument positions of the float values per kind, and a
+    pool of device scenes of that topology.  A later argument list with the same
+    topology gets a pooled scene rebuilt on the device from its values
+    (dvg_scene_update) instead of a host build and upload."""
+
+    MAX_IDLE = 64
+
+    def __init__(self, args, flat):
+        self.nargs = len(args)
+        self.flat = flat
+        self.float_pos = sorted({i for (i, *_rest) in flat.grad_map} | set(flat.pos['sw']))
+        tensor_pos = set(self.float_pos) | set(flat.pos['ncp']) | set(flat.pos['ids'])
+        self.scalar_pos = [i for i in range(self.nargs) if i != 6 and i not in tensor_pos]
+        self.scalar_vals = tuple(args[i] for i in self.scalar_pos)
+        self.float_shapes = tuple(tuple(args[i].shape) for i in self.float_pos)
+        self.int_pos = flat.pos['ncp'] + flat.pos['ids']
+        self.int_bytes = flat.int_bytes
+        self.idle = []
+        self._dev_tables = {}
+
+    def matches(self, args):
+        if len(args) != self.nargs:
+            return False
+        try:
+            if tuple([args[i] for i in self.scalar_pos]) != self.scalar_vals:
+                return False
+            if tuple([tuple(args[i].shape) for i in self.float_pos]) != self.float_shapes:
+                return False
+        except (RuntimeError, TypeError, AttributeError):
+            return False
+        ncp = _cat_int([args[i] for i in self.flat.pos['ncp']])
+        gids = _cat_int([args[i] for i in self.flat.pos['ids']])
+        return ncp.tobytes() + b'|' + gids.tobytes() == self.int_bytes
+
+    def _tables(self, dev):
+        t = self._dev_tables.get(dev)
+        if t is None:
+            f = self.flat
+            npts = f.desc['num_points_total']
+            thick_idx = None
+            if f.thick_ranges:
+                thick_idx = torch.from_numpy(np.concatenate([np.arange(o, o + n) for o, n in f.thick_ranges])).to(dev)
+            t = dict(sw_first=torch.from_numpy(f.sw_first).to(dev),
+                     sw_zero=torch.from_numpy(f.sw_zero.astype(np.bool_)).to(dev),
+                     thick_idx=thick_idx, npts=npts)
+            self._dev_tables[dev] = t
+        return t
+
+    def device_params(self, args, dev):
+        """Float values of `args` as device arrays in the description's layout."""
+        f, T = self.flat, self._tables(dev)
+        with torch.no_grad():
+            def cat(kind):
+                srcs = [args[i] for i in f.pos[kind]]
+                if not srcs:
+                    return None
+                return _cat_kind(srcs).to(dev).contiguous()
+            params = cat('params')
+            points = cat('points')
+            thickness = None
+            if T['thick_idx'] is not None:
+                thickness = torch.zeros(T['npts'], dtype=torch.float32, device=dev)
+                thickness[T['thick_idx']] = cat('thick')
+            sw = cat('sw')
+            if sw is not None:
+                sw = torch.where(T['sw_zero'], torch.zeros((), device=dev), sw[T['sw_first']]).contiguous()
+            return dict(shape_params=params, points=points, thickness=thickness, stroke_width=sw,
+                        color_params=cat('color'), shape_to_canvas=cat('xform'))
+
+    def acquire(self, dev_index):
+        if self.idle:
+            return self.idle.pop()
+        return _lib.DeviceScene(self.flat.desc, dev_index, _lib.MODE_TILED)
+
+    def release(self, scene):
+        if len(self.idle) < self.MAX_IDLE:
+            self.idle.append(scene)
+
+
+class _Lease:
+    """A pooled device scene held by one forward pass (and its backward); it goes back to
+    its template's pool when the autograd context lets go of it."""
+
+    def __init__(self, template, scene):
+        self._template, self._scene = template, scene
+
+    def __getattr__(self, name):
+        return getattr(self._scene, name)
+
+    def __del__(self):
+        try:
+            self._template.release(self._scene)
+        except Exception:
+            pass
+
+
+_TEMPLATES = []
+_MAX_TEMPLATES = 8
+
+
+def _device_scene_for(args, dev, check=True):
+    """(flat, scene) for a TILED render of `args`: a pooled scene of the same topology
+    rebuilt on the device when one exists, else a host build that starts a new pool.
+    With check=False the caller reports the rebuild's findings (scene.status())."""
+    dev_index = dev.index if dev.index is not None else -1
+    for k, tpl in enumerate(_TEMPLATES):
+        if tpl.matches(args):
+            if k:
+                _TEMPLATES.insert(0, _TEMPLATES.pop(k))
+            vals = tpl.device_params(args, dev)
+            scene = tpl.acquire(dev_index)
+            scene.update(float(args[-1]), _stream(), **vals)
+            if check:
+                # what Scene construction rejects is reported now, as the reference reports
+                # it (one wait for the rebuild kernels, before the render is enqueued)
+                try:
+                    check_rebuild(scene)
+                except Exception:
+                    tpl.release(scene)
+                    raise
+            f = _Flat()
+            f.__dict__.update(tpl.flat.__dict__)
+            f.eval_positions = args[6]
+            return f, _Lease(tpl, scene)
+    flat = flatten_args(args)
+    tpl = _SceneTemplate(args, flat)
+    _TEMPLATES.insert(0, tpl)
+    del _TEMPLATES[_MAX_TEMPLATES:]
+    scene = _lib.DeviceScene(flat.desc, dev_index, _lib.MODE_TILED)
+    return flat, _Lease(tpl, scene)
+
+
+def check_rebuild(scene):
+    """Raises what Scene construction raises for the values of the last rebuild (an
+    AssertionError for non-finite points, as render_pytorch.py:63 asserts)."""
+    try:
+        scene.status()
+    except RuntimeError as e:
+        if 'non-finite path points' in str(e):
+            raise AssertionError('non-finite path points or thickness') from None
+        raise
+
+
+def clear_scene_cache():
+    """Drops the pooled device scenes (they are rebuilt on demand)."""
+    _TEMPLATES.clear()
 
 
 def _check_gpu():
@@ -391,10 +567,14 @@ class RenderFunction(torch.autograd.Function):
     @staticmethod
     def _prepare(width, height, background_image, args, output_type=None):
         _check_gpu()
-        flat = flatten_args(args)
         dev = get_device()
         start = time.time()
-        scene = _lib.DeviceScene(flat.desc, dev.index if dev.index is not None else -1, _render_mode)
+        if _render_mode == _lib.MODE_TILED:
+            flat, scene = _device_scene_for(args, dev)
+        else:
+            # the comparator: a host-built scene per call, as the reference builds it
+            flat = flatten_args(args)
+            scene = _lib.DeviceScene(flat.desc, dev.index if dev.index is not None else -1, _render_mode)
         if print_timing:
             print('Scene construction, time: %.5f s' % (time.time() - start))
         return flat, scene

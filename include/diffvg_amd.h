@@ -148,6 +148,42 @@ int dvg_scene_set_mode(dvg_scene *scene, int mode);
  * summation order.  No reference counterpart (the reference has no tiles). */
 int dvg_scene_set_tiling(dvg_scene *scene, int tile_width);
 
+/* Device-resident parameter values of a scene: DEVICE pointers on the scene's GPU, in the
+ * layout of the float arrays of dvg_scene_desc (points as x,y pairs; thickness 0 where a
+ * path has none; stroke width 0 for paths with thickness).  NULL keeps an array's
+ * current values. */
+typedef struct dvg_scene_params {
+    const float *shape_params;    /* [num_shape_params] */
+    const float *points;          /* [2 * num_points_total] */
+    const float *thickness;       /* [num_points_total] */
+    const float *stroke_width;    /* [num_shapes] */
+    const float *color_params;    /* [num_color_params] */
+    const float *shape_to_canvas; /* [9 * num_groups] */
+} dvg_scene_params;
+
+/* Rebuilds the scene on the device for new parameter values with the same topology
+ * (shape kinds, segment counts, group members, paint kinds): the equivalent of
+ * dvg_scene_create from a description that differs only in its float arrays, as the
+ * reference constructs a new Scene for every forward pass (render_pytorch.py:366-368,
+ * scene.cpp:919-998), without a host round trip.  Enqueued on `stream`: the values are
+ * copied into the scene, then kernels recompute boxes, boundary lengths, sampling CDFs,
+ * canvas_to_shape, and the segment and group BVH leaf orders with the reference's sort
+ * (so results equal those of a scene created from the same values).  The gradient
+ * buffers are not touched.  Conditions Scene construction rejects (non-finite points or
+ * lengths, a total boundary length <= 0) are reported by dvg_scene_status.  The PLAIN
+ * schedule's top-level BVH is not rebuilt: PLAIN renders of an updated scene fail.
+ * Batches created earlier keep the filter radius they were created with. */
+int dvg_scene_update(dvg_scene *scene, const dvg_scene_params *params, float filter_radius, void *stream);
+/* Waits for the scene's last stream and reports the conditions its last update and
+ * render found (the reference asserts or throws on them): 0 when there are none. */
+int dvg_scene_status(dvg_scene *scene);
+/* Copies the scene's device block (its derived tables; layout internal to this library
+ * version) to host memory `dst` of `cap` bytes; `bytes` receives the block size.  The
+ * PLAIN schedule's top-level BVH and the rebuild's sort scratch read as zeros, so the
+ * blocks of a rebuilt scene and of a scene created from the same values are equal.  For
+ * tests comparing a device rebuild with a host build; no reference counterpart. */
+int dvg_scene_debug_block(dvg_scene *scene, void *dst, int64_t cap, int64_t *bytes);
+
 /* Batched render: many (scene, render args) jobs -- scenes of any size, resolution,
  * sample count, output (colour or SDF) and prefiltering -- rendered with one kernel
  * per stage over the work items of all jobs.  It replaces the reference's loop of one

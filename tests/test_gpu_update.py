@@ -1,0 +1,200 @@
+"""Scene construction on the device (dvg_scene_update) against construction on the host
+(dvg_scene_create, which follows scene.cpp with the reference's own std::sort).
+
+A scene created from values V1 and rebuilt on the device from values V2 must hold the
+same tables, byte for byte, as a scene created from V2: boxes, boundary lengths,
+sampling CDFs, canvas_to_shape, SDF anchors and the BVH leaf visit orders, including
+scenes whose sort keys tie (segments with equal box y-centres, group members in one
+Morton cell, more than 16 of each, so that the sort's partitioning decides the order).
+Renders of the two scenes then agree bitwise in the forward pass.  The autograd front
+end reuses pooled scenes of an unchanged topology through this rebuild; its results
+must equal those of a fresh scene.
+"""
+import numpy as np
+import pytest
+import torch
+
+import diffvg_amd as pd
+from diffvg_amd import _lib
+from diffvg_amd import render_pytorch as rp
+import scenes
+from refdrive import flatten
+
+pytestmark = pytest.mark.gpu
+
+FLOAT_KEYS = ('shape_params', 'points', 'thickness', 'stroke_width', 'color_params', 'shape_to_canvas')
+
+
+def ties(size=64):
+    """A polyline of 20 horizontal segments at one height (equal y-centres) and a group of
+    20 tiny circles in one Morton cell, next to ordinary shapes."""
+    xs = np.linspace(4, 60, 21)
+    pts = np.stack([xs, np.full_like(xs, 30.0)], 1)
+    line = pd.Path(num_control_points=torch.zeros(20, dtype=torch.int32), points=scenes.T(pts), is_closed=False,
+                   stroke_width=scenes.T(1.5))
+    dots = [pd.Circle(radius=scenes.T(1.0 + 0.01 * k), center=scenes.T([20.0 + 0.001 * k, 20.0])) for k in range(20)]
+    zig = np.array([[8 + 2.5 * k, 50 + (3 if k % 2 else -3)] for k in range(21)], np.float32)
+    zz = pd.Path(num_control_points=torch.zeros(20, dtype=torch.int32), points=scenes.T(zig), is_closed=False,
+                 stroke_width=scenes.T(1.0))
+    shapes = [line] + dots + [zz]
+    groups = [pd.ShapeGroup(shape_ids=torch.tensor([0]), fill_color=None, stroke_color=scenes.T([1, 0, 0, 1])),
+              pd.ShapeGroup(shape_ids=torch.arange(1, 21), fill_color=scenes.T([0, 0.5, 1, 0.8]),
+                            stroke_color=scenes.T([0, 0, 0, 1])),
+              pd.ShapeGroup(shape_ids=torch.tensor([21]), fill_color=None, stroke_color=scenes.T([0, 1, 0, 1]))]
+    return dict(canvas=(size, size), shapes=shapes, groups=groups, filter=('box', 0.5))
+
+
+CASES = sorted(scenes.CASES) + ['ties']
+
+
+def _flat(case):
+    return scenes.flat_case(ties() if case == 'ties' else case)
+
+
+def _perturb(f, seed, rigid=False):
+    """The same topology with other float values (`rigid`: points moved by one common
+    offset, which keeps ties between sort keys)."""
+    rng = np.random.default_rng(seed)
+    g = dict(f)
+    if rigid:
+        g['points'] = (f['points'].reshape(-1, 2) + np.array([0.5, 0.25], np.float32)).reshape(f['points'].shape)
+    else:
+        g['points'] = (f['points'] + rng.normal(0, 0.7, f['points'].shape)).astype(np.float32)
+    g['stroke_width'] = (f['stroke_width'] * rng.uniform(0.7, 1.3, f['stroke_width'].shape)).astype(np.float32)
+    if f['thickness'] is not None and f['thickness'].size:
+        g['thickness'] = (f['thickness'] * rng.uniform(0.7, 1.3, f['thickness'].shape)).astype(np.float32)
+    g['shape_params'] = (f['shape_params'] * rng.uniform(0.9, 1.1, f['shape_params'].shape)).astype(np.float32)
+    g['color_params'] = np.clip(f['color_params'] + rng.normal(0, 0.05, f['color_params'].shape), 0, 1).astype(
+        np.float32)
+    x = f['shape_to_canvas'].reshape(-1, 3, 3).copy()
+    x[:, :2, 2] += rng.normal(0, 0.5, (x.shape[0], 2))
+    g['shape_to_canvas'] = x.reshape(-1).astype(np.float32)
+    g['filter_radius'] = float(f['filter_radius'])
+    return g
+
+
+def _dev(a):
+    if a is None or a.size == 0:
+        return None
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def _render(scene, w, h, d_img=None):
+    img = None if d_img is not None else torch.zeros(h, w, 4, device='cuda')
+    di = None if d_img is None else torch.from_numpy(d_img).cuda()
+    scene.zero_grads(None)
+    scene.render(w, h, 2, 2, 3, render_image=img, d_render_image=di)
+    torch.cuda.synchronize()
+    return img.cpu().numpy() if img is not None else scene.grads()
+
+
+@pytest.mark.parametrize('case', CASES)
+def test_device_rebuild_equals_host_build(case):
+    print('rebuild', case, flush=True)
+    f1 = _flat(case)
+    f2 = _perturb(f1, 1, rigid=case == 'ties')
+    a = _lib.DeviceScene(f1, 0)
+    tensors = {k: _dev(f2[k]) for k in FLOAT_KEYS}
+    a.update(f2['filter_radius'], None, **tensors)
+    a.status()
+    b = _lib.DeviceScene(f2, 0)
+    ba, bb = a.debug_block(), b.debug_block()
+    assert ba.shape == bb.shape
+    diff = np.flatnonzero(ba != bb)
+    assert diff.size == 0, ('first differing byte', int(diff[0]))
+    w, h = f1['canvas_width'], f1['canvas_height']
+    assert np.array_equal(_render(a, w, h), _render(b, w, h))
+    d = np.random.default_rng(2).uniform(-1, 1, (h, w, 4)).astype(np.float32)
+    ga, gb = _render(a, w, h, d), _render(b, w, h, d)
+    for k in gb:
+        scale = max(float(np.abs(gb[k]).max()), 1e-12)
+        assert np.abs(ga[k] - gb[k]).max() <= 1e-5 * scale, k
+
+
+def test_rebuild_twice_and_keep_arrays():
+    """A second rebuild from other values, and a rebuild that passes only the points."""
+    f1 = _flat('overlap_eo')
+    f2, f3 = _perturb(f1, 2), _perturb(f1, 3)
+    a = _lib.DeviceScene(f1, 0)
+    a.update(f2['filter_radius'], None, **{k: _dev(f2[k]) for k in FLOAT_KEYS})
+    a.update(f3['filter_radius'], None, **{k: _dev(f3[k]) for k in FLOAT_KEYS})
+    a.status()
+    assert np.array_equal(a.debug_block(), _lib.DeviceScene(f3, 0).debug_block())
+    f4 = dict(f3)
+    f4['points'] = _perturb(f1, 4)['points']
+    a.update(f4['filter_radius'], None, points=_dev(f4['points']))
+    a.status()
+    assert np.array_equal(a.debug_block(), _lib.DeviceScene(f4, 0).debug_block())
+
+
+def test_rebuild_reports_what_construction_rejects():
+    f1 = _flat('curve_fill')
+    a = _lib.DeviceScene(f1, 0)
+    bad = f1['points'].copy()
+    bad[3] = np.nan
+    a.update(f1['filter_radius'], None, points=_dev(bad))
+    with pytest.raises(RuntimeError, match='non-finite path points'):
+        a.status()
+    flat_pts = np.tile(f1['points'][:2], len(f1['points']) // 2).astype(np.float32)
+    a.update(f1['filter_radius'], None, points=_dev(flat_pts))
+    with pytest.raises(RuntimeError, match='total length'):
+        a.status()
+    a.update(f1['filter_radius'], None, points=_dev(f1['points']))
+    a.status()
+    a.set_mode(_lib.MODE_PLAIN)
+    with pytest.raises(RuntimeError, match='PLAIN'):
+        a.render(64, 64, 2, 2, 0, render_image=torch.zeros(64, 64, 4, device='cuda'))
+
+
+def _leaf_scene():
+    sc = scenes.overlap(size=64, n=10)
+    for s in sc['shapes']:
+        s.points.requires_grad_(True)
+        s.stroke_width.requires_grad_(True)
+    for g in sc['groups']:
+        g.fill_color.requires_grad_(True)
+    return sc
+
+
+def _step(sc, target):
+    args = pd.RenderFunction.serialize_scene(64, 64, sc['shapes'], sc['groups'])
+    img = pd.RenderFunction.apply(64, 64, 2, 2, 5, None, *args)
+    loss = ((img - target) ** 2).sum()
+    for s in sc['shapes']:
+        s.points.grad = None
+    loss.backward()
+    return img.detach().cpu().numpy(), [s.points.grad.cpu().numpy().copy() for s in sc['shapes']]
+
+
+def test_render_function_pools_scenes_and_matches_fresh():
+    pd.set_use_gpu(True)
+    pd.set_device(torch.device('cuda:0'))
+    rp.clear_scene_cache()
+    sc = _leaf_scene()
+    target = torch.rand(64, 64, 4, device='cuda', generator=torch.Generator('cuda').manual_seed(0))
+    _step(sc, target)  # host build; starts a pool for this topology
+    assert len(rp._TEMPLATES) == 1
+    with torch.no_grad():
+        for s in sc['shapes']:
+            s.points += 0.8
+    img_pooled, g_pooled = _step(sc, target)  # device rebuild of the pooled scene
+    assert len(rp._TEMPLATES) == 1 and len(rp._TEMPLATES[0].idle) >= 1
+    rp.clear_scene_cache()
+    img_fresh, g_fresh = _step(sc, target)  # host build from the same values
+    assert np.array_equal(img_pooled, img_fresh)
+    for a, b in zip(g_pooled, g_fresh):
+        assert np.abs(a - b).max() <= 1e-5 * max(float(np.abs(b).max()), 1e-12)
+
+
+def test_render_function_rebuild_rejects_non_finite_points():
+    pd.set_use_gpu(True)
+    pd.set_device(torch.device('cuda:0'))
+    rp.clear_scene_cache()
+    sc = _leaf_scene()
+    target = torch.zeros(64, 64, 4, device='cuda')
+    _step(sc, target)
+    with torch.no_grad():
+        sc['shapes'][2].points[1, 0] = float('nan')
+    with pytest.raises(AssertionError):
+        _step(sc, target)
+    rp.clear_scene_cache()
