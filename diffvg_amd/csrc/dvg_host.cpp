@@ -812,6 +812,7 @@ int dvg_scene_get_grads(dvg_scene *scene, dvg_grads *o) {
     SceneHost &h = scene->h;
     if (h.last_stream) HIP_OK(hipStreamSynchronize(h.last_stream));
     HIP_OK(hipDeviceSynchronize());
+    if (int e = scene_status(h)) return e;
     auto cp = [&](float *dst, size_t off, size_t n) -> int {
         if (dst && n) HIP_OK(hipMemcpy(dst, h.grad + off, n * sizeof(float), hipMemcpyDeviceToHost));
         return 0;

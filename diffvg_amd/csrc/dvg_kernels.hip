@@ -1275,7 +1275,7 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
         }
     }
     DVG_CHECK(hipGetLastError());
-    if (a.d_render_image || pre) return check_err_flag(sc, st);
+    // nothing waits here: conditions the kernels flag are reported by scene_status
     return 0;
 }
 

@@ -407,3 +407,5 @@ def render(scene, background_image, render_image, render_sdf, width, height, num
                         d_render_image=P(d_render_image), d_render_sdf=P(d_render_sdf),
                         d_translation=P(d_translation), use_prefiltering=use_prefiltering,
                         eval_positions=P(eval_positions), num_eval_positions=num_eval_positions)
+    # the reference's render returns when the device is done (diffvg.cpp:1639-1641)
+    scene._scene.status()

@@ -332,7 +332,23 @@ class _SceneTemplate:
         tensor_pos = set(self.float_pos) | set(flat.pos['ncp']) | set(flat.pos['ids'])
         self.scalar_pos = [i for i in range(self.nargs) if i != 6 and i not in tensor_pos]
         self.scalar_vals = tuple(args[i] for i in self.scalar_pos)
-        self.float_shapes = tuple(tuple(args[i].shape) for i in self.float_pos)
+        self.float_shapes = [args[i].shape for i in self.float_pos]
+        # how each kind's tensors concatenate (their ranks are part of the topology)
+        self.cat_mode = {}
+        for kind in ('params', 'points', 'thick', 'sw', 'color', 'xform'):
+            ts = [args[i] for i in flat.pos[kind]]
+            ranks = {t.dim() for t in ts}
+            if not ts:
+                mode = None
+            elif ranks == {0}:
+                mode = 'stack'
+            elif ranks == {1}:
+                mode = 'cat'
+            elif len(ranks) == 1 and len({tuple(t.shape[1:]) for t in ts}) == 1:
+                mode = 'cat_flat'
+            else:
+                mode = 'flatten_each'
+            self.cat_mode[kind] = mode
         self.int_pos = flat.pos['ncp'] + flat.pos['ids']
         self.int_bytes = flat.int_bytes
         self.idle = []
@@ -344,7 +360,7 @@ class _SceneTemplate:
         try:
             if tuple([args[i] for i in self.scalar_pos]) != self.scalar_vals:
                 return False
-            if tuple([tuple(args[i].shape) for i in self.float_pos]) != self.float_shapes:
+            if [args[i].shape for i in self.float_pos] != self.float_shapes:
                 return False
         except (RuntimeError, TypeError, AttributeError):
             return False
@@ -371,10 +387,22 @@ class _SceneTemplate:
         f, T = self.flat, self._tables(dev)
         with torch.no_grad():
             def cat(kind):
-                srcs = [args[i] for i in f.pos[kind]]
-                if not srcs:
+                mode = self.cat_mode[kind]
+                if mode is None:
                     return None
-                return _cat_kind(srcs).to(dev).contiguous()
+                srcs = [args[i] for i in f.pos[kind]]
+                try:
+                    if mode == 'stack':
+                        out = torch.stack(srcs)
+                    elif mode == 'cat':
+                        out = torch.cat(srcs)
+                    elif mode == 'cat_flat':
+                        out = torch.cat(srcs).reshape(-1)
+                    else:
+                        out = torch.cat([t.reshape(-1) for t in srcs])
+                except RuntimeError:
+                    out = _cat_kind(srcs)
+                return out.to(device=dev, dtype=torch.float32).contiguous()
             params = cat('params')
             points = cat('points')
             thickness = None
@@ -521,7 +549,8 @@ class RenderFunction(torch.autograd.Function):
                 args += [diffvg.ShapeType.ellipse, shape.radius, shape.center]
             elif isinstance(shape, Path):
                 assert shape.points.shape[1] == 2
-                args += [diffvg.ShapeType.path, shape.num_control_points.to(torch.int32), shape.points]
+                ncp = shape.num_control_points
+                args += [diffvg.ShapeType.path, ncp if ncp.dtype == torch.int32 else ncp.to(torch.int32), shape.points]
                 if len(shape.stroke_width.shape) > 0 and shape.stroke_width.shape[0] > 1:
                     use_thickness = True
                     args.append(shape.stroke_width)
@@ -539,7 +568,8 @@ class RenderFunction(torch.autograd.Function):
                 raise TypeError('unsupported shape %r' % (type(shape),))
             args.append(torch.tensor(0.0) if use_thickness else shape.stroke_width)
         for shape_group in shape_groups:
-            args.append(shape_group.shape_ids.to(torch.int32))
+            ids = shape_group.shape_ids
+            args.append(ids if ids.dtype == torch.int32 else ids.to(torch.int32))
             for kind, color in (('fill', shape_group.fill_color), ('stroke', shape_group.stroke_color)):
                 if color is None:
                     args.append(None)
@@ -610,6 +640,9 @@ class RenderFunction(torch.autograd.Function):
                      render_sdf=rendered_image if flat.output_type == OutputType.sdf else None,
                      use_prefiltering=flat.use_prefiltering, eval_positions=ev,
                      num_eval_positions=0 if ev is None else ev.shape[0], stream=_stream())
+        if flat.use_prefiltering:
+            # a prefiltered stroke of a circle group: the reference asserts in the forward
+            scene.status()
         if print_timing:
             torch.cuda.synchronize()
             print('Forward pass, time: %.5f s' % (time.time() - start))
@@ -656,6 +689,7 @@ class RenderFunction(torch.autograd.Function):
                      d_render_sdf=grad_img if flat.output_type == OutputType.sdf else None,
                      d_translation=translation_grad_image, use_prefiltering=flat.use_prefiltering,
                      eval_positions=ev, num_eval_positions=0 if ev is None else ev.shape[0], stream=_stream())
+        scene.status()
         if print_timing:
             torch.cuda.synchronize()
             print('Gradient pass, time: %.5f s' % (time.time() - start))
@@ -666,7 +700,10 @@ class RenderFunction(torch.autograd.Function):
     def backward(ctx, grad_img):
         if not grad_img.is_contiguous():
             grad_img = grad_img.contiguous()
-        assert torch.isfinite(grad_img).all()
+        # the reference asserts a finite grad_img before rendering; the check is taken on
+        # the device and read after the backward is enqueued, so the host prepares the
+        # gradient views while the device works
+        finite = torch.isfinite(grad_img).all()
         scene = ctx.scene
         flat = ctx.flat
         background_image = ctx.background_image
@@ -682,9 +719,9 @@ class RenderFunction(torch.autograd.Function):
                      num_eval_positions=0 if ctx.eval_positions is None else ctx.eval_positions.shape[0],
                      stream=_stream())
         block, layout = scene.grads_device(grad_img.device, _stream())
-        if print_timing:
-            torch.cuda.synchronize()
-            print('Backward pass, time: %.5f s' % (time.time() - start))
-        assert bool(torch.isfinite(block).all())
         d_scene = _grad_tensors(flat, block, layout, ctx.args, flat.num_args)
+        scene.status()
+        assert bool(finite)
+        if print_timing:
+            print('Backward pass, time: %.5f s' % (time.time() - start))
         return tuple([None, None, None, None, None, d_background_image] + d_scene)

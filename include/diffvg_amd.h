@@ -125,12 +125,14 @@ int dvg_scene_destroy(dvg_scene *scene);
 
 /* render(...) (reference diffvg.cpp:1477-1649).  Forward when render_image/render_sdf
  * are set; backward when d_render_image/d_render_sdf are set, accumulating into the
- * scene's gradient buffers.  Asynchronous on args->stream in DVG_MODE_TILED;
- * synchronous in DVG_MODE_PLAIN, like the reference. */
+ * scene's gradient buffers.  Asynchronous on args->stream in DVG_MODE_TILED (conditions
+ * the reference asserts on -- fragment overflow, prefiltered circle strokes -- are then
+ * reported by dvg_scene_status and dvg_scene_get_grads); synchronous in DVG_MODE_PLAIN,
+ * like the reference. */
 int dvg_render(dvg_scene *scene, const dvg_render_args *args);
 
 /* Copies the accumulated gradients to host memory (synchronises the scene's last
- * stream first). */
+ * stream first and fails with the conditions dvg_scene_status reports). */
 int dvg_scene_get_grads(dvg_scene *scene, dvg_grads *out);
 /* Layout of the scene's gradient block on the device: float offsets of the seven
  * arrays of dvg_grads (in that order) and the block size in floats. */
