@@ -10,7 +10,8 @@ from .shape import *  # noqa: F401,F403
 from .color import *  # noqa: F401,F403
 from .pixel_filter import *  # noqa: F401,F403
 from . import diffvg  # noqa: F401
-from .render_pytorch import RenderFunction, OutputType, set_print_timing, set_render_mode, get_render_mode  # noqa: F401
+from .render_pytorch import (RenderFunction, OutputType, set_print_timing, set_render_mode,  # noqa: F401
+                             get_render_mode, synchronize)
 from .batch import Batch  # noqa: F401
 from .parse_svg import svg_to_scene  # noqa: F401
 from .save_svg import save_svg  # noqa: F401

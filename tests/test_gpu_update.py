@@ -192,6 +192,7 @@ def test_render_function_pools_scenes_and_matches_fresh():
         for s in sc['shapes']:
             s.points += 0.8
     img_pooled, g_pooled = _step(sc, target)  # device rebuild of the pooled scene
+    pd.synchronize()  # the last backward's scene returns to the pool once its findings are read
     assert len(rp._TEMPLATES) == 1 and len(rp._TEMPLATES[0].idle) >= 1
     rp.clear_scene_cache()
     img_fresh, g_fresh = _step(sc, target)  # host build from the same values
@@ -212,3 +213,23 @@ def test_render_function_rebuild_rejects_non_finite_points():
     with pytest.raises(AssertionError):
         _step(sc, target)
     rp.clear_scene_cache()
+
+
+def test_backward_findings_are_reported_at_the_next_sync():
+    """The backward returns while its kernels run; what the reference's backward would
+    stop on (here a non-finite grad_img, render_pytorch.py:678) is raised at
+    synchronize() or at the next entry into the render path, and then cleared."""
+    pd.set_use_gpu(True)
+    pd.set_device(torch.device('cuda:0'))
+    rp.synchronize()
+    sc = _leaf_scene()
+    args = pd.RenderFunction.serialize_scene(64, 64, sc['shapes'], sc['groups'])
+    img = pd.RenderFunction.apply(64, 64, 2, 2, 5, None, *args)
+    img.backward(torch.full_like(img, float('nan')))
+    with pytest.raises(AssertionError):
+        pd.synchronize()
+    pd.synchronize()
+    target = torch.zeros(64, 64, 4, device='cuda')
+    img2, g2 = _step(sc, target)
+    pd.synchronize()
+    assert np.isfinite(img2).all() and all(np.isfinite(g).all() for g in g2)
