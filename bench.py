@@ -154,8 +154,9 @@ def bench_batched(a):
     each with an L2 loss against a per-scene target.  Three measurements:
       graph    -- scenes resident; forward batch, loss gradient and backward batch
                   captured in one HIP graph and replayed (value);
-      batched  -- per step: marshal and build every scene, one batched forward and one
-                  batched backward, gradients handed to every parameter tensor;
+      batched  -- per step, the user-level API: serialize every scene, render_batch
+                  (pooled scenes rebuilt on the device, one batched forward and one
+                  batched backward), L2 loss, backward into every parameter tensor;
       plain    -- per step: one RenderFunction call per scene on the PLAIN schedule
                   (the reference's one-scene-per-launch structure), loss, backward."""
     import numpy as np
@@ -248,26 +249,20 @@ def bench_batched(a):
     nk = L.dvg_profile_read(names, 4096, ms, cnts, 32)
     kern = {nm: round(ms[i], 4) for i, nm in enumerate(names.value.decode().strip().split('\n')[:nk])}
 
-    # ---- batched end to end (marshalling and scene builds every step)
+    # ---- batched end to end: the user-level autograd API over all scenes per step
+    # (serialize_scene per scene, pooled scenes rebuilt on the device, one batched
+    # forward and one batched backward, gradients delivered to every parameter tensor)
+    from diffvg_amd.batch_render import render_batch
+    scene_list = [(sizes[i][0], sizes[i][1], items[i]['shapes'], items[i]['groups']) for i in range(n)]
+    targets = [view(target_flat, i) for i in range(n)]
+
     def batched_step():
         for ps in params:
             for p in ps:
                 p.grad = None
-        bl = [build(i) for i in range(n)]
-        f1 = Batch([(ds, dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0,
-                              render_image=view(img_flat, i))) for i, (_, _, ds) in enumerate(bl)], zero_grads=False)
-        img_flat.zero_()
-        f1.render(None)
-        torch.mul(img_flat - target_flat, 2.0, out=d_flat)
-        b1 = Batch([(ds, dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0,
-                              d_render_image=view(d_flat, i))) for i, (_, _, ds) in enumerate(bl)], zero_grads=True)
-        b1.render(None)
-        for args, fl, ds in bl:
-            block, layout = ds.grads_device(dev)
-            for t, gt in zip(args, _grad_tensors(fl, block, layout, args, fl.num_args)):
-                if gt is not None and isinstance(t, torch.Tensor) and t.requires_grad:
-                    t.grad = gt
-        b1.status()
+        imgs = render_batch(scene_list, 2, 2, [0] * n)
+        loss = sum(((im - t) ** 2).sum() for im, t in zip(imgs, targets))
+        loss.backward()
 
     for _ in range(2):
         batched_step()

@@ -17,7 +17,8 @@ import torch
 from . import _lib
 from .batch import Batch
 from .device import get_device
-from .render_pytorch import (OutputType, _check_gpu, _device_scene_for, _grad_tensors, _stream, check_rebuild)
+from .render_pytorch import (OutputType, _PENDING, _check_gpu, _device_scene_for, _grad_tensors, _stream,
+                             check_rebuild, synchronize)
 
 __all__ = ['BatchRenderFunction', 'render_batch']
 
@@ -29,7 +30,7 @@ class BatchRenderFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, spec, *args):
-        _check_gpu()
+        _check_gpu()  # also reads the findings of earlier backward passes
         dev = get_device()
         st = _stream()
         scenes, flats, images, jobs, spans = [], [], [], [], []
@@ -62,6 +63,10 @@ class BatchRenderFunction(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, *grad_imgs):
+        synchronize()
+        # the forward batch holds the output images, whose graph holds this context:
+        # dropping it here lets the scenes go back to their pool without a cycle collection
+        ctx.fwd = None
         dev = get_device()
         st = _stream()
         jobs, keep = [], []
@@ -74,15 +79,27 @@ class BatchRenderFunction(torch.autograd.Function):
         bwd = Batch(jobs, zero_grads=True)
         bwd.render(st)
         out = [None] * len(ctx.args)
-        blocks = []
         for sc, flat, (k, n) in zip(ctx.scenes, ctx.flats, ctx.spans):
             block, layout = sc.grads_device(dev, st)
-            blocks.append(block)
             for j, t in enumerate(_grad_tensors(flat, block, layout, ctx.args[k:k + n], flat.num_args)):
-                out[k + j] = t
-        bwd.status()
-        assert all(bool(torch.isfinite(b).all()) for b in blocks)
+                if t is not None:
+                    out[k + j] = t
+        # as RenderFunction.backward: returns while the kernels run; the batch's findings
+        # (and a non-finite grad_img) are read at the next entry or at synchronize()
+        finite = torch.stack([g.isfinite().all() for g in keep]).all()
+        _PENDING.append((_BatchCheck(bwd, ctx.scenes), finite))
+        ctx.scenes = None
         return (None,) + tuple(out)
+
+
+class _BatchCheck:
+    """Keeps a backward batch and its scene leases until its findings are read."""
+
+    def __init__(self, batch, scenes):
+        self.batch, self.scenes = batch, scenes
+
+    def status(self):
+        self.batch.status()
 
 
 def render_batch(items, num_samples_x=2, num_samples_y=2, seeds=None):

@@ -1918,28 +1918,37 @@ __global__ __launch_bounds__(kBlock) void k_refit_groups(RefitArgs A) {
 }
 
 // build_shape_cdfs (scene.cpp:207-246): the running sum is sequential, as in the
-// reference, so the CDF has its rounding; then every entry is normalised.
+// reference, so the CDF has its rounding.  The workgroup gathers the lengths into LDS
+// in chunks; one lane adds them in order; then every entry is normalised in parallel.
 __global__ __launch_bounds__(kBlock) void k_refit_cdf(RefitArgs A) {
     const DevScene &S = A.S;
     const int total = S.num_total_shapes;
     float *cdf = writable(S.shapes_cdf), *pmf = writable(S.shapes_pmf);
-    __shared__ float s_norm;
-    if (threadIdx.x == 0) {
-        float acc = 0.f;
-        for (int k = 0; k < total; k++) {
-            float l = S.shape_length[S.sample_shape_id[k]];
-            acc = k == 0 ? l : l + acc;
-            cdf[k] = acc;
-            pmf[k] = l;
+    constexpr int kChunk = 8192;
+    __shared__ float s_len[kChunk];
+    __shared__ float s_acc;
+    for (int base = 0; base < total; base += kChunk) {
+        const int n = min(kChunk, total - base);
+        for (int k = threadIdx.x; k < n; k += blockDim.x) s_len[k] = S.shape_length[S.sample_shape_id[base + k]];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float acc = base == 0 ? 0.f : s_acc;
+            for (int k = 0; k < n; k++) {
+                float l = s_len[k];
+                acc = base + k == 0 ? l : l + acc;
+                s_len[k] = acc;
+            }
+            s_acc = acc;
         }
-        s_norm = acc;
-        if (!(acc > 0.f) || !isfinite(acc)) atomicOr(A.uerr, kUpdTotalLength);
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += blockDim.x) cdf[base + k] = s_len[k];
+        __syncthreads();
     }
-    __syncthreads();
-    const float norm = s_norm;
+    const float norm = total > 0 ? s_acc : 0.f;
+    if (threadIdx.x == 0 && (!(norm > 0.f) || !isfinite(norm))) atomicOr(A.uerr, kUpdTotalLength);
     for (int k = threadIdx.x; k < total; k += blockDim.x) {
-        cdf[k] /= norm;
-        pmf[k] /= norm;
+        cdf[k] = cdf[k] / norm;
+        pmf[k] = S.shape_length[S.sample_shape_id[k]] / norm;
     }
 }
 

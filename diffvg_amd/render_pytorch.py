@@ -442,8 +442,12 @@ class _Lease:
             pass
 
 
-_TEMPLATES = []
-_MAX_TEMPLATES = 8
+_TEMPLATES = {}  # number of arguments -> templates, most recently used first
+_MAX_TEMPLATES = 1024
+
+
+def _templates():
+    return [t for ts in _TEMPLATES.values() for t in ts]
 
 
 def _device_scene_for(args, dev, check=True):
@@ -451,10 +455,11 @@ def _device_scene_for(args, dev, check=True):
     rebuilt on the device when one exists, else a host build that starts a new pool.
     With check=False the caller reports the rebuild's findings (scene.status())."""
     dev_index = dev.index if dev.index is not None else -1
-    for k, tpl in enumerate(_TEMPLATES):
+    bucket = _TEMPLATES.setdefault(len(args), [])
+    for k, tpl in enumerate(bucket):
         if tpl.matches(args):
             if k:
-                _TEMPLATES.insert(0, _TEMPLATES.pop(k))
+                bucket.insert(0, bucket.pop(k))
             vals = tpl.device_params(args, dev)
             scene = tpl.acquire(dev_index)
             scene.update(float(args[-1]), _stream(), **vals)
@@ -472,8 +477,10 @@ def _device_scene_for(args, dev, check=True):
             return f, _Lease(tpl, scene)
     flat = flatten_args(args)
     tpl = _SceneTemplate(args, flat)
-    _TEMPLATES.insert(0, tpl)
-    del _TEMPLATES[_MAX_TEMPLATES:]
+    if sum(len(ts) for ts in _TEMPLATES.values()) >= _MAX_TEMPLATES:
+        _TEMPLATES.clear()
+        bucket = _TEMPLATES.setdefault(len(args), [])
+    bucket.insert(0, tpl)
     scene = _lib.DeviceScene(flat.desc, dev_index, _lib.MODE_TILED)
     return flat, _Lease(tpl, scene)
 
@@ -505,9 +512,9 @@ def synchronize():
     """Waits for the backward passes still in flight and raises what they found (the
     reference's backward asserts a finite grad_img and aborts on fragment overflow)."""
     while _PENDING:
-        scene, finite = _PENDING.pop(0)
+        work, finite = _PENDING.pop(0)
         try:
-            scene.status()
+            work.status()
         except RuntimeError as e:
             raise RuntimeError('%s (found by an earlier backward pass)' % e) from None
         assert bool(finite), 'non-finite grad_img in an earlier backward pass'

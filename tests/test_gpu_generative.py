@@ -162,3 +162,35 @@ def test_opacity_strokes_match_per_item_loop():
         out.append(generative.render(48, 48, shapes, groups, samples=4))
     loop = torch.stack(out, 0).permute(0, 3, 1, 2).contiguous()
     assert torch.equal(batched, loop)
+
+
+def test_render_batch_scenes_return_to_the_pool():
+    """A batched step releases its pooled scenes when its backward's findings are read,
+    so the next step of the same topologies rebuilds them instead of building new ones."""
+    from diffvg_amd import render_pytorch as rp
+    rp.synchronize()
+    rp.clear_scene_cache()
+    items = [scenes.CASES[c]() for c in ('overlap_eo', 'strokes', 'curve_fill')]
+    for sc in items:
+        for s in sc['shapes']:
+            s.points.requires_grad_(True)
+    lst = [(sc['canvas'][0], sc['canvas'][1], sc['shapes'], sc['groups']) for sc in items]
+    for step in range(3):
+        imgs = render_batch(lst, 2, 2, [1, 2, 3])
+        sum(im.sum() for im in imgs).backward()
+        del imgs
+        pd.synchronize()
+        tpls = rp._templates()
+        assert len(tpls) == 3
+        assert all(len(t.idle) == 1 for t in tpls), [len(t.idle) for t in tpls]
+
+
+def test_render_batch_reports_non_finite_grad_at_sync():
+    sc = scenes.CASES['overlap_eo']()
+    for s in sc['shapes']:
+        s.points.requires_grad_(True)
+    (img,) = render_batch([(sc['canvas'][0], sc['canvas'][1], sc['shapes'], sc['groups'])], 2, 2, [0])
+    img.backward(torch.full_like(img, float('nan')))
+    with pytest.raises(AssertionError):
+        pd.synchronize()
+    pd.synchronize()

@@ -187,13 +187,13 @@ def test_render_function_pools_scenes_and_matches_fresh():
     sc = _leaf_scene()
     target = torch.rand(64, 64, 4, device='cuda', generator=torch.Generator('cuda').manual_seed(0))
     _step(sc, target)  # host build; starts a pool for this topology
-    assert len(rp._TEMPLATES) == 1
+    assert len(rp._templates()) == 1
     with torch.no_grad():
         for s in sc['shapes']:
             s.points += 0.8
     img_pooled, g_pooled = _step(sc, target)  # device rebuild of the pooled scene
     pd.synchronize()  # the last backward's scene returns to the pool once its findings are read
-    assert len(rp._TEMPLATES) == 1 and len(rp._TEMPLATES[0].idle) >= 1
+    assert len(rp._templates()) == 1 and len(rp._templates()[0].idle) >= 1
     rp.clear_scene_cache()
     img_fresh, g_fresh = _step(sc, target)  # host build from the same values
     assert np.array_equal(img_pooled, img_fresh)
