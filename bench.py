@@ -48,7 +48,7 @@ def cpu_baseline_main(cfg):
     sys.path.insert(0, os.path.join(ROOT, 'tests'))
     from diffvg_amd import workloads
     from refdrive import flatten, load_ref, OracleRunner, RefRunner, load_oracle
-    c = workloads.config2(device='cpu', filter_type='tent') if cfg == 2 else workloads.CONFIGS[cfg](device='cpu')
+    c = workloads.closed_paths_300(device='cpu', filter_type='tent') if cfg == 2 else workloads.WORKLOADS[cfg](device='cpu')
     w, h = c['canvas']
     nsx, nsy = c['nsx'], c['nsy']
     f = flatten(w, h, c['shapes'], c['groups'], c['filter'].type, float(c['filter'].radius))
@@ -169,7 +169,7 @@ def bench_batched(a):
     dev = torch.device('cuda:0')
     pydiffvg.set_use_gpu(True)
     pydiffvg.set_device(dev)
-    items = workloads.config4_scenes(device=dev)
+    items = workloads.uneven_scenes(device=dev)
     n = len(items)
     params = [_scene_params(it['shapes'], it['groups']) for it in items]
     sizes = [it['canvas'] for it in items]
@@ -302,7 +302,7 @@ def bench_batched(a):
         'value': round(samples / t_graph / 1e6, 4), 'unit': 'Msamples/s', 'n_gpus': 1, 'steps': reps,
         'warmup': a.warmup, 'ms_per_step': round(t_graph * 1e3, 4), 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': 'f32 (f64 root solves)', 'data': 'synthetic',
-        'config': {'workload': 'config4_256_uneven_scenes', 'scenes': n, 'samples': samples,
+        'config': {'workload': 'uneven_scenes_256', 'scenes': n, 'samples': samples,
                    'paths': int(sum(len(it['shapes']) for it in items)), 'spp': [2, 2], 'filter': 'box r=0.5',
                    'mode': 'batched, one HIP graph per fwd+bwd'},
         'batched_end_to_end': {'ms_per_step': round(t_batched * 1e3, 3),
@@ -337,7 +337,7 @@ def bench_serving(a):
     n_req = a.requests
     specs = []
     for i in range(n_req):
-        r = workloads.config5_request(rng, device='cpu')
+        r = workloads.serving_request(rng, device='cpu')
         w, h = r['canvas']
         args = pydiffvg.RenderFunction.serialize_scene(w, h, r['shapes'], r['groups'], r['filter'])
         scene = _lib.DeviceScene(flatten_args(args).desc, 0)
@@ -388,7 +388,7 @@ def bench_serving(a):
         'value': round(samples / span_b / 1e6, 4), 'unit': 'Msamples/s', 'n_gpus': 1, 'steps': n_req,
         'warmup': 8, 'ms_per_step': round(span_b / n_req * 1e3, 4), 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': 'f32 (f64 root solves)', 'data': 'synthetic',
-        'config': {'workload': 'config5_serving_stream', 'requests': n_req, 'samples': samples,
+        'config': {'workload': 'serving_stream', 'requests': n_req, 'samples': samples,
                    'mean_paths': float(np.mean([s_[8] for s_ in specs])),
                    'sdf_fraction': float(np.mean([s_[5] for s_ in specs])),
                    'arrival_rate_rps': round(lam, 3), 'plain_service_rate_rps': round(mu, 3),
@@ -474,7 +474,7 @@ def main():
     pydiffvg.set_use_gpu(True)
     pydiffvg.set_device(dev)
     pydiffvg.set_render_mode(a.mode)
-    c = workloads.CONFIGS[a.config](device=dev)
+    c = workloads.WORKLOADS[a.config](device=dev)
     w, h = c['canvas']
     nsx, nsy = c['nsx'], c['nsy']
     params = make_params(c, dev)

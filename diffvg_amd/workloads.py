@@ -2,19 +2,19 @@
 
 Each generator returns a dict with canvas size, shapes, shape groups, filter, sample
 counts and, where the configuration defines one, a target image.  Generators draw from
-numpy.random.default_rng(config number); parameters are float32 torch tensors on
+numpy.random.default_rng(workload number); parameters are float32 torch tensors on
 `device`.
 
-  config1  single_circle: circle (128,128) r=40, fill (0.3,0.6,0.3,1), 256x256, 2x2, box
+  1 single_circle      circle (128,128) r=40, fill (0.3,0.6,0.3,1), 256x256, 2x2, box
            0.5; target circle (100,120) r=60 (apps/single_circle.py:9-27)
-  config2  512x512, 2x2, Gaussian filter (sigma 0.5, radius 1.5): 300 closed cubic
+  2 closed_paths_300   512x512, 2x2, Gaussian filter (sigma 0.5, radius 1.5): 300 closed cubic
            paths with 4-20 segments, points = centre + N(0, 15^2); fill alpha in
            [0.3, 1]; 30% also stroked with width in [0.5, 3]; even-odd fill
-  config3  512x512, 2x2, box 0.5: 2048 open 3-cubic strokes, random walk N(0, 20^2),
+  3 strokes_2048       512x512, 2x2, box 0.5: 2048 open 3-cubic strokes, random walk N(0, 20^2),
            width in [1, 4], uniform-noise target
-  config4  one scene of the uneven batch: path count log-uniform in [1, 512], 2-8
+  4 uneven_scenes      one scene of the uneven batch: path count log-uniform in [1, 512], 2-8
            cubics per path, resolution in {28, 64, 128, 256}, 10% linear/radial fills
-  config5  one request of the serving stream: path count log-uniform in [1, 4096],
+  5 serving_request    one request of the serving stream: path count log-uniform in [1, 4096],
            resolution in {128, 256, 512, 1024}, colour or SDF, prefilter on or off
 """
 import math
@@ -32,12 +32,12 @@ def _t(x, device):
     return torch.tensor(np.asarray(x, dtype=np.float32), device=device)
 
 
-def config1(device='cpu'):
+def single_circle(device='cpu'):
     c = Circle(radius=_t(40.0, device), center=_t([128.0, 128.0], device))
     g = ShapeGroup(shape_ids=torch.tensor([0], dtype=torch.int32), fill_color=_t([0.3, 0.6, 0.3, 1.0], device))
     tc = Circle(radius=_t(60.0, device), center=_t([100.0, 120.0], device))
     tg = ShapeGroup(shape_ids=torch.tensor([0], dtype=torch.int32), fill_color=_t([0.3, 0.6, 0.3, 1.0], device))
-    return dict(name='config1_single_circle', canvas=(256, 256), shapes=[c], groups=[g],
+    return dict(name='single_circle_256', canvas=(256, 256), shapes=[c], groups=[g],
                 filter=PixelFilter(type=diffvg.FilterType.box, radius=torch.tensor(0.5)), nsx=2, nsy=2,
                 target_scene=([tc], [tg]))
 
@@ -71,14 +71,14 @@ def closed_paths(rng, n_paths, size, seg_lo, seg_hi, spread, device, stroke_frac
     return shapes, groups
 
 
-def config2(device='cpu', filter_type='gaussian'):
+def closed_paths_300(device='cpu', filter_type='gaussian'):
     rng = np.random.default_rng(2)
     shapes, groups = closed_paths(rng, 300, 512.0, 4, 20, 15.0, device)
     if filter_type == 'gaussian':
         filt = PixelFilter(type=diffvg.FilterType.gaussian, radius=torch.tensor(1.5))
     else:  # parity twin (the reference has no Gaussian filter)
         filt = PixelFilter(type=diffvg.FilterType.tent, radius=torch.tensor(1.0))
-    return dict(name='config2_300_closed_cubic_paths_512', canvas=(512, 512), shapes=shapes, groups=groups,
+    return dict(name='closed_paths_300_512', canvas=(512, 512), shapes=shapes, groups=groups,
                 filter=filt, nsx=2, nsy=2)
 
 
@@ -94,16 +94,16 @@ def strokes(rng, n_paths, size, step, device, wlo=1.0, whi=4.0):
     return shapes, groups
 
 
-def config3(device='cpu'):
+def strokes_2048(device='cpu'):
     rng = np.random.default_rng(3)
     shapes, groups = strokes(rng, 2048, 512.0, 20.0, device)
     target = torch.from_numpy(rng.uniform(0, 1, (512, 512, 4)).astype(np.float32)).to(device)
-    return dict(name='config3_2048_strokes_512', canvas=(512, 512), shapes=shapes, groups=groups,
+    return dict(name='strokes_2048_512', canvas=(512, 512), shapes=shapes, groups=groups,
                 filter=PixelFilter(type=diffvg.FilterType.box, radius=torch.tensor(0.5)), nsx=2, nsy=2,
                 target=target)
 
 
-def config4_scenes(n_scenes=256, device='cpu', seed=4):
+def uneven_scenes(n_scenes=256, device='cpu', seed=4):
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n_scenes):
@@ -111,12 +111,12 @@ def config4_scenes(n_scenes=256, device='cpu', seed=4):
         n = max(1, int(math.floor(math.exp(rng.uniform(0, math.log(512))))))
         shapes, groups = closed_paths(rng, n, float(res), 2, 8, res * 15.0 / 512.0 * 4, device, stroke_frac=0.0,
                                       grad_frac=0.1)
-        out.append(dict(name='config4_scene%d' % k, canvas=(res, res), shapes=shapes, groups=groups,
+        out.append(dict(name='uneven_scene%d' % k, canvas=(res, res), shapes=shapes, groups=groups,
                         filter=PixelFilter(type=diffvg.FilterType.box, radius=torch.tensor(0.5)), nsx=2, nsy=2))
     return out
 
 
-def config5_request(rng, device='cpu'):
+def serving_request(rng, device='cpu'):
     res = int(rng.choice([128, 256, 512, 1024]))
     n = max(1, int(math.floor(math.exp(rng.uniform(0, math.log(4096))))))
     # the path size shrinks with the path count so that the expected overlap depth stays
@@ -125,9 +125,9 @@ def config5_request(rng, device='cpu'):
     shapes, groups = closed_paths(rng, n, float(res), 2, 8, spread, device, stroke_frac=0.3)
     sdf = rng.uniform() < 0.5
     prefilter = rng.uniform() < 0.5
-    return dict(name='config5_request', canvas=(res, res), shapes=shapes, groups=groups,
+    return dict(name='serving_request', canvas=(res, res), shapes=shapes, groups=groups,
                 filter=PixelFilter(type=diffvg.FilterType.box, radius=torch.tensor(0.5)),
                 nsx=1 if sdf else 2, nsy=1 if sdf else 2, sdf=sdf, prefilter=prefilter)
 
 
-CONFIGS = {1: config1, 2: config2, 3: config3}
+WORKLOADS = {1: single_circle, 2: closed_paths_300, 3: strokes_2048}

@@ -59,10 +59,10 @@ def fixture_name(case, filt):
     return case if filt is None else '%s_%s' % (case, filt[0])
 
 
-def config1_target():
+def single_circle_target():
     from refdrive import flatten
     from diffvg_amd import workloads
-    c = workloads.config1()
+    c = workloads.single_circle()
     shapes, groups = c['target_scene']
     w, h = c['canvas']
     return flatten(w, h, shapes, groups, 0, 0.5)
@@ -88,7 +88,7 @@ def main():
         out['image'] = r.render(w, h, NS, NS, SEED)['image']
         if case == 'single_circle_256':
             # the single_circle.py iteration: L2 loss against the circle at (100,120), r = 60
-            tw = config1_target()
+            tw = single_circle_target()
             target = RefRunner(tw).render(w, h, NS, NS, SEED)['image']
             out['target'] = target
             d_img = (2.0 * (out['image'] - target)).astype(np.float32)

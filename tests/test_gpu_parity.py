@@ -126,7 +126,7 @@ def test_gpu_full_workloads_vs_oracle(cfg):
     """The full 512x512 benchmark scenes: forward, prefiltered forward and the backward
     with edge sampling, against the oracle at the same seed."""
     print('full workload', cfg, flush=True)
-    c = workloads.config2(device='cpu', filter_type='tent') if cfg == 2 else workloads.config3(device='cpu')
+    c = workloads.closed_paths_300(device='cpu', filter_type='tent') if cfg == 2 else workloads.strokes_2048(device='cpu')
     w, h = c['canvas']
     f = flatten(w, h, c['shapes'], c['groups'], c['filter'].type, float(c['filter'].radius))
     o, g = OracleRunner(f), GpuRunner(f)
