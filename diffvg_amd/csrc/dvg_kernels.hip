@@ -34,6 +34,28 @@ namespace dvg {
 
 constexpr int kBlock = 256;
 constexpr int kCandCap = 2048;
+// Waves per SIMD each heavy shading kernel is compiled for, i.e. its register budget
+// (512 / waves VGPRs per lane; beyond it the compiler spills to scratch).  Chosen by
+// measurement on the 300-path and 2,048-stroke scenes: these kernels wait on memory and
+// on long FP64 dependency chains, and 5-6 waves hide that better than 2 waves without
+// spills (k_edge 5.03 -> 3.16 ms, k_color 1.45 -> 0.92 ms, k_color_bwd 1.90 -> 1.33 ms).
+#ifndef DVG_WAVES_COLOR
+#define DVG_WAVES_COLOR 5
+#endif
+#ifndef DVG_WAVES_COLOR_BWD
+#define DVG_WAVES_COLOR_BWD 5
+#endif
+#ifndef DVG_WAVES_EDGE
+#define DVG_WAVES_EDGE 6
+#endif
+#ifndef DVG_WAVES_SDF
+#define DVG_WAVES_SDF 5
+#endif
+// the batched backward (its job lookup adds live state) measured fastest at 2 waves
+// (256 uneven scenes: 29.1 ms at 2 waves, 38.0 ms at 5)
+#ifndef DVG_WAVES_BATCH_COLOR_BWD
+#define DVG_WAVES_BATCH_COLOR_BWD 2
+#endif
 
 // Render geometry shared by the kernels.
 struct Geom {
@@ -295,7 +317,7 @@ DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float 
     if (overflow == 2) atomicOr(err, 2);
 }
 template <int PRE>
-__global__ __launch_bounds__(kBlock) void k_color(DevScene S, Geom G, const float *bg, float4 *colors, int *err) {
+__global__ __launch_bounds__(kBlock, DVG_WAVES_COLOR) void k_color(DevScene S, Geom G, const float *bg, float4 *colors, int *err) {
     color_tile<PRE>(S, G, blockIdx.x, bg, colors, err);
 }
 
@@ -451,7 +473,7 @@ DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const fl
     if (overflow) atomicOr(err, overflow);
 }
 template <int PRE>
-__global__ __launch_bounds__(kBlock) void k_color_bwd(DevScene S, Geom G, const float *bg, const float *weight,
+__global__ __launch_bounds__(kBlock, DVG_WAVES_COLOR_BWD) void k_color_bwd(DevScene S, Geom G, const float *bg, const float *weight,
                                                       const float *d_img, float *d_bg, float *d_translation,
                                                       int *err) {
     color_bwd_tile<PRE>(S, G, blockIdx.x, bg, weight, d_img, d_bg, d_translation, err);
@@ -612,7 +634,7 @@ DVG_DEV void edge_chunk(const DevScene &S, const Geom &G, int chunk, const EdgeS
         }
     }
 }
-__global__ __launch_bounds__(kBlock) void k_edge(DevScene S, Geom G, const EdgeSample *edges, const int *tile_off,
+__global__ __launch_bounds__(kBlock, DVG_WAVES_EDGE) void k_edge(DevScene S, Geom G, const EdgeSample *edges, const int *tile_off,
                                                  const int *tile_count, const int *chunk_off, int ntiles,
                                                  const int *sorted, const float *bg, const float *weight,
                                                  const float *d_img, float *d_translation) {
@@ -794,7 +816,7 @@ DVG_DEV void sdf_tile(const DevScene &S, const Geom &G, int tile, float *sdf, co
     }
     if (valid && sdf) aadd(sdf + (y * G.W + x), dist_out);
 }
-__global__ __launch_bounds__(kBlock) void k_sdf_tiled(DevScene S, Geom G, float *sdf, const float *d_sdf,
+__global__ __launch_bounds__(kBlock, DVG_WAVES_SDF) void k_sdf_tiled(DevScene S, Geom G, float *sdf, const float *d_sdf,
                                                       float *d_translation) {
     sdf_tile(S, G, blockIdx.x, sdf, d_sdf, d_translation);
 }
@@ -1435,7 +1457,7 @@ __global__ __launch_bounds__(kBlock) void kb_weight(const Job *jobs, const int *
 }
 
 template <int PRE>
-__global__ __launch_bounds__(kBlock, 2) void kb_color(const Job *jobs, const int *pref, int nj) {
+__global__ __launch_bounds__(kBlock, DVG_WAVES_COLOR) void kb_color(const Job *jobs, const int *pref, int nj) {
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     if (!J.img) return;
@@ -1450,7 +1472,7 @@ __global__ __launch_bounds__(kBlock) void kb_resolve(const Job *jobs, const int 
 }
 
 template <int PRE>
-__global__ __launch_bounds__(kBlock, 2) void kb_color_bwd(const Job *jobs, const int *pref, int nj) {
+__global__ __launch_bounds__(kBlock, DVG_WAVES_BATCH_COLOR_BWD) void kb_color_bwd(const Job *jobs, const int *pref, int nj) {
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     if (!J.d_img) return;
@@ -1479,7 +1501,7 @@ __global__ __launch_bounds__(kBlock) void kb_edge_bin(const Job *jobs, const int
                  J.edge_sorted);
 }
 
-__global__ __launch_bounds__(kBlock, 2) void kb_edge(const Job *jobs, const int *pref, int nj) {
+__global__ __launch_bounds__(kBlock, DVG_WAVES_EDGE) void kb_edge(const Job *jobs, const int *pref, int nj) {
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     if (!J.d_img) return;
@@ -1487,7 +1509,7 @@ __global__ __launch_bounds__(kBlock, 2) void kb_edge(const Job *jobs, const int 
                J.edge_sorted, J.bg, J.weight, J.d_img, J.d_tr);
 }
 
-__global__ __launch_bounds__(kBlock, 2) void kb_sdf(const Job *jobs, const int *pref, int nj) {
+__global__ __launch_bounds__(kBlock, DVG_WAVES_SDF) void kb_sdf(const Job *jobs, const int *pref, int nj) {
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     if (J.eval)
