@@ -308,7 +308,9 @@ DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float 
                 shade_list(S, pt, s_list, n, comp, nullptr, th);
             }
         }
-        __syncthreads();
+        // the list is rebuilt only for another range of groups: after the last one, a
+        // wave that is done does not wait for the slower waves of its workgroup
+        if (c0 + kCandCap < S.ng) __syncthreads();
     }
     if (valid) {
         V4 c = comp.result(has_bg, bgc);
@@ -387,7 +389,7 @@ DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const fl
             else
                 shade_list_record(S, pt, s_list, n, comp, frags, overflow);
         }
-        __syncthreads();
+        if (c0 + kCandCap < S.ng) __syncthreads();
     }
     float d_radius = 0.f;
     int kind = -1;
@@ -601,7 +603,7 @@ DVG_DEV void edge_chunk(const DevScene &S, const Geom &G, int chunk, const EdgeS
                 shade_list(S, pin, s_list, nc, ci, &qin, hin);
                 shade_list(S, pout, s_list, nc, co, &qout, hout);
             }
-            __syncthreads();
+            if (c0 + kCandCap < S.ng) __syncthreads();
         }
         // every lane stays to the end: the gradient scatter combines lanes of a wave
         bool active = valid;
@@ -792,7 +794,7 @@ DVG_DEV void sdf_tile(const DevScene &S, const Geom &G, int tile, float *sdf, co
                 }
             }
         }
-        __syncthreads();
+        if (c1 - kCandCap > 0) __syncthreads();
     }
     float weight = 1.f / G.spp;
     float dist_out = min_distance;

@@ -13,9 +13,10 @@
 //     (ends and extrema) must be at least `ey` away from zero, so each piece crosses
 //     the ray at most once, away from the piece ends (no root near t = 0 or 1, no
 //     tangency);
-//   * each crossing is bracketed by bisection to 2^-12, and |y'| must stay above a
-//     margin over the bracket, so the sign the reference computes at its root is the
-//     crossing direction;
+//   * each crossing is bracketed by bisection to 2^-16 of its piece (fine enough that
+//     points 1e-4 of the canvas from the curve, where the edge kernel shades both sides
+//     of a boundary, are decided), and |y'| must stay above a margin over the bracket,
+//     so the sign the reference computes at its root is the crossing direction;
 //   * x(t) - pt.x must be certainly positive or certainly negative over the bracket.
 // When any test is uncertain it returns false and the caller runs the reference's
 // solve.  Host and device compile the same code (no floating-point contraction).
@@ -120,10 +121,10 @@ DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float
     for (int i = 0; i + 1 < nb; i++) {
         const float fa = fv[i], fb = fv[i + 1];
         if ((fa > 0.f) == (fb > 0.f)) continue;  // no crossing on this monotone piece
-        // bisection to a bracket of width (b - a) / 4096
+        // bisection to a bracket of width (b - a) / 65536
         float lo = br[i], hi = br[i + 1];
         const bool rising = fb > 0.f;
-        for (int it = 0; it < 12; it++) {
+        for (int it = 0; it < 16; it++) {
             float mid = 0.5f * (lo + hi);
             float fm = bern3(c0, c1, c2, c3, mid);
             if ((fm > 0.f) == rising)

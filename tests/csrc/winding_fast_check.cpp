@@ -38,6 +38,20 @@ void wf_fuzz(int mode, int64_t n, uint64_t seed, int64_t *out, float *bad) {
             float y1 = std::fmax(std::fmax(p[0].y, p[1].y), std::fmax(p[2].y, p[3].y));
             pt = orc::V2{x0 - 0.3f * (x1 - x0) + 1.3f * (x1 - x0) * U(g), y0 + (y1 - y0) * U(g)};
         }
+        if (mode == 6) {
+            // both sides of a boundary sample, as the edge kernel shades them: a curve
+            // point moved 1e-4 of a 512-unit canvas along the normal
+            for (int k = 0; k < 4; k++) p[k] = orc::V2{p[k].x * 8.f, p[k].y * 8.f};
+            float t = U(g);
+            float s = 1.f - t;
+            float x = s * s * s * p[0].x + 3 * s * s * t * p[1].x + 3 * s * t * t * p[2].x + t * t * t * p[3].x;
+            float y = s * s * s * p[0].y + 3 * s * s * t * p[1].y + 3 * s * t * t * p[2].y + t * t * t * p[3].y;
+            float dx = 3 * s * s * (p[1].x - p[0].x) + 6 * s * t * (p[2].x - p[1].x) + 3 * t * t * (p[3].x - p[2].x);
+            float dy = 3 * s * s * (p[1].y - p[0].y) + 6 * s * t * (p[2].y - p[1].y) + 3 * t * t * (p[3].y - p[2].y);
+            float l = std::sqrt(dx * dx + dy * dy) + 1e-30f;
+            float side = U(g) < 0.5f ? 1.f : -1.f;
+            pt = orc::V2{x - side * 0.0512f * dy / l, y + side * 0.0512f * dx / l};
+        }
         if (mode == 2) {
             // the y of an interior extremum of the curve, nudged by a few ulps
             float d0 = p[1].y - p[0].y, d1 = p[2].y - p[1].y, d2 = p[3].y - p[2].y;
