@@ -15,3 +15,4 @@ from .render_pytorch import (RenderFunction, OutputType, set_print_timing, set_r
 from .batch import Batch  # noqa: F401
 from .parse_svg import svg_to_scene  # noqa: F401
 from .save_svg import save_svg  # noqa: F401
+from .optimize_svg import SvgOptimizationSettings, OptimizableSvg  # noqa: F401
