@@ -117,8 +117,12 @@ def _cat_int(tensors):
     if not tensors:
         return np.zeros(0, np.int32)
     with torch.no_grad():
-        return torch.cat([t.reshape(-1) for t in tensors] if any(t.dim() != 1 for t in tensors)
-                         else tensors).to(torch.int32).cpu().numpy()
+        flat = [t.reshape(-1) for t in tensors] if any(t.dim() != 1 for t in tensors) else tensors
+        if len({t.device for t in flat}) > 1:
+            # index tensors on several devices (shape ids made on the host, control
+            # point counts moved to the GPU): gather them on the host
+            flat = [t.cpu() for t in flat]
+        return torch.cat(flat).to(torch.int32).cpu().numpy()
 
 
 def flatten_args(args):

@@ -106,17 +106,21 @@ def test_training_recovers_rotation_and_translation(tmp_path):
     o = pd.OptimizableSvg(str(src), s)
     node = o.root.children[0].transform_optim
     losses = []
-    for it in range(60):
+    for it in range(150):
         o.zero_grad()
         img = o.render(seed=it)
         loss = (img - target).pow(2).mean()
         loss.backward()
         losses.append(float(loss))
         o.step()
+        if it % 25 == 0:
+            print(it, losses[-1], float(node.Theta), float(node.scale), node.translation.tolist(), flush=True)
     pd.synchronize()
     theta = float(node.Theta)
-    # the group's own translation is applied before its rotation: recover the canvas shift
+    # the group's translation is applied before its rotation and scale: compare the
+    # composed matrix's offset with the target's canvas offset
     M = node.get_transform().detach().cpu().numpy()
-    assert np.mean(losses[-5:]) < 0.25 * np.mean(losses[:3]), losses
-    assert abs(theta - angle) < 0.25 * angle, theta
-    np.testing.assert_allclose(M[0:2, 2], [32 + shift[0], 32 + shift[1]], atol=1.0)
+    print('final', losses[-1], theta, M.tolist(), flush=True)
+    assert np.mean(losses[-5:]) < 0.1 * np.mean(losses[:3]), losses
+    assert abs(theta - angle) < 0.1 * angle, theta
+    np.testing.assert_allclose(M[0:2, 2], [32 + shift[0], 32 + shift[1]], atol=0.5)
