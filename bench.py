@@ -504,14 +504,19 @@ def main():
     names = ctypes.create_string_buffer(4096)
     ms = (ctypes.c_double * 32)()
     cnts = (ctypes.c_int * 32)()
-    L.dvg_profile_read(names, 4096, ms, cnts, 32)
-    L.dvg_profile_enable(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # per-kernel times: the same steps again with the library's events around each launch
+    # (kept out of the timed steps above, whose launches they would space out)
+    L.dvg_profile_read(names, 4096, ms, cnts, 32)
+    L.dvg_profile_enable(1)
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
     L.dvg_profile_enable(0)
     nk = L.dvg_profile_read(names, 4096, ms, cnts, 32)
     kern = {}

@@ -17,8 +17,8 @@ import torch
 from . import _lib
 from .batch import Batch
 from .device import get_device
-from .render_pytorch import (OutputType, _PENDING, _check_gpu, _device_scene_for, _grad_tensors, _stream,
-                             check_rebuild, synchronize)
+from .render_pytorch import (OutputType, _PENDING, _Lease, _check_gpu, _device_scene_for, _grad_tensors,
+                             _stream, check_rebuild, synchronize)
 
 __all__ = ['BatchRenderFunction', 'render_batch']
 
@@ -30,7 +30,7 @@ class BatchRenderFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, spec, *args):
-        _check_gpu()  # also reads the findings of earlier backward passes
+        _check_gpu(read_findings=False)
         dev = get_device()
         st = _stream()
         scenes, flats, images, jobs, spans = [], [], [], [], []
@@ -52,8 +52,17 @@ class BatchRenderFunction(torch.autograd.Function):
                                      use_prefiltering=flat.use_prefiltering)))
         if k != len(args):
             raise ValueError('spec does not account for every argument')
-        for sc in scenes:
-            check_rebuild(sc)
+        try:
+            for sc in scenes:
+                check_rebuild(sc)
+            # findings of earlier backward passes, read once this call's scene work is on
+            # the stream (the rebuild checks have waited for their kernels)
+            synchronize()
+        except BaseException:
+            for sc in scenes:
+                if isinstance(sc, _Lease):
+                    sc.release()
+            raise
         fwd = Batch(jobs, zero_grads=False)
         fwd.render(st)
         ctx.spec, ctx.spans, ctx.scenes, ctx.flats = spec, spans, scenes, flats

@@ -439,9 +439,15 @@ class _Lease:
     def __getattr__(self, name):
         return getattr(self._scene, name)
 
+    def release(self):
+        """Returns the scene to its pool now (once); the lease is unusable afterwards."""
+        tpl, self._template = self._template, None
+        if tpl is not None:
+            tpl.release(self._scene)
+
     def __del__(self):
         try:
-            self._template.release(self._scene)
+            self.release()
         except Exception:
             pass
 
@@ -524,8 +530,9 @@ def synchronize():
         assert bool(finite), 'non-finite grad_img in an earlier backward pass'
 
 
-def _check_gpu():
-    synchronize()
+def _check_gpu(read_findings=True):
+    if read_findings:
+        synchronize()
     if not get_use_gpu() or get_device().type != 'cuda':
         raise RuntimeError('diffvg_amd renders on the GPU only; call diffvg_amd.set_use_gpu(True) '
                            'on a machine with a ROCm GPU')
@@ -703,7 +710,7 @@ class RenderFunction(torch.autograd.Function):
 
     @staticmethod
     def _prepare(width, height, background_image, args, output_type=None):
-        _check_gpu()
+        _check_gpu(read_findings=False)
         dev = get_device()
         start = time.time()
         if _render_mode == _lib.MODE_TILED:
@@ -712,6 +719,15 @@ class RenderFunction(torch.autograd.Function):
             # the comparator: a host-built scene per call, as the reference builds it
             flat = flatten_args(args)
             scene = _lib.DeviceScene(flat.desc, dev.index if dev.index is not None else -1, _render_mode)
+        # Findings of earlier backward passes, read once this call's scene work is on the
+        # stream: the host prepares this render while the previous backward's kernels
+        # run, and the rebuild check above has already waited for them.
+        try:
+            synchronize()
+        except BaseException:
+            if isinstance(scene, _Lease):
+                scene.release()
+            raise
         if print_timing:
             print('Scene construction, time: %.5f s' % (time.time() - start))
         return flat, scene
