@@ -1826,75 +1826,225 @@ DVG_DEV float4 to_f4(const bld::Box &b) { return make_float4(b.minx, b.miny, b.m
 
 // Per shape (scene.cpp:113-333, 496-619): local box, boundary length; for paths the
 // segment boxes, radii, length CDF/PMF and the segment BVH's leaf visit order.
-__global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
-    const DevScene &S = A.S;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S.ns) return;
-    const int type = S.shape_type[s];
-    bld::Box box;
-    float len;
-    if (type != 2) {
-        len = bld::basic_shape(type, S.params + S.param_off[s], box);
-    } else {
-        const int c0 = S.seg_begin[s], nb = S.nsegs[s], p0 = S.pt_off[s], np = S.npt[s];
-        const bool thick = S.has_thick[s] != 0;
-        const float r = S.stroke_width[s];
-        bool finite = true;
-        box = bld::Box{S.points[p0].x, S.points[p0].y, S.points[p0].x, S.points[p0].y};
-        for (int i = 0; i < np; i++) {
-            float2 q = S.points[p0 + i];
-            finite = finite && isfinite(q.x) && isfinite(q.y) && (!thick || isfinite(S.thickness[p0 + i]));
-            if (i > 0) box = bld::merge(box, bld::P2{q.x, q.y});
+//
+// One wave per shape.  Every value is the one the host build computes: boxes are folds
+// whose result does not depend on the order of the points (the first point attaining
+// each extreme, as the sequential fold keeps it), the length sum and the CDF are
+// sequential (one lane, from LDS), and the leaf order is the sorted order of the
+// segments' y-centres -- by rank when the keys are distinct, else by the same introsort
+// on one lane.  Paths with more than kRefitSegCap segments run on one lane from global
+// memory.
+constexpr int kRefitWaves = kBlock / 64;
+constexpr int kRefitSegCap = 512;
+
+// (v, i) <- the first (lowest index) of the extreme values in the wave, as a sequential
+// fold `cur = p < cur ? p : cur` (MIN) or `cur = p > cur ? p : cur` keeps it
+template <bool MIN>
+DVG_DEV void wave_first_extreme(float &v, int &i) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        float ov = __shfl_xor(v, off);
+        int oi = __shfl_xor(i, off);
+        bool better = MIN ? ov < v : ov > v;
+        bool worse = MIN ? v < ov : v > ov;
+        if (better || (!worse && oi < i)) {
+            v = ov;
+            i = oi;
         }
-        float4 *seg_box = writable(S.seg_box);
-        float *seg_maxr = writable(S.seg_maxr), *seg_pmf = writable(S.seg_pmf), *seg_cdf = writable(S.seg_cdf);
-        float length = 0.f;
-        for (int i = 0; i < nb; i++) {
-            const int seg = c0 + i, k = S.seg_kind[seg];
-            const int4 ix = S.seg_idx[seg];
-            const int id[4] = {ix.x, ix.y, ix.z, ix.w};
-            bld::P2 p[4];
-            bld::Box b = bld::box_empty();
-            float rr = -1;
-            for (int j = 0; j <= k + 1; j++) {
-                float2 q = S.points[id[j]];
-                p[j] = bld::P2{q.x, q.y};
-                b = bld::merge(b, p[j]);
-                float rj = thick ? S.thickness[id[j]] : r;
-                rr = j == 0 ? rj : bld::mx(rr, rj);
-            }
-            seg_box[seg] = to_f4(b);
-            seg_maxr[seg] = rr;
-            float l = bld::segment_length(k, p);
-            seg_pmf[seg] = l;
-            length += l;
-        }
-        len = 0.f + length;
-        const float inv_length = 1.f / len;
-        for (int i = 0; i < nb; i++) {
-            float dd = seg_pmf[c0 + i] * inv_length;
-            seg_pmf[c0 + i] = dd;
-            seg_cdf[c0 + i] = i == 0 ? dd : dd + seg_cdf[c0 + i - 1];
-        }
-        // leaves sorted by box y-centre (scene.cpp:602-611), visited in the pairwise
-        // BVH's depth-first order
-        int *order = A.R.order + c0;
-        for (int i = 0; i < nb; i++) order[i] = i;
-        int stack[192];
-        stl::sort(
-            order, nb,
-            [&](int a, int b) {
-                return bld::box_ycentre(to_box(seg_box[c0 + a])) < bld::box_ycentre(to_box(seg_box[c0 + b]));
-            },
-            stack);
-        int *seg_dfs = writable(S.seg_dfs);
-        for (int i = 0; i < nb; i++) seg_dfs[c0 + i] = c0 + order[A.R.seg_perm[c0 + i]];
-        A.R.thick_leaf0[s] = seg_maxr[c0 + order[0]];
-        if (!finite) atomicOr(A.uerr, kUpdNonFinitePoints);
     }
+}
+
+DVG_DEV void refit_path_serial(const RefitArgs &A, int s) {
+    const DevScene &S = A.S;
+    const int c0 = S.seg_begin[s], nb = S.nsegs[s], p0 = S.pt_off[s], np = S.npt[s];
+    const bool thick = S.has_thick[s] != 0;
+    const float r = S.stroke_width[s];
+    bool finite = true;
+    bld::Box box{S.points[p0].x, S.points[p0].y, S.points[p0].x, S.points[p0].y};
+    for (int i = 0; i < np; i++) {
+        float2 q = S.points[p0 + i];
+        finite = finite && isfinite(q.x) && isfinite(q.y) && (!thick || isfinite(S.thickness[p0 + i]));
+        if (i > 0) box = bld::merge(box, bld::P2{q.x, q.y});
+    }
+    float4 *seg_box = writable(S.seg_box);
+    float *seg_maxr = writable(S.seg_maxr), *seg_pmf = writable(S.seg_pmf), *seg_cdf = writable(S.seg_cdf);
+    float length = 0.f;
+    for (int i = 0; i < nb; i++) {
+        const int seg = c0 + i, k = S.seg_kind[seg];
+        const int4 ix = S.seg_idx[seg];
+        const int id[4] = {ix.x, ix.y, ix.z, ix.w};
+        bld::P2 p[4];
+        bld::Box b = bld::box_empty();
+        float rr = -1;
+        for (int j = 0; j <= k + 1; j++) {
+            float2 q = S.points[id[j]];
+            p[j] = bld::P2{q.x, q.y};
+            b = bld::merge(b, p[j]);
+            float rj = thick ? S.thickness[id[j]] : r;
+            rr = j == 0 ? rj : bld::mx(rr, rj);
+        }
+        seg_box[seg] = to_f4(b);
+        seg_maxr[seg] = rr;
+        float l = bld::segment_length(k, p);
+        seg_pmf[seg] = l;
+        length += l;
+    }
+    const float len = 0.f + length;
+    const float inv_length = 1.f / len;
+    for (int i = 0; i < nb; i++) {
+        float dd = seg_pmf[c0 + i] * inv_length;
+        seg_pmf[c0 + i] = dd;
+        seg_cdf[c0 + i] = i == 0 ? dd : dd + seg_cdf[c0 + i - 1];
+    }
+    int *order = A.R.order + c0;
+    for (int i = 0; i < nb; i++) order[i] = i;
+    int stack[192];
+    stl::sort(
+        order, nb,
+        [&](int a, int b) {
+            return bld::box_ycentre(to_box(seg_box[c0 + a])) < bld::box_ycentre(to_box(seg_box[c0 + b]));
+        },
+        stack);
+    int *seg_dfs = writable(S.seg_dfs);
+    for (int i = 0; i < nb; i++) seg_dfs[c0 + i] = c0 + order[A.R.seg_perm[c0 + i]];
+    A.R.thick_leaf0[s] = seg_maxr[c0 + order[0]];
+    if (!finite) atomicOr(A.uerr, kUpdNonFinitePoints);
     writable(S.shape_box)[s] = to_f4(box);
     writable(S.shape_length)[s] = len;
     if (!isfinite(len)) atomicOr(A.uerr, kUpdNonFiniteLength);
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
+    const DevScene &S = A.S;
+    __shared__ float s_key[kRefitWaves][kRefitSegCap];
+    __shared__ float s_len[kRefitWaves][kRefitSegCap];
+    __shared__ int s_ord[kRefitWaves][kRefitSegCap];
+    __shared__ float s_maxr[kRefitWaves][kRefitSegCap];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int s = blockIdx.x * kRefitWaves + wv;
+    if (s >= S.ns) return;  // whole waves; nothing below synchronises the workgroup
+    const int type = S.shape_type[s];
+    if (type != 2) {
+        if (lane == 0) {
+            bld::Box box;
+            float len = bld::basic_shape(type, S.params + S.param_off[s], box);
+            writable(S.shape_box)[s] = to_f4(box);
+            writable(S.shape_length)[s] = len;
+            if (!isfinite(len)) atomicOr(A.uerr, kUpdNonFiniteLength);
+        }
+        return;
+    }
+    const int c0 = S.seg_begin[s], nb = S.nsegs[s], p0 = S.pt_off[s], np = S.npt[s];
+    if (nb > kRefitSegCap) {
+        if (lane == 0) refit_path_serial(A, s);
+        return;
+    }
+    float *key = s_key[wv], *lens = s_len[wv], *maxr = s_maxr[wv];
+    int *ord = s_ord[wv];
+    const bool thick = S.has_thick[s] != 0;
+    const float r = S.stroke_width[s];
+    // shape box and finiteness over the points
+    float x0 = __builtin_inff(), y0 = x0, x1 = -x0, y1 = -x0;
+    int ix0 = INT_MAX, iy0 = INT_MAX, ix1 = INT_MAX, iy1 = INT_MAX;
+    bool finite = true;
+    for (int i = lane; i < np; i += 64) {
+        float2 q = S.points[p0 + i];
+        finite = finite && isfinite(q.x) && isfinite(q.y) && (!thick || isfinite(S.thickness[p0 + i]));
+        if (ix0 == INT_MAX) {
+            x0 = x1 = q.x;
+            y0 = y1 = q.y;
+            ix0 = iy0 = ix1 = iy1 = i;
+        } else {
+            if (q.x < x0) { x0 = q.x; ix0 = i; }
+            if (q.y < y0) { y0 = q.y; iy0 = i; }
+            if (q.x > x1) { x1 = q.x; ix1 = i; }
+            if (q.y > y1) { y1 = q.y; iy1 = i; }
+        }
+    }
+    wave_first_extreme<true>(x0, ix0);
+    wave_first_extreme<true>(y0, iy0);
+    wave_first_extreme<false>(x1, ix1);
+    wave_first_extreme<false>(y1, iy1);
+    const bool all_finite = __all(finite);
+    // per segment: box, radius, length; y-centre keys for the leaf sort
+    float4 *seg_box = writable(S.seg_box);
+    float *seg_maxr = writable(S.seg_maxr), *seg_pmf = writable(S.seg_pmf), *seg_cdf = writable(S.seg_cdf);
+    for (int i = lane; i < nb; i += 64) {
+        const int seg = c0 + i, k = S.seg_kind[seg];
+        const int4 ix = S.seg_idx[seg];
+        const int id[4] = {ix.x, ix.y, ix.z, ix.w};
+        bld::P2 p[4];
+        bld::Box b = bld::box_empty();
+        float rr = -1;
+        for (int j = 0; j <= k + 1; j++) {
+            float2 q = S.points[id[j]];
+            p[j] = bld::P2{q.x, q.y};
+            b = bld::merge(b, p[j]);
+            float rj = thick ? S.thickness[id[j]] : r;
+            rr = j == 0 ? rj : bld::mx(rr, rj);
+        }
+        seg_box[seg] = to_f4(b);
+        seg_maxr[seg] = rr;
+        maxr[i] = rr;
+        lens[i] = bld::segment_length(k, p);
+        key[i] = bld::box_ycentre(b);
+        ord[i] = i;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // leaf order by rank when the keys are distinct numbers
+    bool plain = true;
+    int rank[kRefitSegCap / 64];
+#pragma unroll
+    for (int t = 0; t < kRefitSegCap / 64; t++) {
+        const int i = lane + 64 * t;
+        rank[t] = 0;
+        if (i >= nb) continue;
+        const float ki = key[i];
+        plain = plain && ki == ki;
+        for (int j = 0; j < nb; j++) {
+            const float kj = key[j];
+            rank[t] += kj < ki ? 1 : 0;
+            plain = plain && (j == i || kj < ki || ki < kj);
+        }
+    }
+    const bool distinct = __all(plain);
+    __builtin_amdgcn_wave_barrier();
+    if (distinct) {
+#pragma unroll
+        for (int t = 0; t < kRefitSegCap / 64; t++) {
+            const int i = lane + 64 * t;
+            if (i < nb) ord[rank[t]] = i;
+        }
+    } else if (lane == 0) {
+        int stack[192];
+        stl::sort(ord, nb, [&](int a, int b) { return key[a] < key[b]; }, stack);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int *seg_dfs = writable(S.seg_dfs);
+    for (int i = lane; i < nb; i += 64) seg_dfs[c0 + i] = c0 + ord[A.R.seg_perm[c0 + i]];
+    if (lane == 0) {
+        // the sequential length sum, PMF and CDF of the host build
+        float length = 0.f;
+        for (int i = 0; i < nb; i++) length += lens[i];
+        const float len = 0.f + length;
+        const float inv_length = 1.f / len;
+        float prev = 0.f;
+        for (int i = 0; i < nb; i++) {
+            float dd = lens[i] * inv_length;
+            seg_pmf[c0 + i] = dd;
+            prev = i == 0 ? dd : dd + prev;
+            seg_cdf[c0 + i] = prev;
+        }
+        A.R.thick_leaf0[s] = maxr[ord[0]];
+        if (!all_finite) atomicOr(A.uerr, kUpdNonFinitePoints);
+        writable(S.shape_box)[s] = make_float4(x0, y0, x1, y1);
+        writable(S.shape_length)[s] = len;
+        if (!isfinite(len)) atomicOr(A.uerr, kUpdNonFiniteLength);
+    }
 }
 
 // Per group (scene.cpp:621-684, shape.h:123): canvas_to_shape, member visit order of the
@@ -1994,7 +2144,7 @@ int update_scene(SceneHost &sc, const dvg_scene_params &p, float filter_radius, 
     {
         DVG_PROF(st, "k_refit");
         k_refit_clear<<<1, 64, 0, st>>>(A.uerr);
-        k_refit_shapes<<<(D.ns + kBlock - 1) / kBlock, kBlock, 0, st>>>(A);
+        k_refit_shapes<<<(D.ns + kRefitWaves - 1) / kRefitWaves, kBlock, 0, st>>>(A);
         if (D.ng > 0) k_refit_groups<<<(D.ng + kBlock - 1) / kBlock, kBlock, 0, st>>>(A);
         k_refit_cdf<<<1, kBlock, 0, st>>>(A);
     }

@@ -44,11 +44,40 @@ def ties(size=64):
     return dict(canvas=(size, size), shapes=shapes, groups=groups, filter=('box', 0.5))
 
 
-CASES = sorted(scenes.CASES) + ['ties']
+def long_paths(size=96):
+    """Paths longer than one wave: a closed 200-cubic loop (distinct sort keys over
+    several lanes per segment), a 150-segment staircase whose risers share y-centres
+    (ties), and a 600-segment zigzag (past the rebuild's per-wave segment limit)."""
+    rng = np.random.default_rng(7)
+    t = np.linspace(0, 2 * np.pi, 601)[:-1]
+    rad = 30 + 6 * np.sin(5 * t) + rng.uniform(-1, 1, t.shape)
+    loop = np.stack([48 + rad * np.cos(t), 48 + rad * np.sin(t)], 1)
+    p_loop = pd.Path(num_control_points=torch.full((200,), 2, dtype=torch.int32), points=scenes.T(loop),
+                     is_closed=True, stroke_width=scenes.T(1.0))
+    stair = [[4.0, 90.0]]
+    for k in range(75):
+        x, y = stair[-1]
+        stair += [[x + 1.1, y], [x + 1.1, y - 1.0 * ((k % 5) == 0)]]
+    p_stair = pd.Path(num_control_points=torch.zeros(150, dtype=torch.int32), points=scenes.T(np.array(stair)),
+                      is_closed=False, stroke_width=scenes.T(0.8))
+    xs = np.linspace(2, 94, 601)
+    zig = np.stack([xs, 10 + 4 * (np.arange(601) % 2) + 0.01 * np.arange(601)], 1)
+    p_zig = pd.Path(num_control_points=torch.zeros(600, dtype=torch.int32), points=scenes.T(zig), is_closed=False,
+                    stroke_width=scenes.T(0.7))
+    shapes = [p_loop, p_stair, p_zig]
+    groups = [pd.ShapeGroup(shape_ids=torch.tensor([0]), fill_color=scenes.T([0.2, 0.4, 0.9, 0.7]),
+                            stroke_color=scenes.T([0, 0, 0, 1])),
+              pd.ShapeGroup(shape_ids=torch.tensor([1]), fill_color=None, stroke_color=scenes.T([1, 0, 0, 1])),
+              pd.ShapeGroup(shape_ids=torch.tensor([2]), fill_color=None, stroke_color=scenes.T([0, 0.6, 0, 1]))]
+    return dict(canvas=(size, size), shapes=shapes, groups=groups, filter=('box', 0.5))
+
+
+EXTRA = {'ties': ties, 'long_paths': long_paths}
+CASES = sorted(scenes.CASES) + sorted(EXTRA)
 
 
 def _flat(case):
-    return scenes.flat_case(ties() if case == 'ties' else case)
+    return scenes.flat_case(EXTRA[case]() if case in EXTRA else case)
 
 
 def _perturb(f, seed, rigid=False):
@@ -92,7 +121,7 @@ def _render(scene, w, h, d_img=None):
 def test_device_rebuild_equals_host_build(case):
     print('rebuild', case, flush=True)
     f1 = _flat(case)
-    f2 = _perturb(f1, 1, rigid=case == 'ties')
+    f2 = _perturb(f1, 1, rigid=case in ('ties', 'long_paths'))
     a = _lib.DeviceScene(f1, 0)
     tensors = {k: _dev(f2[k]) for k in FLOAT_KEYS}
     a.update(f2['filter_radius'], None, **tensors)
