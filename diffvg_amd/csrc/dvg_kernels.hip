@@ -584,17 +584,30 @@ DVG_DEV void edge_chunk(const DevScene &S, const Geom &G, int chunk, const EdgeS
         const int b0 = (chunk - chunk_off[tile]) * kBlock;
         int k = b0 + threadIdx.x;
         bool valid = k < cnt;
-        EdgeSample es;
-        if (valid) es = edges[sorted[off + k]];
-        V2 bp{es.px, es.py}, n{es.nx, es.ny};
-        V2 qi = bp - 1e-4f * n, qo = bp + 1e-4f * n;
-        V2 pin{qi.x * S.cw, qi.y * S.ch}, pout{qo.x * S.cw, qo.y * S.ch};
-        int bx = (int)(es.px * G.W), by = (int)(es.py * G.H);
-        V4 bgc = (valid && has_bg) ? load_bg(bg, by * G.W + bx) : V4{0, 0, 0, 0};
+        const int eidx = valid ? sorted[off + k] : 0;
+        // Only what shading needs stays live across the candidate loop (the two points,
+        // the target, the compositors); the rest of the sample is read again afterwards,
+        // which keeps the loop within the register budget of the occupancy target.
+        V2 pin{0, 0}, pout{0, 0};
+        int tgroup = -1, tshape = -1;
         Compositor ci, co;
-        ci.init(has_bg, bgc);
-        co.init(has_bg, bgc);
-        EdgeQuery qin{valid ? es.group : -1, valid ? es.shape : -1, false};
+        {
+            EdgeSample es0{};
+            if (valid) es0 = edges[eidx];
+            V2 bp{es0.px, es0.py}, n0{es0.nx, es0.ny};
+            V2 qi = bp - 1e-4f * n0, qo = bp + 1e-4f * n0;
+            pin = V2{qi.x * S.cw, qi.y * S.ch};
+            pout = V2{qo.x * S.cw, qo.y * S.ch};
+            if (valid) {
+                tgroup = es0.group;
+                tshape = es0.shape;
+            }
+            int bx0 = (int)(es0.px * G.W), by0 = (int)(es0.py * G.H);
+            V4 bg0 = (valid && has_bg) ? load_bg(bg, by0 * G.W + bx0) : V4{0, 0, 0, 0};
+            ci.init(has_bg, bg0);
+            co.init(has_bg, bg0);
+        }
+        EdgeQuery qin{tgroup, tshape, false};
         EdgeQuery qout = qin;
         bool hin = false, hout = false;
         for (int c0 = 0; c0 < S.ng; c0 += kCandCap) {
@@ -605,6 +618,12 @@ DVG_DEV void edge_chunk(const DevScene &S, const Geom &G, int chunk, const EdgeS
             }
             if (c0 + kCandCap < S.ng) __syncthreads();
         }
+        asm volatile("" ::: "memory");  // read the sample again, do not keep it live
+        EdgeSample es{};
+        if (valid) es = edges[eidx];
+        V2 n{es.nx, es.ny};
+        int bx = (int)(es.px * G.W), by = (int)(es.py * G.H);
+        V4 bgc = (valid && has_bg) ? load_bg(bg, by * G.W + bx) : V4{0, 0, 0, 0};
         // every lane stays to the end: the gradient scatter combines lanes of a wave
         bool active = valid;
         bool in_hit = false, out_hit = false;
