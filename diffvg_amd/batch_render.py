@@ -39,7 +39,7 @@ class BatchRenderFunction(torch.autograd.Function):
             a = args[k:k + n]
             spans.append((k, n))
             k += n
-            flat, scene = _device_scene_for(a, dev, check=False)
+            flat, scene, _ = _device_scene_for(a, dev, check=False)
             if flat.output_type != OutputType.color or flat.eval_positions.shape[0] != 0:
                 raise NotImplementedError('BatchRenderFunction renders colour images (OutputType.color)')
             if nsx * nsy > 256:

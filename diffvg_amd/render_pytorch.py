@@ -386,33 +386,41 @@ class _SceneTemplate:
             self._dev_tables[dev] = t
         return t
 
-    def device_params(self, args, dev):
-        """Float values of `args` as device arrays in the description's layout."""
+    def device_params(self, args, dev, grad=False):
+        """Float values of `args` as device arrays in the description's layout.  With
+        `grad` the arrays are built by differentiable ops, so that gradients of the
+        arrays flow back to the argument tensors (through torch's own cat/index
+        backward, one node per kind)."""
         f, T = self.flat, self._tables(dev)
-        with torch.no_grad():
+        with torch.set_grad_enabled(grad):
             def cat(kind):
                 mode = self.cat_mode[kind]
                 if mode is None:
                     return None
                 srcs = [args[i] for i in f.pos[kind]]
-                try:
-                    if mode == 'stack':
-                        out = torch.stack(srcs)
-                    elif mode == 'cat':
-                        out = torch.cat(srcs)
-                    elif mode == 'cat_flat':
-                        out = torch.cat(srcs).reshape(-1)
-                    else:
-                        out = torch.cat([t.reshape(-1) for t in srcs])
-                except RuntimeError:
-                    out = _cat_kind(srcs)
+                for attempt in range(2):
+                    try:
+                        if mode == 'stack':
+                            out = torch.stack(srcs)
+                        elif mode == 'cat':
+                            out = torch.cat(srcs)
+                        elif mode == 'cat_flat':
+                            out = torch.cat(srcs).reshape(-1)
+                        else:
+                            out = torch.cat([t.reshape(-1) for t in srcs])
+                        break
+                    except RuntimeError:
+                        if attempt:
+                            raise
+                        # tensors of one kind on several devices: move each first
+                        srcs = [t.to(dev) for t in srcs]
                 return out.to(device=dev, dtype=torch.float32).contiguous()
             params = cat('params')
             points = cat('points')
             thickness = None
             if T['thick_idx'] is not None:
-                thickness = torch.zeros(T['npts'], dtype=torch.float32, device=dev)
-                thickness[T['thick_idx']] = cat('thick')
+                thickness = torch.zeros(T['npts'], dtype=torch.float32, device=dev).index_put(
+                    (T['thick_idx'],), cat('thick'))
             sw = cat('sw')
             if sw is not None:
                 sw = torch.where(T['sw_zero'], torch.zeros((), device=dev), sw[T['sw_first']]).contiguous()
@@ -460,17 +468,19 @@ def _templates():
     return [t for ts in _TEMPLATES.values() for t in ts]
 
 
-def _device_scene_for(args, dev, check=True):
-    """(flat, scene) for a TILED render of `args`: a pooled scene of the same topology
-    rebuilt on the device when one exists, else a host build that starts a new pool.
-    With check=False the caller reports the rebuild's findings (scene.status())."""
+def _device_scene_for(args, dev, check=True, grad=False):
+    """(flat, scene, arrays) for a TILED render of `args`: a pooled scene of the same
+    topology rebuilt on the device when one exists, else a host build that starts a new
+    pool.  `arrays` are the float arrays in the description's layout (differentiable
+    with `grad`; None for a host build without `grad`).  With check=False the caller
+    reports the rebuild's findings (scene.status())."""
     dev_index = dev.index if dev.index is not None else -1
     bucket = _TEMPLATES.setdefault(len(args), [])
     for k, tpl in enumerate(bucket):
         if tpl.matches(args):
             if k:
                 bucket.insert(0, bucket.pop(k))
-            vals = tpl.device_params(args, dev)
+            vals = tpl.device_params(args, dev, grad)
             scene = tpl.acquire(dev_index)
             scene.update(float(args[-1]), _stream(), **vals)
             if check:
@@ -484,7 +494,7 @@ def _device_scene_for(args, dev, check=True):
             f = _Flat()
             f.__dict__.update(tpl.flat.__dict__)
             f.eval_positions = args[6]
-            return f, _Lease(tpl, scene)
+            return f, _Lease(tpl, scene), vals
     flat = flatten_args(args)
     tpl = _SceneTemplate(args, flat)
     if sum(len(ts) for ts in _TEMPLATES.values()) >= _MAX_TEMPLATES:
@@ -492,7 +502,8 @@ def _device_scene_for(args, dev, check=True):
         bucket = _TEMPLATES.setdefault(len(args), [])
     bucket.insert(0, tpl)
     scene = _lib.DeviceScene(flat.desc, dev_index, _lib.MODE_TILED)
-    return flat, _Lease(tpl, scene)
+    vals = tpl.device_params(args, dev, True) if grad else None
+    return flat, _Lease(tpl, scene), vals
 
 
 def check_rebuild(scene):
@@ -709,16 +720,19 @@ class RenderFunction(torch.autograd.Function):
         return args
 
     @staticmethod
-    def _prepare(width, height, background_image, args, output_type=None):
+    def _prepare(width, height, background_image, args, output_type=None, grad=False):
+        """-> (flat, scene, arrays); `arrays` (the float arrays in the description's
+        layout, differentiable) only with `grad`."""
         _check_gpu(read_findings=False)
         dev = get_device()
         start = time.time()
         if _render_mode == _lib.MODE_TILED:
-            flat, scene = _device_scene_for(args, dev)
+            flat, scene, vals = _device_scene_for(args, dev, grad=grad)
         else:
             # the comparator: a host-built scene per call, as the reference builds it
             flat = flatten_args(args)
             scene = _lib.DeviceScene(flat.desc, dev.index if dev.index is not None else -1, _render_mode)
+            vals = _SceneTemplate(args, flat).device_params(args, dev, True) if grad else None
         # Findings of earlier backward passes, read once this call's scene work is on the
         # stream: the host prepares this render while the previous backward's kernels
         # run, and the rebuild check above has already waited for them.
@@ -730,11 +744,77 @@ class RenderFunction(torch.autograd.Function):
             raise
         if print_timing:
             print('Scene construction, time: %.5f s' % (time.time() - start))
-        return flat, scene
+        return flat, scene, vals
+
+    @classmethod
+    def apply(cls, width, height, num_samples_x, num_samples_y, seed, background_image, *args):
+        """Renders the serialize_scene argument list `args` (render_pytorch.py:174-428).
+
+        The autograd node is taken over the scene's float arrays -- one tensor per kind
+        (shape parameters, points, thicknesses, stroke widths, paint parameters,
+        transforms) gathered from the argument tensors by differentiable concatenation --
+        and the filter radius, not over every argument tensor: each argument tensor that
+        requires a gradient receives the same .grad as a node over all of them would give
+        it, while the per-argument cost of the autograd machinery stays out of the call."""
+        flat, scene, vals = RenderFunction._prepare(width, height, background_image, args,
+                                                    grad=torch.is_grad_enabled())
+        radius = args[flat.num_args - 1]
+        if vals is None:
+            vals = {k: None for k in _KINDS}
+        meta = (flat, scene, width, height, num_samples_x, num_samples_y, seed)
+        return _RenderNode.apply(meta, background_image, radius if isinstance(radius, torch.Tensor) else None,
+                                 *[vals[k] for k in _KINDS])
 
     @staticmethod
-    def forward(ctx, width, height, num_samples_x, num_samples_y, seed, background_image, *args):
-        flat, scene = RenderFunction._prepare(width, height, background_image, args)
+    def render_grad(grad_img, width, height, num_samples_x, num_samples_y, seed, background_image, *args):
+        """Screen-space gradient of the image w.r.t. a translation of the scene
+        (render_pytorch.py:430-666)."""
+        if not grad_img.is_contiguous():
+            grad_img = grad_img.contiguous()
+        assert torch.isfinite(grad_img).all()
+        flat, scene, _ = RenderFunction._prepare(width, height, background_image, args)
+        dev = get_device()
+        if flat.output_type == OutputType.color:
+            assert grad_img.shape[2] == 4
+        else:
+            assert grad_img.shape[2] == 1
+        if background_image is not None:
+            background_image = background_image.to(dev)
+            if background_image.shape[2] == 3:
+                background_image = torch.cat((background_image, torch.ones(
+                    background_image.shape[0], background_image.shape[1], 1, device=dev)), dim=2)
+            background_image = background_image.contiguous().to(torch.float32)
+            assert background_image.shape[0] == grad_img.shape[0]
+            assert background_image.shape[1] == grad_img.shape[1]
+        grad_img = grad_img.to(dev).to(torch.float32).contiguous()
+        translation_grad_image = torch.zeros(height, width, 2, device=dev)
+        ev = flat.eval_positions
+        ev = ev.to(dev).to(torch.float32).contiguous() if ev.shape[0] > 0 else None
+        start = time.time()
+        scene.render(width, height, num_samples_x, num_samples_y, seed, background=background_image,
+                     d_render_image=grad_img if flat.output_type == OutputType.color else None,
+                     d_render_sdf=grad_img if flat.output_type == OutputType.sdf else None,
+                     d_translation=translation_grad_image, use_prefiltering=flat.use_prefiltering,
+                     eval_positions=ev, num_eval_positions=0 if ev is None else ev.shape[0], stream=_stream())
+        scene.status()
+        if print_timing:
+            torch.cuda.synchronize()
+            print('Gradient pass, time: %.5f s' % (time.time() - start))
+        assert torch.isfinite(translation_grad_image).all()
+        return translation_grad_image
+
+
+_KINDS = ('shape_params', 'points', 'thickness', 'stroke_width', 'color_params', 'shape_to_canvas')
+
+
+class _RenderNode(torch.autograd.Function):
+    """The render as one autograd node: inputs are the background image, the filter
+    radius and the scene's float arrays by kind (RenderFunction.apply builds them);
+    the backward returns slices of the scene's gradient block for them."""
+
+    @staticmethod
+    def forward(ctx, meta, background_image, filter_radius, *kinds):
+        flat, scene, width, height, num_samples_x, num_samples_y, seed = meta
         dev = get_device()
         eval_positions = flat.eval_positions
         if flat.output_type == OutputType.color:
@@ -771,7 +851,6 @@ class RenderFunction(torch.autograd.Function):
             print('Forward pass, time: %.5f s' % (time.time() - start))
         ctx.scene = scene
         ctx.flat = flat
-        ctx.args = args
         ctx.background_image = background_image
         ctx.width = width
         ctx.height = height
@@ -779,45 +858,9 @@ class RenderFunction(torch.autograd.Function):
         ctx.num_samples_y = num_samples_y
         ctx.seed = seed
         ctx.eval_positions = ev
+        ctx.radius_shape = None if filter_radius is None else filter_radius.shape
+        ctx.kind_shapes = [None if k is None else k.shape for k in kinds]
         return rendered_image
-
-    @staticmethod
-    def render_grad(grad_img, width, height, num_samples_x, num_samples_y, seed, background_image, *args):
-        """Screen-space gradient of the image w.r.t. a translation of the scene
-        (render_pytorch.py:430-666)."""
-        if not grad_img.is_contiguous():
-            grad_img = grad_img.contiguous()
-        assert torch.isfinite(grad_img).all()
-        flat, scene = RenderFunction._prepare(width, height, background_image, args)
-        dev = get_device()
-        if flat.output_type == OutputType.color:
-            assert grad_img.shape[2] == 4
-        else:
-            assert grad_img.shape[2] == 1
-        if background_image is not None:
-            background_image = background_image.to(dev)
-            if background_image.shape[2] == 3:
-                background_image = torch.cat((background_image, torch.ones(
-                    background_image.shape[0], background_image.shape[1], 1, device=dev)), dim=2)
-            background_image = background_image.contiguous().to(torch.float32)
-            assert background_image.shape[0] == grad_img.shape[0]
-            assert background_image.shape[1] == grad_img.shape[1]
-        grad_img = grad_img.to(dev).to(torch.float32).contiguous()
-        translation_grad_image = torch.zeros(height, width, 2, device=dev)
-        ev = flat.eval_positions
-        ev = ev.to(dev).to(torch.float32).contiguous() if ev.shape[0] > 0 else None
-        start = time.time()
-        scene.render(width, height, num_samples_x, num_samples_y, seed, background=background_image,
-                     d_render_image=grad_img if flat.output_type == OutputType.color else None,
-                     d_render_sdf=grad_img if flat.output_type == OutputType.sdf else None,
-                     d_translation=translation_grad_image, use_prefiltering=flat.use_prefiltering,
-                     eval_positions=ev, num_eval_positions=0 if ev is None else ev.shape[0], stream=_stream())
-        scene.status()
-        if print_timing:
-            torch.cuda.synchronize()
-            print('Gradient pass, time: %.5f s' % (time.time() - start))
-        assert torch.isfinite(translation_grad_image).all()
-        return translation_grad_image
 
     @staticmethod
     def backward(ctx, grad_img):
@@ -843,9 +886,23 @@ class RenderFunction(torch.autograd.Function):
                      num_eval_positions=0 if ctx.eval_positions is None else ctx.eval_positions.shape[0],
                      stream=_stream())
         block, layout = scene.grads_device(grad_img.device, _stream())
-        d_scene = _grad_tensors(flat, block, layout, ctx.args, flat.num_args)
+        need = ctx.needs_input_grad
+        d_radius = None
+        if ctx.radius_shape is not None and need[2]:
+            o = layout['filter_radius']
+            d_radius = block[o:o + 1].view(ctx.radius_shape)
+        d_kinds = []
+        for j, (name, shape) in enumerate(zip(_KINDS, ctx.kind_shapes)):
+            if shape is None or not need[3 + j]:
+                d_kinds.append(None)
+                continue
+            o = layout[name]
+            n = 1
+            for d in shape:
+                n *= d
+            d_kinds.append(block[o:o + n].view(shape))
         if print_timing:
             scene.status()
             print('Backward pass, time: %.5f s' % (time.time() - start))
         _PENDING.append((scene, finite))
-        return tuple([None, None, None, None, None, d_background_image] + d_scene)
+        return (None, d_background_image, d_radius) + tuple(d_kinds)

@@ -288,9 +288,15 @@ def trim_grads(flat, g):
 
 
 class OracleRunner:
-    """Runs the oracle on one flattened scene."""
+    """Runs the oracle on one flattened scene.
 
-    def __init__(self, flat, threads=0):
+    One thread by default: with several, the reference's CPU schedule splats samples with
+    float atomics in arrival order, so a pixel's sum (64 terms under a Gaussian filter of
+    radius 1.5 at 2x2 samples) rounds differently from run to run by up to ~1e-5, the
+    size of the image tolerance the GPU results are held to.  With one thread the order
+    is the sample order and the oracle's image is the same in every run."""
+
+    def __init__(self, flat, threads=1):
         self.lib = load_oracle()
         self.flat = flat
         self.lib.orc_set_num_threads(threads)
