@@ -164,7 +164,7 @@ def bench_batched(a):
     import diffvg_amd as pydiffvg
     from diffvg_amd import workloads, _lib
     from diffvg_amd.batch import Batch
-    from diffvg_amd.render_pytorch import flatten_args, _grad_tensors
+    from diffvg_amd.render_pytorch import flatten_args
 
     dev = torch.device('cuda:0')
     pydiffvg.set_use_gpu(True)

@@ -17,8 +17,8 @@ import torch
 from . import _lib
 from .batch import Batch
 from .device import get_device
-from .render_pytorch import (OutputType, _PENDING, _Lease, _check_gpu, _device_scene_for, _grad_tensors,
-                             _stream, check_rebuild, synchronize)
+from .render_pytorch import (_KINDS, OutputType, _PENDING, _Lease, _check_gpu, _device_scene_for, _stream,
+                             check_rebuild, synchronize)
 
 __all__ = ['BatchRenderFunction', 'render_batch']
 
@@ -26,33 +26,35 @@ __all__ = ['BatchRenderFunction', 'render_batch']
 class BatchRenderFunction(torch.autograd.Function):
     """apply(spec, *args): spec is a tuple of (width, height, num_samples_x,
     num_samples_y, seed, num_args) per scene; args are the scenes' serialize_scene lists
-    concatenated.  Returns one image tensor per scene."""
+    concatenated.  Returns one image tensor per scene.
 
-    @staticmethod
-    def forward(ctx, spec, *args):
+    As RenderFunction.apply, the autograd node is taken over each scene's float arrays
+    by kind and its filter radius, gathered from the argument tensors by differentiable
+    concatenation, not over every argument tensor."""
+
+    @classmethod
+    def apply(cls, spec, *args):
         _check_gpu(read_findings=False)
         dev = get_device()
-        st = _stream()
-        scenes, flats, images, jobs, spans = [], [], [], [], []
+        grad = torch.is_grad_enabled()
+        scenes, flats, inputs = [], [], []
         k = 0
-        for (w, h, nsx, nsy, seed, n) in spec:
-            a = args[k:k + n]
-            spans.append((k, n))
-            k += n
-            flat, scene, _ = _device_scene_for(a, dev, check=False)
-            if flat.output_type != OutputType.color or flat.eval_positions.shape[0] != 0:
-                raise NotImplementedError('BatchRenderFunction renders colour images (OutputType.color)')
-            if nsx * nsy > 256:
-                raise ValueError('a batched render takes at most 256 samples per pixel')
-            img = torch.zeros(h, w, 4, device=dev)
-            scenes.append(scene)
-            flats.append(flat)
-            images.append(img)
-            jobs.append((scene, dict(width=w, height=h, nsx=nsx, nsy=nsy, seed=seed, render_image=img,
-                                     use_prefiltering=flat.use_prefiltering)))
-        if k != len(args):
-            raise ValueError('spec does not account for every argument')
         try:
+            for (w, h, nsx, nsy, seed, n) in spec:
+                a = args[k:k + n]
+                k += n
+                flat, scene, vals = _device_scene_for(a, dev, check=False, grad=grad)
+                scenes.append(scene)
+                flats.append(flat)
+                if flat.output_type != OutputType.color or flat.eval_positions.shape[0] != 0:
+                    raise NotImplementedError('BatchRenderFunction renders colour images (OutputType.color)')
+                if nsx * nsy > 256:
+                    raise ValueError('a batched render takes at most 256 samples per pixel')
+                radius = a[flat.num_args - 1]
+                inputs.append(radius if isinstance(radius, torch.Tensor) else None)
+                inputs += [None] * len(_KINDS) if vals is None else [vals[kd] for kd in _KINDS]
+            if k != len(args):
+                raise ValueError('spec does not account for every argument')
             for sc in scenes:
                 check_rebuild(sc)
             # findings of earlier backward passes, read once this call's scene work is on
@@ -63,10 +65,28 @@ class BatchRenderFunction(torch.autograd.Function):
                 if isinstance(sc, _Lease):
                     sc.release()
             raise
+        return _BatchNode.apply((tuple(spec), scenes, flats), *inputs)
+
+
+class _BatchNode(torch.autograd.Function):
+    """One batched forward and backward; inputs per scene: filter radius, then the float
+    arrays in _KINDS order."""
+
+    @staticmethod
+    def forward(ctx, meta, *inputs):
+        spec, scenes, flats = meta
+        dev = get_device()
+        st = _stream()
+        images, jobs = [], []
+        for (w, h, nsx, nsy, seed, n), scene, flat in zip(spec, scenes, flats):
+            img = torch.zeros(h, w, 4, device=dev)
+            images.append(img)
+            jobs.append((scene, dict(width=w, height=h, nsx=nsx, nsy=nsy, seed=seed, render_image=img,
+                                     use_prefiltering=flat.use_prefiltering)))
         fwd = Batch(jobs, zero_grads=False)
         fwd.render(st)
-        ctx.spec, ctx.spans, ctx.scenes, ctx.flats = spec, spans, scenes, flats
-        ctx.args = args
+        ctx.spec, ctx.scenes, ctx.flats = spec, scenes, flats
+        ctx.shapes = [None if t is None else t.shape for t in inputs]
         ctx.fwd = fwd
         return tuple(images)
 
@@ -87,13 +107,24 @@ class BatchRenderFunction(torch.autograd.Function):
                                   use_prefiltering=flat.use_prefiltering)))
         bwd = Batch(jobs, zero_grads=True)
         bwd.render(st)
-        out = [None] * len(ctx.args)
-        for sc, flat, (k, n) in zip(ctx.scenes, ctx.flats, ctx.spans):
+        need = ctx.needs_input_grad
+        per = 1 + len(_KINDS)
+        out = [None] * len(ctx.shapes)
+        for i, sc in enumerate(ctx.scenes):
+            base = per * i
+            if not any(need[1 + base + j] for j in range(per)):
+                continue
             block, layout = sc.grads_device(dev, st)
-            for j, t in enumerate(_grad_tensors(flat, block, layout, ctx.args[k:k + n], flat.num_args)):
-                if t is not None:
-                    out[k + j] = t
-        # as RenderFunction.backward: returns while the kernels run; the batch's findings
+            for j, name in enumerate(('filter_radius',) + _KINDS):
+                shape = ctx.shapes[base + j]
+                if shape is None or not need[1 + base + j]:
+                    continue
+                o = layout[name]
+                cnt = 1
+                for d in shape:
+                    cnt *= d
+                out[base + j] = block[o:o + cnt].view(shape)
+        # as RenderFunction's backward: returns while the kernels run; the batch's findings
         # (and a non-finite grad_img) are read at the next entry or at synchronize()
         finite = torch.stack([g.isfinite().all() for g in keep]).all()
         _PENDING.append((_BatchCheck(bwd, ctx.scenes), finite))

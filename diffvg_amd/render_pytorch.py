@@ -557,98 +557,6 @@ def ctypes_stream(s):
     return s.cuda_stream
 
 
-def _grad_plan(flat):
-    """Groups of grad_map entries whose views one torch call can cut out of the block:
-    entries of one kind that tile a range and share a shape pattern (('rows', k): (n, k)
-    arrays such as points and 3x3 transforms, 'scalars': 0-d, 'vectors': 1-d); the rest
-    are cut one by one."""
-    plan = getattr(flat, 'grad_plan', None)
-    if plan is not None:
-        return plan
-    plan = []
-    run = []
-
-    def pattern(e):
-        shape = e[4]
-        if len(shape) == 2 and shape[1] > 0:
-            return ('rows', shape[1])
-        if len(shape) == 0 and e[3] == 1:
-            return 'scalars'
-        if len(shape) == 1:
-            return 'vectors'
-        return None
-
-    def close():
-        if run:
-            plan.append((pattern(run[0]), list(run)))
-            run.clear()
-
-    kinds = {}
-    for e in flat.grad_map:
-        kinds.setdefault(e[1], []).append(e)
-    for entries in kinds.values():
-        for e in sorted(entries, key=lambda e: e[2]):
-            pat = pattern(e)
-            if run and (pat is None or pat != pattern(run[-1]) or e[2] != run[-1][2] + run[-1][3]):
-                close()
-            if pat is None:
-                plan.append((None, [e]))
-            else:
-                run.append(e)
-        close()
-    flat.grad_plan = plan
-    return plan
-
-
-def _grad_tensors(flat, block, layout, args, num_args):
-    """Gradient list in argument order (render_pytorch.py:713-868): views of one
-    gradient block for the inputs that require a gradient (autograd drops the others),
-    taken from one copy of the block per other device or dtype that inputs live on."""
-    d = [None] * num_args
-    dev = block.device
-    f32 = torch.float32
-    other = {}
-
-    def source(src):
-        if src.dtype == f32 and src.device == dev:
-            return block
-        key = (src.device, src.dtype)
-        b = other.get(key)
-        if b is None:
-            b = other[key] = block.to(device=src.device, dtype=src.dtype)
-        return b
-
-    for pat, entries in _grad_plan(flat):
-        need = [e for e in entries if args[e[0]].requires_grad]
-        if not need:
-            continue
-        if pat is None or len(need) < 8:
-            for (ai, kind, off, cnt, shape) in need:
-                b = source(args[ai])
-                base = layout[kind] + off
-                d[ai] = b[base:base + cnt].view(shape)
-            continue
-        b = source(args[need[0][0]])
-        if any(source(args[e[0]]) is not b for e in need):
-            for (ai, kind, off, cnt, shape) in need:
-                base = layout[kind] + off
-                d[ai] = source(args[ai])[base:base + cnt].view(shape)
-            continue
-        first, last = entries[0], entries[-1]
-        lo = layout[first[1]] + first[2]
-        seg = b[lo:lo + last[2] + last[3] - first[2]]
-        if pat[0] == 'rows':
-            pieces = seg.view(-1, pat[1]).split([e[4][0] for e in entries])
-        elif pat == 'scalars':
-            pieces = seg.unbind(0)
-        else:
-            pieces = seg.split([e[3] for e in entries])
-        for e, t in zip(entries, pieces):
-            if args[e[0]].requires_grad:
-                d[e[0]] = t
-    return d
-
-
 class RenderFunction(torch.autograd.Function):
     """The PyTorch interface of the render path."""
 
