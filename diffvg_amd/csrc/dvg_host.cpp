@@ -621,9 +621,12 @@ void free_scene(SceneHost &sc) {
     cache_free(sc.device, sc.block, sc.block_granted);
     cache_free(sc.device, sc.grad, sc.grad_granted);
     cache_free(sc.device, sc.err, 2 * sizeof(int));
+    if (sc.frag) cache_free(sc.device, sc.frag, sc.frag_granted);
     sc.block = nullptr;
     sc.grad = nullptr;
     sc.err = nullptr;
+    sc.frag = nullptr;
+    sc.frag_valid = false;
     if (prev >= 0) (void)hipSetDevice(prev);
 }
 

@@ -79,6 +79,16 @@ struct SceneHost {
     size_t top_offset = 0, top_bytes = 0;          // top-level BVH in the block
     size_t scratch_offset[2] = {0, 0}, scratch_bytes[2] = {0, 0};  // rebuild sort scratch in the block
     bool top_stale = false;       // the top-level BVH (PLAIN schedule) predates a device update
+    uint64_t version = 0;         // advanced by every device update of the parameters
+    // Fragment lists of the last non-prefiltered colour forward of the TILED schedule:
+    // [kFragSlots][nsamp] group ids (2 group + is_stroke) in compositing order, then
+    // [nsamp] fragment counts.  The backward of the same render reads them instead of
+    // shading its samples again.
+    int *frag = nullptr;
+    size_t frag_granted = 0;
+    bool frag_valid = false;
+    uint64_t frag_key[4] = {0, 0, 0, 0};
+    hipStream_t frag_stream = nullptr;
     DevScene dev{};
 };
 

@@ -34,6 +34,10 @@ namespace dvg {
 
 constexpr int kBlock = 256;
 constexpr int kCandCap = 2048;
+// fragment lists kept from a colour forward for its backward: up to kFragSlots fragments
+// per sample (a sample with more is shaded again), at most kFragCacheBytes per scene
+constexpr int kFragSlots = 8;
+constexpr size_t kFragCacheBytes = size_t(256) << 20;
 // Waves per SIMD each heavy shading kernel is compiled for, i.e. its register budget
 // (512 / waves VGPRs per lane; beyond it the compiler spills to scratch).  Chosen by
 // measurement on the 300-path and 2,048-stroke scenes: these kernels wait on memory and
@@ -251,6 +255,43 @@ DVG_DEV void shade_list_record(const DevScene &S, V2 pt, const int *list, int n,
     }
 }
 
+// shade_list that also writes the group ids of the fragments it composites (2 g +
+// is_stroke, at most kFragSlots of them) to ids[slot * nsamp + idx].
+DVG_DEV void shade_list_ids(const DevScene &S, V2 pt, const int *list, int n, Compositor &comp, int *ids,
+                            size_t nsamp, int idx) {
+    for (int i = 0; i < n; i++) {
+        int g = list[i];
+        if (S.ng > 1 && !in_box(S.g_box[g], pt, S.g_maxr[g])) continue;
+        M3 c2s = m3_load(S.c2s + 9 * g);
+        V2 local = xform_pt(c2s, pt);
+        int st = S.stroke_type[g];
+        if (st >= 0 && within_distance_local(S, g, local, nullptr)) {
+            if (comp.nf < kFragSlots) ids[(size_t)comp.nf * nsamp + idx] = 2 * g + 1;
+            comp.add(sample_paint(S, st, S.stroke_off[g], S.stroke_ns[g], pt), g, -1);
+        }
+        int ft = S.fill_type[g];
+        if (ft >= 0 && is_inside_local(S, g, local, nullptr)) {
+            if (comp.nf < kFragSlots) ids[(size_t)comp.nf * nsamp + idx] = 2 * g;
+            comp.add(sample_paint(S, ft, S.fill_off[g], S.fill_ns[g], pt), g, -1);
+        }
+    }
+}
+
+// The fragment records shade_list_record makes, from the group ids a forward kept: the
+// same paints at the same point composited in the same order.
+DVG_DEV void replay_frags(const DevScene &S, V2 pt, const int *ids, size_t nsamp, int idx, int nf, Compositor &comp,
+                          FragRec *frags) {
+    for (int j = 0; j < nf; j++) {
+        const int gid = ids[(size_t)j * nsamp + idx];
+        const int g = gid >> 1;
+        const bool stroke = (gid & 1) != 0;
+        V4 c = stroke ? sample_paint(S, S.stroke_type[g], S.stroke_off[g], S.stroke_ns[g], pt)
+                      : sample_paint(S, S.fill_type[g], S.fill_off[g], S.fill_ns[g], pt);
+        frags[comp.nf] = FragRec{V3{c.x, c.y, c.z}, c.w, comp.color, comp.alpha, gid};
+        comp.add(c, g, -1);
+    }
+}
+
 DVG_DEV V4 load_bg(const float *bg, int pix) {
     float4 b = reinterpret_cast<const float4 *>(bg)[pix];
     return V4{b.x, b.y, b.z, b.w};
@@ -283,7 +324,8 @@ __global__ __launch_bounds__(kBlock) void k_weight(DevScene S, Geom G, float *we
 }
 
 template <int PRE>
-DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float *bg, float4 *colors, int *err) {
+DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float *bg, float4 *colors, int *err,
+                        int *frag = nullptr) {
     constexpr int prefilter = PRE;
     __shared__ int s_list[kCandCap];
     __shared__ int s_w[kBlock / 64];
@@ -303,6 +345,8 @@ DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float 
         if (valid) {
             if constexpr (PRE) {
                 shade_list_prefiltered<false>(S, pt, s_list, n, comp, nullptr, overflow);
+            } else if (frag) {
+                shade_list_ids(S, pt, s_list, n, comp, frag, (size_t)G.W * G.H * G.spp, idx);
             } else {
                 bool th = false;
                 shade_list(S, pt, s_list, n, comp, nullptr, th);
@@ -315,12 +359,14 @@ DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float 
     if (valid) {
         V4 c = comp.result(has_bg, bgc);
         colors[idx] = make_float4(c.x, c.y, c.z, c.w);
+        if (!PRE && frag) frag[(size_t)kFragSlots * G.W * G.H * G.spp + idx] = comp.nf;
     }
     if (overflow == 2) atomicOr(err, 2);
 }
 template <int PRE>
-__global__ __launch_bounds__(kBlock, DVG_WAVES_COLOR) void k_color(DevScene S, Geom G, const float *bg, float4 *colors, int *err) {
-    color_tile<PRE>(S, G, blockIdx.x, bg, colors, err);
+__global__ __launch_bounds__(kBlock, DVG_WAVES_COLOR) void k_color(DevScene S, Geom G, const float *bg, float4 *colors, int *err,
+                                                  int *frag) {
+    color_tile<PRE>(S, G, blockIdx.x, bg, colors, err, frag);
 }
 
 DVG_DEV void resolve_pixel(const DevScene &S, const Geom &G, int pix, int prefilter, const float *weight,
@@ -358,7 +404,8 @@ __global__ __launch_bounds__(kBlock) void k_resolve(DevScene S, Geom G, int pref
 
 template <int PRE>
 DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const float *bg, const float *weight,
-                            const float *d_img, float *d_bg, float *d_translation, int *err) {
+                            const float *d_img, float *d_bg, float *d_translation, int *err,
+                            const int *frag = nullptr) {
     constexpr int prefilter = PRE;
     __shared__ int s_list[kCandCap];
     __shared__ int s_w[kBlock / 64];
@@ -381,15 +428,25 @@ DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const fl
     PFragRec pfrags[PRE ? kMaxPFrags : 1];
     int overflow = 0;
     float4 tb = tile_box(S, G, tx, ty, 0.f);
-    for (int c0 = 0; c0 < S.ng; c0 += kCandCap) {
-        int n = build_candidates(S, tb, c0, min(S.ng, c0 + kCandCap), s_list, s_w);
-        if (valid) {
-            if constexpr (PRE)
-                shade_list_prefiltered<true>(S, pt, s_list, n, comp, pfrags, overflow);
-            else
-                shade_list_record(S, pt, s_list, n, comp, frags, overflow);
+    // a sample whose forward kept its fragment list replays it; the others shade
+    const size_t nsamp = (size_t)G.W * G.H * G.spp;
+    int kept = -1;
+    if (!PRE && frag && valid) kept = frag[(size_t)kFragSlots * nsamp + idx];
+    const bool replay = kept >= 0 && kept <= kFragSlots;
+    if (__syncthreads_or(valid && !replay)) {
+        for (int c0 = 0; c0 < S.ng; c0 += kCandCap) {
+            int n = build_candidates(S, tb, c0, min(S.ng, c0 + kCandCap), s_list, s_w);
+            if (valid && !replay) {
+                if constexpr (PRE)
+                    shade_list_prefiltered<true>(S, pt, s_list, n, comp, pfrags, overflow);
+                else
+                    shade_list_record(S, pt, s_list, n, comp, frags, overflow);
+            }
+            if (c0 + kCandCap < S.ng) __syncthreads();
         }
-        if (c0 + kCandCap < S.ng) __syncthreads();
+    }
+    if constexpr (!PRE) {
+        if (valid && replay) replay_frags(S, pt, frag, nsamp, idx, kept, comp, frags);
     }
     float d_radius = 0.f;
     int kind = -1;
@@ -477,8 +534,8 @@ DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const fl
 template <int PRE>
 __global__ __launch_bounds__(kBlock, DVG_WAVES_COLOR_BWD) void k_color_bwd(DevScene S, Geom G, const float *bg, const float *weight,
                                                       const float *d_img, float *d_bg, float *d_translation,
-                                                      int *err) {
-    color_bwd_tile<PRE>(S, G, blockIdx.x, bg, weight, d_img, d_bg, d_translation, err);
+                                                      int *err, const int *frag) {
+    color_bwd_tile<PRE>(S, G, blockIdx.x, bg, weight, d_img, d_bg, d_translation, err, frag);
 }
 
 DVG_DEV void edge_sample_one(const DevScene &S, const Geom &G, int idx, int count, EdgeSample *edges,
@@ -1257,13 +1314,39 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
             DVG_PROF(st, "k_weight");
             k_weight<<<(unsigned)((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(S, G, ws.weight);
         }
+        // The forward keeps each sample's fragment list (group ids in compositing order)
+        // in the scene when it fits the budget; a backward of the same render (same seed,
+        // sizes and parameter version, enqueued on the same stream) replays it.
+        const uint64_t key[4] = {a.seed, (uint64_t)(uint32_t)G.W | ((uint64_t)(uint32_t)G.H << 32),
+                                 (uint64_t)(uint32_t)G.nsx | ((uint64_t)(uint32_t)G.nsy << 32), sc.version};
         if (a.render_image) {
+            int *frag = nullptr;
+            sc.frag_valid = false;
+            const size_t frag_bytes = (size_t)(kFragSlots + 1) * nsamp * sizeof(int);
+            if (!pre && frag_bytes <= kFragCacheBytes) {
+                // a backward on another stream may still read the lists being replaced
+                if (sc.frag && sc.frag_stream != st) DVG_CHECK(hipStreamSynchronize(sc.frag_stream));
+                if (!sc.frag || sc.frag_granted < frag_bytes) {
+                    if (sc.frag) {
+                        DVG_CHECK(hipStreamSynchronize(st));
+                        cache_free(sc.device, sc.frag, sc.frag_granted);
+                        sc.frag = nullptr;
+                    }
+                    sc.frag = (int *)cache_alloc(sc.device, frag_bytes, &sc.frag_granted);
+                }
+                frag = sc.frag;
+            }
             {
                 DVG_PROF(st, "k_color");
                 if (pre)
-                    k_color<1><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.colors, sc.err);
+                    k_color<1><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.colors, sc.err, nullptr);
                 else
-                    k_color<0><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.colors, sc.err);
+                    k_color<0><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.colors, sc.err, frag);
+            }
+            if (frag) {
+                sc.frag_valid = true;
+                for (int k = 0; k < 4; k++) sc.frag_key[k] = key[k];
+                sc.frag_stream = st;
             }
             {
                 DVG_PROF(st, "k_resolve");
@@ -1274,14 +1357,16 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
         if (a.d_render_image) {
             {
                 DVG_PROF(st, "k_color_bwd");
+                bool kept = !pre && sc.frag_valid && sc.frag_stream == st;
+                for (int k = 0; k < 4; k++) kept = kept && sc.frag_key[k] == key[k];
                 if (pre)
                     k_color_bwd<1><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.weight,
                                                               a.d_render_image, a.d_background_image,
-                                                              a.d_translation, sc.err);
+                                                              a.d_translation, sc.err, nullptr);
                 else
                     k_color_bwd<0><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.weight,
                                                               a.d_render_image, a.d_background_image,
-                                                              a.d_translation, sc.err);
+                                                              a.d_translation, sc.err, kept ? sc.frag : nullptr);
             }
             if (!pre) {
                 DVG_CHECK(hipMemsetAsync(ws.tile_count, 0, sizeof(int) * ntiles, st));
@@ -2146,6 +2231,8 @@ __global__ __launch_bounds__(kBlock) void k_refit_cdf(RefitArgs A) {
 }
 
 int update_scene(SceneHost &sc, const dvg_scene_params &p, float filter_radius, hipStream_t st) {
+    sc.version++;
+    sc.frag_valid = false;
     if (!(filter_radius > 0) || !std::isfinite(filter_radius)) return set_error("filter radius must be positive");
     sc.dev.filter_radius = filter_radius;
     if (!sc.device_set || sc.num_shapes == 0) return 0;

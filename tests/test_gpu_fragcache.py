@@ -1,0 +1,87 @@
+"""A colour backward that follows its forward (same seed, sizes and parameters) replays
+the fragment lists the forward kept instead of shading its samples again.  Its results
+must be those of a backward that shades: compared here against a fresh scene's backward
+(no forward before it), with scenes whose samples have more fragments than are kept
+(those samples shade again), and with forwards whose lists must not be used (another
+seed, parameters changed by a device rebuild)."""
+import numpy as np
+import pytest
+import torch
+
+import diffvg_amd as pd
+from gpudrive import GpuRunner
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ('shape_params', 'points', 'thickness', 'stroke_width', 'color_params', 'shape_to_canvas', 'filter_radius')
+
+
+def layers(size=48, n=14):
+    """n overlapping translucent circles: the centre samples have n fragments, more than
+    a forward keeps."""
+    shapes = [pd.Circle(radius=scenes.T(14.0 - 0.5 * k), center=scenes.T([24.0 + 0.3 * k, 24.0 - 0.2 * k]))
+              for k in range(n)]
+    groups = [pd.ShapeGroup(shape_ids=torch.tensor([k]), fill_color=scenes.T([0.1 * (k % 7), 0.5, 0.9 - 0.05 * k, 0.3]),
+                            stroke_color=scenes.T([0, 0, 0, 0.5]) if k % 3 == 0 else None)
+              for k in range(n)]
+    for k, s in enumerate(shapes):
+        s.stroke_width = scenes.T(0.8)
+    return dict(canvas=(size, size), shapes=shapes, groups=groups, filter=('box', 0.5))
+
+
+def _flat(case):
+    return scenes.flat_case(layers() if case == 'layers' else case)
+
+
+def _same(a, b, label):
+    assert np.array_equal(a['d_translation'], b['d_translation']) or \
+        np.abs(a['d_translation'] - b['d_translation']).max() <= 1e-3 * max(np.abs(b['d_translation']).max(), 1e-8), label
+    for k in KEYS:
+        x, y = a['grads'][k], b['grads'][k]
+        if y.size == 0:
+            continue
+        assert np.abs(x - y).max() <= 1e-3 * max(np.abs(y).max(), 1e-8), (label, k)
+
+
+@pytest.mark.parametrize('case', ['overlap_eo', 'gradients', 'strokes', 'transformed', 'layers'])
+def test_replayed_backward_equals_shading_backward(case):
+    print('fragcache', case, flush=True)
+    f = _flat(case)
+    w, h = f['canvas_width'], f['canvas_height']
+    d = np.random.default_rng(3).uniform(-1, 1, (h, w, 4)).astype(np.float32)
+    a = GpuRunner(f)
+    fwd = a.render(w, h, 2, 2, 5)
+    kept = a.render(w, h, 2, 2, 5, d_image=d, translation=True)
+    fresh = GpuRunner(f).render(w, h, 2, 2, 5, d_image=d, translation=True)
+    _same(kept, fresh, case)
+    # a second forward of the same render gives the same image
+    assert np.array_equal(a.render(w, h, 2, 2, 5)['image'], fwd['image'])
+
+
+def test_lists_of_another_seed_are_not_used():
+    print('fragcache seed', flush=True)
+    f = _flat('layers')
+    w, h = f['canvas_width'], f['canvas_height']
+    d = np.random.default_rng(4).uniform(-1, 1, (h, w, 4)).astype(np.float32)
+    a = GpuRunner(f)
+    a.render(w, h, 2, 2, 1)
+    other = a.render(w, h, 2, 2, 2, d_image=d, translation=True)
+    fresh = GpuRunner(f).render(w, h, 2, 2, 2, d_image=d, translation=True)
+    _same(other, fresh, 'seed')
+
+
+def test_lists_are_dropped_by_a_rebuild():
+    print('fragcache rebuild', flush=True)
+    f1 = _flat('layers')
+    w, h = f1['canvas_width'], f1['canvas_height']
+    f2 = dict(f1)
+    f2['shape_params'] = (f1['shape_params'] * 1.15).astype(np.float32)
+    d = np.random.default_rng(5).uniform(-1, 1, (h, w, 4)).astype(np.float32)
+    a = GpuRunner(f1)
+    a.render(w, h, 2, 2, 7)
+    a.scene.update(f2['filter_radius'], None, shape_params=torch.from_numpy(f2['shape_params']).cuda())
+    a.scene.status()
+    after = a.render(w, h, 2, 2, 7, d_image=d, translation=True)
+    fresh = GpuRunner(f2).render(w, h, 2, 2, 7, d_image=d, translation=True)
+    _same(after, fresh, 'rebuild')
