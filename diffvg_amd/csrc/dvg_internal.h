@@ -81,10 +81,10 @@ struct SceneHost {
     bool top_stale = false;       // the top-level BVH (PLAIN schedule) predates a device update
     uint64_t version = 0;         // advanced by every device update of the parameters
     // Fragment lists of the last non-prefiltered colour forward of the TILED schedule:
-    // [kFragSlots][nsamp] group ids (2 group + is_stroke) in compositing order, then
-    // [nsamp] fragment counts.  The backward of the same render reads them instead of
-    // shading its samples again.
-    int *frag = nullptr;
+    // [kFragSlots][nsamp] 16-bit group ids (2 group + is_stroke) in compositing order,
+    // then [nsamp] fragment counts.  The backward of the same render reads them instead
+    // of shading its samples again.
+    uint16_t *frag = nullptr;
     size_t frag_granted = 0;
     bool frag_valid = false;
     uint64_t frag_key[4] = {0, 0, 0, 0};

@@ -36,8 +36,9 @@ constexpr int kBlock = 256;
 constexpr int kCandCap = 2048;
 // fragment lists kept from a colour forward for its backward: up to kFragSlots fragments
 // per sample (a sample with more is shaded again), at most kFragCacheBytes per scene
-constexpr int kFragSlots = 8;
-constexpr size_t kFragCacheBytes = size_t(256) << 20;
+constexpr int kFragSlots = 16;
+constexpr size_t kFragCacheBytes = size_t(512) << 20;
+constexpr int kFragMaxGroups = 32768;  // group ids 2 g + is_stroke are kept in 16 bits
 // Waves per SIMD each heavy shading kernel is compiled for, i.e. its register budget
 // (512 / waves VGPRs per lane; beyond it the compiler spills to scratch).  Chosen by
 // measurement on the 300-path and 2,048-stroke scenes: these kernels wait on memory and
@@ -257,7 +258,7 @@ DVG_DEV void shade_list_record(const DevScene &S, V2 pt, const int *list, int n,
 
 // shade_list that also writes the group ids of the fragments it composites (2 g +
 // is_stroke, at most kFragSlots of them) to ids[slot * nsamp + idx].
-DVG_DEV void shade_list_ids(const DevScene &S, V2 pt, const int *list, int n, Compositor &comp, int *ids,
+DVG_DEV void shade_list_ids(const DevScene &S, V2 pt, const int *list, int n, Compositor &comp, uint16_t *ids,
                             size_t nsamp, int idx) {
     for (int i = 0; i < n; i++) {
         int g = list[i];
@@ -266,12 +267,12 @@ DVG_DEV void shade_list_ids(const DevScene &S, V2 pt, const int *list, int n, Co
         V2 local = xform_pt(c2s, pt);
         int st = S.stroke_type[g];
         if (st >= 0 && within_distance_local(S, g, local, nullptr)) {
-            if (comp.nf < kFragSlots) ids[(size_t)comp.nf * nsamp + idx] = 2 * g + 1;
+            if (comp.nf < kFragSlots) ids[(size_t)comp.nf * nsamp + idx] = (uint16_t)(2 * g + 1);
             comp.add(sample_paint(S, st, S.stroke_off[g], S.stroke_ns[g], pt), g, -1);
         }
         int ft = S.fill_type[g];
         if (ft >= 0 && is_inside_local(S, g, local, nullptr)) {
-            if (comp.nf < kFragSlots) ids[(size_t)comp.nf * nsamp + idx] = 2 * g;
+            if (comp.nf < kFragSlots) ids[(size_t)comp.nf * nsamp + idx] = (uint16_t)(2 * g);
             comp.add(sample_paint(S, ft, S.fill_off[g], S.fill_ns[g], pt), g, -1);
         }
     }
@@ -279,8 +280,8 @@ DVG_DEV void shade_list_ids(const DevScene &S, V2 pt, const int *list, int n, Co
 
 // The fragment records shade_list_record makes, from the group ids a forward kept: the
 // same paints at the same point composited in the same order.
-DVG_DEV void replay_frags(const DevScene &S, V2 pt, const int *ids, size_t nsamp, int idx, int nf, Compositor &comp,
-                          FragRec *frags) {
+DVG_DEV void replay_frags(const DevScene &S, V2 pt, const uint16_t *ids, size_t nsamp, int idx, int nf,
+                          Compositor &comp, FragRec *frags) {
     for (int j = 0; j < nf; j++) {
         const int gid = ids[(size_t)j * nsamp + idx];
         const int g = gid >> 1;
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_weight(DevScene S, Geom G, float *we
 
 template <int PRE>
 DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float *bg, float4 *colors, int *err,
-                        int *frag = nullptr) {
+                        uint16_t *frag = nullptr) {
     constexpr int prefilter = PRE;
     __shared__ int s_list[kCandCap];
     __shared__ int s_w[kBlock / 64];
@@ -359,13 +360,13 @@ DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float 
     if (valid) {
         V4 c = comp.result(has_bg, bgc);
         colors[idx] = make_float4(c.x, c.y, c.z, c.w);
-        if (!PRE && frag) frag[(size_t)kFragSlots * G.W * G.H * G.spp + idx] = comp.nf;
+        if (!PRE && frag) frag[(size_t)kFragSlots * G.W * G.H * G.spp + idx] = (uint16_t)min(comp.nf, 65535);
     }
     if (overflow == 2) atomicOr(err, 2);
 }
 template <int PRE>
 __global__ __launch_bounds__(kBlock, DVG_WAVES_COLOR) void k_color(DevScene S, Geom G, const float *bg, float4 *colors, int *err,
-                                                  int *frag) {
+                                                  uint16_t *frag) {
     color_tile<PRE>(S, G, blockIdx.x, bg, colors, err, frag);
 }
 
@@ -405,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(DevScene S, Geom G, int pref
 template <int PRE>
 DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const float *bg, const float *weight,
                             const float *d_img, float *d_bg, float *d_translation, int *err,
-                            const int *frag = nullptr) {
+                            const uint16_t *frag = nullptr) {
     constexpr int prefilter = PRE;
     __shared__ int s_list[kCandCap];
     __shared__ int s_w[kBlock / 64];
@@ -534,7 +535,7 @@ DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const fl
 template <int PRE>
 __global__ __launch_bounds__(kBlock, DVG_WAVES_COLOR_BWD) void k_color_bwd(DevScene S, Geom G, const float *bg, const float *weight,
                                                       const float *d_img, float *d_bg, float *d_translation,
-                                                      int *err, const int *frag) {
+                                                      int *err, const uint16_t *frag) {
     color_bwd_tile<PRE>(S, G, blockIdx.x, bg, weight, d_img, d_bg, d_translation, err, frag);
 }
 
@@ -1320,10 +1321,10 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
         const uint64_t key[4] = {a.seed, (uint64_t)(uint32_t)G.W | ((uint64_t)(uint32_t)G.H << 32),
                                  (uint64_t)(uint32_t)G.nsx | ((uint64_t)(uint32_t)G.nsy << 32), sc.version};
         if (a.render_image) {
-            int *frag = nullptr;
+            uint16_t *frag = nullptr;
             sc.frag_valid = false;
-            const size_t frag_bytes = (size_t)(kFragSlots + 1) * nsamp * sizeof(int);
-            if (!pre && frag_bytes <= kFragCacheBytes) {
+            const size_t frag_bytes = (size_t)(kFragSlots + 1) * nsamp * sizeof(uint16_t);
+            if (!pre && frag_bytes <= kFragCacheBytes && S.ng < kFragMaxGroups) {
                 // a backward on another stream may still read the lists being replaced
                 if (sc.frag && sc.frag_stream != st) DVG_CHECK(hipStreamSynchronize(sc.frag_stream));
                 if (!sc.frag || sc.frag_granted < frag_bytes) {
@@ -1332,7 +1333,7 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
                         cache_free(sc.device, sc.frag, sc.frag_granted);
                         sc.frag = nullptr;
                     }
-                    sc.frag = (int *)cache_alloc(sc.device, frag_bytes, &sc.frag_granted);
+                    sc.frag = (uint16_t *)cache_alloc(sc.device, frag_bytes, &sc.frag_granted);
                 }
                 frag = sc.frag;
             }
