@@ -434,8 +434,10 @@ def load_traffic(kernel, workload):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--steps', type=int, default=None,
+                    help='timed steps (default 50 for one scene, 20 for the 256-scene batch)')
+    ap.add_argument('--warmup', type=int, default=None,
+                    help='untimed steps first (default 20 for one scene: clocks and pools settle; 3 for the batch)')
     ap.add_argument('--config', type=int, default=2)
     ap.add_argument('--mode', default='tiled', choices=['tiled', 'plain'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
@@ -446,6 +448,10 @@ def main():
     ap.add_argument('--hbm-budget', type=int, default=64, help='GiB that requests in flight may hold (config 5)')
     ap.add_argument('--max-batch', type=int, default=32, help='requests per batch (config 5)')
     a = ap.parse_args()
+    if a.steps is None:
+        a.steps = 20 if a.config == 4 else 50
+    if a.warmup is None:
+        a.warmup = 3 if a.config == 4 else 20
     if a.cpu_baseline_only:
         cpu_baseline_main(a.config)
         return

@@ -355,6 +355,10 @@ class _SceneTemplate:
             self.cat_mode[kind] = mode
         self.int_pos = flat.pos['ncp'] + flat.pos['ids']
         self.int_bytes = flat.int_bytes
+        # the integer topology tensors last seen with these contents, and their versions:
+        # the same tensors, unmodified, need no comparison of contents
+        self._int_seen = [args[i] for i in self.int_pos]
+        self._int_versions = [t._version for t in self._int_seen]
         self.idle = []
         self._dev_tables = {}
 
@@ -368,9 +372,17 @@ class _SceneTemplate:
                 return False
         except (RuntimeError, TypeError, AttributeError):
             return False
+        cur = [args[i] for i in self.int_pos]
+        if all(a is b for a, b in zip(cur, self._int_seen)) and \
+                [t._version for t in cur] == self._int_versions:
+            return True
         ncp = _cat_int([args[i] for i in self.flat.pos['ncp']])
         gids = _cat_int([args[i] for i in self.flat.pos['ids']])
-        return ncp.tobytes() + b'|' + gids.tobytes() == self.int_bytes
+        if ncp.tobytes() + b'|' + gids.tobytes() != self.int_bytes:
+            return False
+        self._int_seen = cur
+        self._int_versions = [t._version for t in cur]
+        return True
 
     def _tables(self, dev):
         t = self._dev_tables.get(dev)
@@ -617,8 +629,8 @@ class RenderFunction(torch.autograd.Function):
                 else:
                     raise TypeError('unsupported paint %r' % (type(color),))
                 if kind == 'fill' and color is not None:
-                    for shape_id in shape_group.shape_ids:
-                        if isinstance(shapes[int(shape_id)], Path) and not shapes[int(shape_id)].is_closed:
+                    for shape_id in shape_group.shape_ids.tolist():
+                        if isinstance(shapes[shape_id], Path) and not shapes[shape_id].is_closed:
                             warnings.warn('Detected non-closed paths with fill color. This might causes '
                                           'unexpected results.', Warning)
             args.append(shape_group.use_even_odd_rule)
