@@ -92,7 +92,7 @@ class _BatchNode(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, *grad_imgs):
-        synchronize()
+        # earlier findings are read at the next render call or synchronize(), not here
         # the forward batch holds the output images, whose graph holds this context:
         # dropping it here lets the scenes go back to their pool without a cycle collection
         ctx.fwd = None

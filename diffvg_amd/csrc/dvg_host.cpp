@@ -622,6 +622,9 @@ void free_scene(SceneHost &sc) {
     cache_free(sc.device, sc.grad, sc.grad_granted);
     cache_free(sc.device, sc.err, 2 * sizeof(int));
     if (sc.frag) cache_free(sc.device, sc.frag, sc.frag_granted);
+    if (sc.pkeep) cache_free(sc.device, sc.pkeep, sc.pkeep_granted);
+    sc.pkeep = nullptr;
+    sc.pkeep_valid = false;
     sc.block = nullptr;
     sc.grad = nullptr;
     sc.err = nullptr;

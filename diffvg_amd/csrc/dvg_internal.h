@@ -89,6 +89,12 @@ struct SceneHost {
     bool frag_valid = false;
     uint64_t frag_key[4] = {0, 0, 0, 0};
     hipStream_t frag_stream = nullptr;
+    // the same for the prefiltered forward: per fragment the words its adjoint needs
+    int *pkeep = nullptr;
+    size_t pkeep_granted = 0;
+    bool pkeep_valid = false;
+    uint64_t pkeep_key[4] = {0, 0, 0, 0};
+    hipStream_t pkeep_stream = nullptr;
     DevScene dev{};
 };
 

@@ -39,6 +39,10 @@ constexpr int kCandCap = 2048;
 constexpr int kFragSlots = 16;
 constexpr size_t kFragCacheBytes = size_t(512) << 20;
 constexpr int kFragMaxGroups = 32768;  // group ids 2 g + is_stroke are kept in 16 bits
+// prefiltered fragments keep what their adjoint needs besides the paint: group id, closest
+// shape, signed distance, closest point, path position and whether a distance was found
+constexpr int kPFragSlots = 8;
+constexpr int kPFragWords = 9;
 // Waves per SIMD each heavy shading kernel is compiled for, i.e. its register budget
 // (512 / waves VGPRs per lane; beyond it the compiler spills to scratch).  Chosen by
 // measurement on the 300-path and 2,048-stroke scenes: these kernels wait on memory and
@@ -228,6 +232,100 @@ DVG_DEV void shade_list_prefiltered(const DevScene &S, V2 pt, const int *list, i
 }
 
 // Shading with the fragment list kept for the adjoint (non-prefiltered).
+// shade_list_prefiltered<false> that also writes, per fragment it composites (at most
+// kPFragSlots), the words replay_pfrags needs: keep[(slot * kPFragWords + w) * nsamp + idx].
+DVG_DEV void shade_list_prefiltered_keep(const DevScene &S, V2 pt, const int *list, int n, Compositor &comp,
+                                         int &overflow, int *keep, size_t nsamp, int idx) {
+    const float inf = __builtin_huge_valf();
+    auto put = [&](int gid, int ms, float d, V2 cp, const PathInfo &info, int within) {
+        if (comp.nf >= kPFragSlots) return;
+        int *k = keep + (size_t)comp.nf * kPFragWords * nsamp + idx;
+        k[0] = gid;
+        k[nsamp] = ms;
+        k[2 * nsamp] = __float_as_int(d);
+        k[3 * nsamp] = __float_as_int(cp.x);
+        k[4 * nsamp] = __float_as_int(cp.y);
+        k[5 * nsamp] = info.base_point_id;
+        k[6 * nsamp] = info.point_id;
+        k[7 * nsamp] = __float_as_int(info.t_root);
+        k[8 * nsamp] = within;
+    };
+    for (int i = 0; i < n; i++) {
+        int g = list[i];
+        if (S.ng > 1 && !in_box(S.g_box[g], pt, S.g_maxr[g])) continue;
+        int st = S.stroke_type[g];
+        if (st >= 0) {
+            int ms = -1;
+            V2 cp{0, 0};
+            PathInfo info{-1, -1, 0.f};
+            float d = inf;
+            compute_distance(S, g, pt, inf, ms, cp, info, d);
+            if (ms >= 0) {
+                float sw = S.stroke_width[ms];
+                float ad = (float)fabs((double)d);
+                float w = smoothstep(ad + sw) - smoothstep(ad - sw);
+                if (w > 0) {
+                    V4 c = sample_paint(S, st, S.stroke_off[g], S.stroke_ns[g], pt);
+                    c.w *= w;
+                    put(2 * g + 1, ms, d, cp, info, 1);
+                    comp.add(c, g, -1);
+                }
+            } else {
+                overflow = 2;  // the reference asserts here (diffvg.cpp:868)
+            }
+        }
+        int ft = S.fill_type[g];
+        if (ft >= 0) {
+            int ms = -1;
+            V2 cp{0, 0};
+            PathInfo info{-1, -1, 0.f};
+            float d = inf;
+            bool found = compute_distance(S, g, pt, 1.f, ms, cp, info, d);
+            M3 c2s = m3_load(S.c2s + 9 * g);
+            bool in = is_inside_local(S, g, xform_pt(c2s, pt), nullptr);
+            if (found || in) {
+                if (!in) d = -d;
+                float w = smoothstep(d);
+                if (w > 0) {
+                    V4 c = sample_paint(S, ft, S.fill_off[g], S.fill_ns[g], pt);
+                    c.w *= w;
+                    put(2 * g, ms, d, cp, info, found ? 1 : 0);
+                    comp.add(c, g, -1);
+                }
+            }
+        }
+    }
+}
+
+// The prefiltered fragment records shade_list_prefiltered<true> makes, from the words a
+// forward kept: paint and coverage weight recomputed from the kept distance.
+DVG_DEV void replay_pfrags(const DevScene &S, V2 pt, const int *keep, size_t nsamp, int idx, int nf,
+                           Compositor &comp, PFragRec *frags) {
+    for (int j = 0; j < nf; j++) {
+        const int *k = keep + (size_t)j * kPFragWords * nsamp + idx;
+        const int gid = k[0], ms = k[nsamp];
+        const float d = __int_as_float(k[2 * nsamp]);
+        const V2 cp{__int_as_float(k[3 * nsamp]), __int_as_float(k[4 * nsamp])};
+        const PathInfo info{k[5 * nsamp], k[6 * nsamp], __int_as_float(k[7 * nsamp])};
+        const int within = k[8 * nsamp];
+        const int g = gid >> 1;
+        V4 c;
+        if (gid & 1) {
+            float sw = S.stroke_width[ms];
+            float ad = (float)fabs((double)d);
+            float w = smoothstep(ad + sw) - smoothstep(ad - sw);
+            c = sample_paint(S, S.stroke_type[g], S.stroke_off[g], S.stroke_ns[g], pt);
+            c.w *= w;
+        } else {
+            float w = smoothstep(d);
+            c = sample_paint(S, S.fill_type[g], S.fill_off[g], S.fill_ns[g], pt);
+            c.w *= w;
+        }
+        frags[comp.nf] = PFragRec{V3{c.x, c.y, c.z}, c.w, comp.color, comp.alpha, gid, ms, d, cp, info, within};
+        comp.add(c, g, -1);
+    }
+}
+
 DVG_DEV void shade_list_record(const DevScene &S, V2 pt, const int *list, int n, Compositor &comp, FragRec *frags,
                                int &overflow) {
     for (int i = 0; i < n; i++) {
@@ -326,7 +424,7 @@ __global__ __launch_bounds__(kBlock) void k_weight(DevScene S, Geom G, float *we
 
 template <int PRE>
 DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float *bg, float4 *colors, int *err,
-                        uint16_t *frag = nullptr) {
+                        uint16_t *frag = nullptr, int *pkeep = nullptr) {
     constexpr int prefilter = PRE;
     __shared__ int s_list[kCandCap];
     __shared__ int s_w[kBlock / 64];
@@ -345,7 +443,11 @@ DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float 
         int n = build_candidates(S, tb, c0, min(S.ng, c0 + kCandCap), s_list, s_w);
         if (valid) {
             if constexpr (PRE) {
-                shade_list_prefiltered<false>(S, pt, s_list, n, comp, nullptr, overflow);
+                if (pkeep)
+                    shade_list_prefiltered_keep(S, pt, s_list, n, comp, overflow, pkeep, (size_t)G.W * G.H * G.spp,
+                                                idx);
+                else
+                    shade_list_prefiltered<false>(S, pt, s_list, n, comp, nullptr, overflow);
             } else if (frag) {
                 shade_list_ids(S, pt, s_list, n, comp, frag, (size_t)G.W * G.H * G.spp, idx);
             } else {
@@ -361,13 +463,14 @@ DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float 
         V4 c = comp.result(has_bg, bgc);
         colors[idx] = make_float4(c.x, c.y, c.z, c.w);
         if (!PRE && frag) frag[(size_t)kFragSlots * G.W * G.H * G.spp + idx] = (uint16_t)min(comp.nf, 65535);
+        if (PRE && pkeep) pkeep[(size_t)kPFragSlots * kPFragWords * G.W * G.H * G.spp + idx] = comp.nf;
     }
     if (overflow == 2) atomicOr(err, 2);
 }
 template <int PRE>
 __global__ __launch_bounds__(kBlock, DVG_WAVES_COLOR) void k_color(DevScene S, Geom G, const float *bg, float4 *colors, int *err,
-                                                  uint16_t *frag) {
-    color_tile<PRE>(S, G, blockIdx.x, bg, colors, err, frag);
+                                                  uint16_t *frag, int *pkeep) {
+    color_tile<PRE>(S, G, blockIdx.x, bg, colors, err, frag, pkeep);
 }
 
 DVG_DEV void resolve_pixel(const DevScene &S, const Geom &G, int pix, int prefilter, const float *weight,
@@ -406,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(DevScene S, Geom G, int pref
 template <int PRE>
 DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const float *bg, const float *weight,
                             const float *d_img, float *d_bg, float *d_translation, int *err,
-                            const uint16_t *frag = nullptr) {
+                            const uint16_t *frag = nullptr, const int *pkeep = nullptr) {
     constexpr int prefilter = PRE;
     __shared__ int s_list[kCandCap];
     __shared__ int s_w[kBlock / 64];
@@ -433,7 +536,8 @@ DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const fl
     const size_t nsamp = (size_t)G.W * G.H * G.spp;
     int kept = -1;
     if (!PRE && frag && valid) kept = frag[(size_t)kFragSlots * nsamp + idx];
-    const bool replay = kept >= 0 && kept <= kFragSlots;
+    if (PRE && pkeep && valid) kept = pkeep[(size_t)kPFragSlots * kPFragWords * nsamp + idx];
+    const bool replay = kept >= 0 && kept <= (PRE ? kPFragSlots : kFragSlots);
     if (__syncthreads_or(valid && !replay)) {
         for (int c0 = 0; c0 < S.ng; c0 += kCandCap) {
             int n = build_candidates(S, tb, c0, min(S.ng, c0 + kCandCap), s_list, s_w);
@@ -448,6 +552,8 @@ DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const fl
     }
     if constexpr (!PRE) {
         if (valid && replay) replay_frags(S, pt, frag, nsamp, idx, kept, comp, frags);
+    } else {
+        if (valid && replay) replay_pfrags(S, pt, pkeep, nsamp, idx, kept, comp, pfrags);
     }
     float d_radius = 0.f;
     int kind = -1;
@@ -535,8 +641,8 @@ DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const fl
 template <int PRE>
 __global__ __launch_bounds__(kBlock, DVG_WAVES_COLOR_BWD) void k_color_bwd(DevScene S, Geom G, const float *bg, const float *weight,
                                                       const float *d_img, float *d_bg, float *d_translation,
-                                                      int *err, const uint16_t *frag) {
-    color_bwd_tile<PRE>(S, G, blockIdx.x, bg, weight, d_img, d_bg, d_translation, err, frag);
+                                                      int *err, const uint16_t *frag, const int *pkeep) {
+    color_bwd_tile<PRE>(S, G, blockIdx.x, bg, weight, d_img, d_bg, d_translation, err, frag, pkeep);
 }
 
 DVG_DEV void edge_sample_one(const DevScene &S, const Geom &G, int idx, int count, EdgeSample *edges,
@@ -1265,9 +1371,13 @@ static int report_flags(const int *h) {
     return 0;
 }
 
+// Reads the conditions found by the last update and by the renders since the previous
+// read, and clears the latter: a condition a forward finds is still there when its
+// backward has rendered on the same scene.
 static int check_err_flag(SceneHost &sc, hipStream_t st) {
     int h[2] = {0, 0};
     DVG_CHECK(hipMemcpyAsync(h, sc.err, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    DVG_CHECK(hipMemsetAsync(sc.err, 0, sizeof(int), st));
     DVG_CHECK(hipStreamSynchronize(st));
     return report_flags(h);
 }
@@ -1298,7 +1408,6 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
     DevScene S = sc.dev;
     if (eval_mode && (a.render_image || a.d_render_image))
         return set_error("eval_positions are supported for the SDF output only (as in the reference)");
-    DVG_CHECK(hipMemsetAsync(sc.err, 0, sizeof(int), st));
     if (sc.num_shapes == 0) return render_empty(a, npix, st);
     const bool color_out = a.render_image || a.d_render_image;
     const bool sdf_out = a.render_sdf || a.d_render_sdf;
@@ -1320,34 +1429,55 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
         // sizes and parameter version, enqueued on the same stream) replays it.
         const uint64_t key[4] = {a.seed, (uint64_t)(uint32_t)G.W | ((uint64_t)(uint32_t)G.H << 32),
                                  (uint64_t)(uint32_t)G.nsx | ((uint64_t)(uint32_t)G.nsy << 32), sc.version};
+        // (re)allocates a kept-list buffer of `bytes`; a backward on another stream may
+        // still read the lists being replaced
+        auto keep_buffer = [&](void *&ptr, size_t &granted, hipStream_t owner, size_t bytes) -> int {
+            if (ptr && owner != st) DVG_CHECK(hipStreamSynchronize(owner));
+            if (!ptr || granted < bytes) {
+                if (ptr) {
+                    DVG_CHECK(hipStreamSynchronize(st));
+                    cache_free(sc.device, ptr, granted);
+                    ptr = nullptr;
+                }
+                ptr = cache_alloc(sc.device, bytes, &granted);
+            }
+            return 0;
+        };
         if (a.render_image) {
             uint16_t *frag = nullptr;
+            int *pkeep = nullptr;
             sc.frag_valid = false;
+            sc.pkeep_valid = false;
             const size_t frag_bytes = (size_t)(kFragSlots + 1) * nsamp * sizeof(uint16_t);
+            const size_t pkeep_bytes = (size_t)(kPFragSlots * kPFragWords + 1) * nsamp * sizeof(int);
             if (!pre && frag_bytes <= kFragCacheBytes && S.ng < kFragMaxGroups) {
-                // a backward on another stream may still read the lists being replaced
-                if (sc.frag && sc.frag_stream != st) DVG_CHECK(hipStreamSynchronize(sc.frag_stream));
-                if (!sc.frag || sc.frag_granted < frag_bytes) {
-                    if (sc.frag) {
-                        DVG_CHECK(hipStreamSynchronize(st));
-                        cache_free(sc.device, sc.frag, sc.frag_granted);
-                        sc.frag = nullptr;
-                    }
-                    sc.frag = (uint16_t *)cache_alloc(sc.device, frag_bytes, &sc.frag_granted);
-                }
+                void *b = sc.frag;
+                if (int e = keep_buffer(b, sc.frag_granted, sc.frag_stream, frag_bytes)) return e;
+                sc.frag = (uint16_t *)b;
                 frag = sc.frag;
+            }
+            if (pre && pkeep_bytes <= kFragCacheBytes) {
+                void *b = sc.pkeep;
+                if (int e = keep_buffer(b, sc.pkeep_granted, sc.pkeep_stream, pkeep_bytes)) return e;
+                sc.pkeep = (int *)b;
+                pkeep = sc.pkeep;
             }
             {
                 DVG_PROF(st, "k_color");
                 if (pre)
-                    k_color<1><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.colors, sc.err, nullptr);
+                    k_color<1><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.colors, sc.err, nullptr, pkeep);
                 else
-                    k_color<0><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.colors, sc.err, frag);
+                    k_color<0><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.colors, sc.err, frag, nullptr);
             }
             if (frag) {
                 sc.frag_valid = true;
                 for (int k = 0; k < 4; k++) sc.frag_key[k] = key[k];
                 sc.frag_stream = st;
+            }
+            if (pkeep) {
+                sc.pkeep_valid = true;
+                for (int k = 0; k < 4; k++) sc.pkeep_key[k] = key[k];
+                sc.pkeep_stream = st;
             }
             {
                 DVG_PROF(st, "k_resolve");
@@ -1359,15 +1489,21 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
             {
                 DVG_PROF(st, "k_color_bwd");
                 bool kept = !pre && sc.frag_valid && sc.frag_stream == st;
-                for (int k = 0; k < 4; k++) kept = kept && sc.frag_key[k] == key[k];
+                bool pkept = pre && sc.pkeep_valid && sc.pkeep_stream == st;
+                for (int k = 0; k < 4; k++) {
+                    kept = kept && sc.frag_key[k] == key[k];
+                    pkept = pkept && sc.pkeep_key[k] == key[k];
+                }
                 if (pre)
                     k_color_bwd<1><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.weight,
                                                               a.d_render_image, a.d_background_image,
-                                                              a.d_translation, sc.err, nullptr);
+                                                              a.d_translation, sc.err, nullptr,
+                                                              pkept ? sc.pkeep : nullptr);
                 else
                     k_color_bwd<0><<<ntiles, kBlock, 0, st>>>(S, G, a.background_image, ws.weight,
                                                               a.d_render_image, a.d_background_image,
-                                                              a.d_translation, sc.err, kept ? sc.frag : nullptr);
+                                                              a.d_translation, sc.err, kept ? sc.frag : nullptr,
+                                                              nullptr);
             }
             if (!pre) {
                 DVG_CHECK(hipMemsetAsync(ws.tile_count, 0, sizeof(int) * ntiles, st));
@@ -2234,6 +2370,7 @@ __global__ __launch_bounds__(kBlock) void k_refit_cdf(RefitArgs A) {
 int update_scene(SceneHost &sc, const dvg_scene_params &p, float filter_radius, hipStream_t st) {
     sc.version++;
     sc.frag_valid = false;
+    sc.pkeep_valid = false;
     if (!(filter_radius > 0) || !std::isfinite(filter_radius)) return set_error("filter radius must be positive");
     sc.dev.filter_radius = filter_radius;
     if (!sc.device_set || sc.num_shapes == 0) return 0;

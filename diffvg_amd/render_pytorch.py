@@ -534,23 +534,26 @@ def clear_scene_cache():
     _TEMPLATES.clear()
 
 
-# Backward passes whose device-side findings (fragment overflow, a non-finite grad_img)
-# have not been read yet: the backward returns while its kernels run, so the autograd
-# engine hands out the gradients meanwhile; the findings are read at the next entry into
-# the render path or at synchronize().
+# Render passes whose device-side findings (fragment overflow, a non-finite grad_img, a
+# prefiltered circle stroke) have not been read yet: the passes return while their
+# kernels run, so the host goes on (the autograd engine hands out the gradients
+# meanwhile); the findings are read at the next entry into the render path or at
+# synchronize().
 _PENDING = []
 
 
 def synchronize():
-    """Waits for the backward passes still in flight and raises what they found (the
-    reference's backward asserts a finite grad_img and aborts on fragment overflow)."""
+    """Waits for the render passes still in flight and raises what they found (the
+    reference's backward asserts a finite grad_img and aborts on fragment overflow; its
+    prefiltered forward asserts on a stroked circle)."""
     while _PENDING:
         work, finite = _PENDING.pop(0)
         try:
             work.status()
         except RuntimeError as e:
-            raise RuntimeError('%s (found by an earlier backward pass)' % e) from None
-        assert bool(finite), 'non-finite grad_img in an earlier backward pass'
+            raise RuntimeError('%s (found by an earlier render pass)' % e) from None
+        if finite is not None:
+            assert bool(finite), 'non-finite grad_img in an earlier backward pass'
 
 
 def _check_gpu(read_findings=True):
@@ -764,8 +767,10 @@ class _RenderNode(torch.autograd.Function):
                      use_prefiltering=flat.use_prefiltering, eval_positions=ev,
                      num_eval_positions=0 if ev is None else ev.shape[0], stream=_stream())
         if flat.use_prefiltering:
-            # a prefiltered stroke of a circle group: the reference asserts in the forward
-            scene.status()
+            # a prefiltered stroke of a circle group, on which the reference asserts in the
+            # forward: read with the other findings, at the next render call or
+            # synchronize(), so the host does not wait for the forward's kernels here
+            _PENDING.append((scene, None))
         if print_timing:
             torch.cuda.synchronize()
             print('Forward pass, time: %.5f s' % (time.time() - start))
@@ -784,7 +789,8 @@ class _RenderNode(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_img):
-        synchronize()
+        # earlier findings are read at the next render call or synchronize(), not here:
+        # this call would wait for its own forward's kernels
         if not grad_img.is_contiguous():
             grad_img = grad_img.contiguous()
         # the reference asserts a finite grad_img before rendering; the check is taken on

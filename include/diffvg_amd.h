@@ -176,8 +176,10 @@ typedef struct dvg_scene_params {
  * schedule's top-level BVH is not rebuilt: PLAIN renders of an updated scene fail.
  * Batches created earlier keep the filter radius they were created with. */
 int dvg_scene_update(dvg_scene *scene, const dvg_scene_params *params, float filter_radius, void *stream);
-/* Waits for the scene's last stream and reports the conditions its last update and
- * render found (the reference asserts or throws on them): 0 when there are none. */
+/* Waits for the scene's last stream and reports the conditions its last update found and
+ * those its renders found since the previous dvg_scene_status or dvg_scene_get_grads call
+ * (the reference asserts or throws on them; the render conditions are cleared by the
+ * call): 0 when there are none. */
 int dvg_scene_status(dvg_scene *scene);
 /* Copies the scene's device block (its derived tables; layout internal to this library
  * version) to host memory `dst` of `cap` bytes; `bytes` receives the block size.  The

@@ -17,13 +17,14 @@ pytestmark = pytest.mark.gpu
 KEYS = ('shape_params', 'points', 'thickness', 'stroke_width', 'color_params', 'shape_to_canvas', 'filter_radius')
 
 
-def layers(size=48, n=14):
+def layers(size=48, n=14, strokes=True):
     """n overlapping translucent circles: the centre samples have n fragments, more than
-    a forward keeps."""
+    a forward keeps (without strokes for prefiltering, where circle strokes are rejected
+    as the reference rejects them)."""
     shapes = [pd.Circle(radius=scenes.T(14.0 - 0.5 * k), center=scenes.T([24.0 + 0.3 * k, 24.0 - 0.2 * k]))
               for k in range(n)]
     groups = [pd.ShapeGroup(shape_ids=torch.tensor([k]), fill_color=scenes.T([0.1 * (k % 7), 0.5, 0.9 - 0.05 * k, 0.3]),
-                            stroke_color=scenes.T([0, 0, 0, 0.5]) if k % 3 == 0 else None)
+                            stroke_color=scenes.T([0, 0, 0, 0.5]) if strokes and k % 3 == 0 else None)
               for k in range(n)]
     for k, s in enumerate(shapes):
         s.stroke_width = scenes.T(0.8)
@@ -31,7 +32,11 @@ def layers(size=48, n=14):
 
 
 def _flat(case):
-    return scenes.flat_case(layers() if case == 'layers' else case)
+    if case == 'layers':
+        return scenes.flat_case(layers())
+    if case == 'layers_fill':
+        return scenes.flat_case(layers(strokes=False))
+    return scenes.flat_case(case)
 
 
 def _same(a, b, label):
@@ -44,19 +49,22 @@ def _same(a, b, label):
         assert np.abs(x - y).max() <= 1e-3 * max(np.abs(y).max(), 1e-8), (label, k)
 
 
-@pytest.mark.parametrize('case', ['overlap_eo', 'gradients', 'strokes', 'transformed', 'layers'])
-def test_replayed_backward_equals_shading_backward(case):
-    print('fragcache', case, flush=True)
+@pytest.mark.parametrize('case,prefilter', [('overlap_eo', False), ('gradients', False), ('strokes', False),
+                                             ('transformed', False), ('layers', False), ('overlap_eo', True),
+                                             ('gradients', True), ('strokes', True), ('transformed', True),
+                                             ('layers_fill', True)])
+def test_replayed_backward_equals_shading_backward(case, prefilter):
+    print('fragcache', case, prefilter, flush=True)
     f = _flat(case)
     w, h = f['canvas_width'], f['canvas_height']
     d = np.random.default_rng(3).uniform(-1, 1, (h, w, 4)).astype(np.float32)
     a = GpuRunner(f)
-    fwd = a.render(w, h, 2, 2, 5)
-    kept = a.render(w, h, 2, 2, 5, d_image=d, translation=True)
-    fresh = GpuRunner(f).render(w, h, 2, 2, 5, d_image=d, translation=True)
+    fwd = a.render(w, h, 2, 2, 5, prefilter=prefilter)
+    kept = a.render(w, h, 2, 2, 5, d_image=d, translation=True, prefilter=prefilter)
+    fresh = GpuRunner(f).render(w, h, 2, 2, 5, d_image=d, translation=True, prefilter=prefilter)
     _same(kept, fresh, case)
     # a second forward of the same render gives the same image
-    assert np.array_equal(a.render(w, h, 2, 2, 5)['image'], fwd['image'])
+    assert np.array_equal(a.render(w, h, 2, 2, 5, prefilter=prefilter)['image'], fwd['image'])
 
 
 def test_lists_of_another_seed_are_not_used():

@@ -262,3 +262,23 @@ def test_backward_findings_are_reported_at_the_next_sync():
     img2, g2 = _step(sc, target)
     pd.synchronize()
     assert np.isfinite(img2).all() and all(np.isfinite(g).all() for g in g2)
+
+
+def test_prefiltered_circle_stroke_is_reported_at_the_next_sync():
+    """The reference's prefiltered forward asserts on a stroked circle (diffvg.cpp:868).
+    Here the forward returns while its kernels run; the condition is raised at
+    synchronize() (or at the next render call) -- also after the backward has rendered
+    on the same scene -- and is then cleared."""
+    pd.set_use_gpu(True)
+    pd.set_device(torch.device('cuda:0'))
+    rp.synchronize()
+    c = pd.Circle(radius=scenes.T(10.0), center=scenes.T([32.0, 32.0]))
+    c.radius.requires_grad_(True)
+    g = pd.ShapeGroup(shape_ids=torch.tensor([0]), fill_color=scenes.T([0.2, 0.4, 0.6, 1.0]),
+                      stroke_color=scenes.T([0.0, 0.0, 0.0, 1.0]))
+    args = pd.RenderFunction.serialize_scene(64, 64, [c], [g], use_prefiltering=True)
+    img = pd.RenderFunction.apply(64, 64, 2, 2, 0, None, *args)
+    img.sum().backward()
+    with pytest.raises(RuntimeError):
+        pd.synchronize()
+    pd.synchronize()
