@@ -34,6 +34,8 @@ namespace dvg {
 
 constexpr int kBlock = 256;
 constexpr int kCandCap = 2048;
+// boundary samples per k_edge workgroup: each lane shades one side of one sample
+constexpr int kEdgeChunk = kBlock / 2;
 // fragment lists kept from a colour forward for its backward: up to kFragSlots fragments
 // per sample (a sample with more is shaded again), at most kFragCacheBytes per scene
 constexpr int kFragSlots = 16;
@@ -666,8 +668,8 @@ __global__ __launch_bounds__(kBlock) void k_edge_sample(DevScene S, Geom G, int 
 }
 
 // Exclusive scans, in one workgroup, of the per-tile boundary-sample counts (-> tile
-// offsets) and of the per-tile chunk counts ceil(count / kBlock) (-> chunk offsets; entry
-// n holds the total), so that k_edge can run one workgroup per 256-sample chunk.
+// offsets) and of the per-tile chunk counts ceil(count / kEdgeChunk) (-> chunk offsets; entry
+// n holds the total), so that k_edge can run one workgroup per chunk.
 DVG_DEV void scan_tiles(const int *in, int *out, int *cursor, int *chunk_off, int n) {
     __shared__ int s[1024];
     __shared__ int sc[1024];
@@ -677,7 +679,7 @@ DVG_DEV void scan_tiles(const int *in, int *out, int *cursor, int *chunk_off, in
     for (int base = 0; base < n; base += 1024) {
         int i = base + threadIdx.x;
         int v = i < n ? in[i] : 0;
-        int vc = (v + kBlock - 1) / kBlock;
+        int vc = (v + kEdgeChunk - 1) / kEdgeChunk;
         s[threadIdx.x] = v;
         sc[threadIdx.x] = vc;
         __syncthreads();
@@ -719,7 +721,7 @@ __global__ __launch_bounds__(kBlock) void k_edge_bin(int count, const int *edge_
     edge_bin_one(blockIdx.x * blockDim.x + threadIdx.x, count, edge_tile, tile_off, cursor, sorted);
 }
 
-// One workgroup per chunk of up to 256 boundary samples of one tile (the grid is an
+// One workgroup per chunk of up to kEdgeChunk boundary samples of one tile (the grid is an
 // upper bound on the chunk count; workgroups past the total exit at once), so a tile on a
 // busy boundary is spread over many workgroups instead of looping inside one.
 DVG_DEV void edge_chunk(const DevScene &S, const Geom &G, int chunk, const EdgeSample *edges, const int *tile_off,
@@ -745,41 +747,34 @@ DVG_DEV void edge_chunk(const DevScene &S, const Geom &G, int chunk, const EdgeS
     int off = tile_off[tile];
     bool has_bg = bg != nullptr;
     {
-        const int b0 = (chunk - chunk_off[tile]) * kBlock;
-        int k = b0 + threadIdx.x;
+        // lanes 2i and 2i + 1 shade the inside and the outside point of sample i
+        const int side = threadIdx.x & 1;
+        const int b0 = (chunk - chunk_off[tile]) * kEdgeChunk;
+        int k = b0 + (threadIdx.x >> 1);
         bool valid = k < cnt;
         const int eidx = valid ? sorted[off + k] : 0;
-        // Only what shading needs stays live across the candidate loop (the two points,
-        // the target, the compositors); the rest of the sample is read again afterwards,
-        // which keeps the loop within the register budget of the occupancy target.
-        V2 pin{0, 0}, pout{0, 0};
+        V2 p{0, 0};
         int tgroup = -1, tshape = -1;
-        Compositor ci, co;
+        Compositor comp;
         {
             EdgeSample es0{};
             if (valid) es0 = edges[eidx];
             V2 bp{es0.px, es0.py}, n0{es0.nx, es0.ny};
-            V2 qi = bp - 1e-4f * n0, qo = bp + 1e-4f * n0;
-            pin = V2{qi.x * S.cw, qi.y * S.ch};
-            pout = V2{qo.x * S.cw, qo.y * S.ch};
+            V2 q = side ? bp + 1e-4f * n0 : bp - 1e-4f * n0;
+            p = V2{q.x * S.cw, q.y * S.ch};
             if (valid) {
                 tgroup = es0.group;
                 tshape = es0.shape;
             }
             int bx0 = (int)(es0.px * G.W), by0 = (int)(es0.py * G.H);
             V4 bg0 = (valid && has_bg) ? load_bg(bg, by0 * G.W + bx0) : V4{0, 0, 0, 0};
-            ci.init(has_bg, bg0);
-            co.init(has_bg, bg0);
+            comp.init(has_bg, bg0);
         }
-        EdgeQuery qin{tgroup, tshape, false};
-        EdgeQuery qout = qin;
-        bool hin = false, hout = false;
+        EdgeQuery q{tgroup, tshape, false};
+        bool hit = false;
         for (int c0 = 0; c0 < S.ng; c0 += kCandCap) {
             int nc = build_candidates(S, tb, c0, min(S.ng, c0 + kCandCap), s_list, s_w);
-            if (valid) {
-                shade_list(S, pin, s_list, nc, ci, &qin, hin);
-                shade_list(S, pout, s_list, nc, co, &qout, hout);
-            }
+            if (valid) shade_list(S, p, s_list, nc, comp, &q, hit);
             if (c0 + kCandCap < S.ng) __syncthreads();
         }
         asm volatile("" ::: "memory");  // read the sample again, do not keep it live
@@ -788,17 +783,17 @@ DVG_DEV void edge_chunk(const DevScene &S, const Geom &G, int chunk, const EdgeS
         V2 n{es.nx, es.ny};
         int bx = (int)(es.px * G.W), by = (int)(es.py * G.H);
         V4 bgc = (valid && has_bg) ? load_bg(bg, by * G.W + bx) : V4{0, 0, 0, 0};
+        // this side's colour and edge hit; the partner lane holds the other side
+        const bool my_hit = valid && comp.edge_hit(hit);
+        const V4 mine = valid ? comp.result(has_bg, bgc) : V4{0, 0, 0, 0};
+        const bool other_hit = __shfl_xor((int)my_hit, 1) != 0;
+        const V4 other{__shfl_xor(mine.x, 1), __shfl_xor(mine.y, 1), __shfl_xor(mine.z, 1), __shfl_xor(mine.w, 1)};
         // every lane stays to the end: the gradient scatter combines lanes of a wave
-        bool active = valid;
-        bool in_hit = false, out_hit = false;
-        if (active) {
-            in_hit = ci.edge_hit(hin);
-            out_hit = co.edge_hit(hout);
-            active = in_hit || out_hit;
-        }
+        const bool in_hit = my_hit, out_hit = other_hit;
+        bool active = valid && side == 0 && (in_hit || out_hit);
         float contrib = 0.f;
         if (active) {
-            V4 cin = ci.result(has_bg, bgc), cout = co.result(has_bg, bgc);
+            V4 cin = mine, cout = other;
             if (!in_hit) {
                 n = -n;
                 V4 tmp = cin;
@@ -1521,7 +1516,7 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
                 {
                     DVG_PROF(st, "k_edge");
                     // at most one partial chunk per tile plus nsamp / 256 full ones
-                    unsigned grid = (unsigned)(ntiles + (nsamp + kBlock - 1) / kBlock);
+                    unsigned grid = (unsigned)(ntiles + (nsamp + kEdgeChunk - 1) / kEdgeChunk);
                     k_edge<<<grid, kBlock, 0, st>>>(S, G, ws.edges, ws.tile_off, ws.tile_count, ws.chunk_off,
                                                     ntiles, ws.edge_sorted, a.background_image, ws.weight,
                                                     a.d_render_image, a.d_translation);
@@ -1870,7 +1865,7 @@ int batch_create(SceneHost *const *scenes, const dvg_render_args *args, int n, i
             case kStPix: return color ? (J.npix + kBlock - 1) / kBlock : 0;
             case kStTile: return color ? J.ntiles : 0;
             case kStSamp: return edges ? (J.nsamp + kBlock - 1) / kBlock : 0;
-            case kStChunk: return edges ? J.ntiles + (J.nsamp + kBlock - 1) / kBlock : 0;
+            case kStChunk: return edges ? J.ntiles + (J.nsamp + kEdgeChunk - 1) / kEdgeChunk : 0;
             case kStSdf: return sdf ? (J.eval ? (J.num_eval + kBlock - 1) / kBlock : J.ntiles) : 0;
             default: return std::max(1, std::min(64, (std::max(J.grad_n, J.ntiles) + kBlock - 1) / kBlock));
         }
