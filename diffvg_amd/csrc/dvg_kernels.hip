@@ -48,13 +48,14 @@ constexpr int kPFragWords = 9;
 // Waves per SIMD each heavy shading kernel is compiled for, i.e. its register budget
 // (512 / waves VGPRs per lane; beyond it the compiler spills to scratch).  Chosen by
 // measurement on the 300-path and 2,048-stroke scenes: these kernels wait on memory and
-// on long FP64 dependency chains, and 5-6 waves hide that better than 2 waves without
-// spills (k_edge 5.03 -> 3.16 ms, k_color 1.45 -> 0.92 ms, k_color_bwd 1.90 -> 1.33 ms).
+// on long FP64 dependency chains, and 4-6 waves hide that better than 2 waves without
+// spills (k_edge 5.03 -> 3.16 ms, k_color 1.45 -> 0.92 ms, k_color_bwd 1.90 -> 1.33 ms);
+// the colour backward, which mostly replays kept fragment lists, is best at 4.
 #ifndef DVG_WAVES_COLOR
 #define DVG_WAVES_COLOR 5
 #endif
 #ifndef DVG_WAVES_COLOR_BWD
-#define DVG_WAVES_COLOR_BWD 5
+#define DVG_WAVES_COLOR_BWD 4
 #endif
 #ifndef DVG_WAVES_EDGE
 #define DVG_WAVES_EDGE 6
