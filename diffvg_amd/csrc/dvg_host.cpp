@@ -608,9 +608,9 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     D.d_xform = sc.grad + sc.g_xform;
     D.d_filter_radius = sc.grad + sc.g_filter;
     size_t eg = 0;
-    sc.err = (int *)cache_alloc(sc.device, 2 * sizeof(int), &eg);
+    sc.err = (int *)cache_alloc(sc.device, kErrInts * sizeof(int), &eg);
     if (!sc.err) return set_error("out of device memory");
-    if (int e = clear_device(sc.device, sc.err, 2 * sizeof(int))) return e;
+    if (int e = clear_device(sc.device, sc.err, kErrInts * sizeof(int))) return e;
     return 0;
 }
 
@@ -620,7 +620,7 @@ void free_scene(SceneHost &sc) {
     if (hipGetDevice(&prev) == hipSuccess) (void)hipSetDevice(sc.device);
     cache_free(sc.device, sc.block, sc.block_granted);
     cache_free(sc.device, sc.grad, sc.grad_granted);
-    cache_free(sc.device, sc.err, 2 * sizeof(int));
+    cache_free(sc.device, sc.err, kErrInts * sizeof(int));
     if (sc.frag) cache_free(sc.device, sc.frag, sc.frag_granted);
     if (sc.pkeep) cache_free(sc.device, sc.pkeep, sc.pkeep_granted);
     sc.pkeep = nullptr;

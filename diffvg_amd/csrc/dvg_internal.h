@@ -61,6 +61,8 @@ struct RefitTables {
     int *gorder = nullptr;             // [n_ids] sort scratch
 };
 
+constexpr int kErrInts = 16;  // ints of SceneHost::err
+
 struct SceneHost {
     int device = 0;
     bool device_set = false;      // a device was chosen and memory may be held on it
@@ -73,7 +75,10 @@ struct SceneHost {
     float *grad = nullptr;        // gradient arrays
     size_t grad_floats = 0, grad_granted = 0;
     size_t g_params = 0, g_points = 0, g_thick = 0, g_sw = 0, g_color = 0, g_xform = 0, g_filter = 0;
-    int *err = nullptr;            // [0] render conditions, [1] conditions found by dvg_scene_update
+    // [0] render conditions, [1] conditions found by dvg_scene_update, [2] parameter
+    // version on the device (advanced by every update), [4..9] and [10..15] the key of the
+    // kept fragment lists (non-prefiltered, prefiltered) a forward last wrote
+    int *err = nullptr;
     hipStream_t last_stream = nullptr;
     RefitTables refit;
     size_t top_offset = 0, top_bytes = 0;          // top-level BVH in the block
@@ -95,6 +100,7 @@ struct SceneHost {
     bool pkeep_valid = false;
     uint64_t pkeep_key[4] = {0, 0, 0, 0};
     hipStream_t pkeep_stream = nullptr;
+    int keep_refs = 0;            // batches holding the kept-list buffers: no reallocation
     DevScene dev{};
 };
 

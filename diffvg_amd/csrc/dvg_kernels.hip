@@ -425,9 +425,35 @@ __global__ __launch_bounds__(kBlock) void k_weight(DevScene S, Geom G, float *we
     weight_pixel(S, G, blockIdx.x * blockDim.x + threadIdx.x, weight);
 }
 
+// Key of kept fragment lists: seed, sizes, sample counts and the scene's parameter
+// version on the device.  The forward that writes the lists writes the key beside them;
+// a backward replays them only when the key matches its own, so a backward that follows
+// another render, or a device update, on the scene shades again -- also when it is
+// replayed from a HIP graph.
+constexpr int kKeyInts = 6;
+DVG_DEV void key_of(const Geom &G, const int *ver, int *k) {
+    k[0] = (int)(uint32_t)G.seed;
+    k[1] = (int)(uint32_t)(G.seed >> 32);
+    k[2] = G.W;
+    k[3] = G.H;
+    k[4] = G.nsx | (G.nsy << 16);
+    k[5] = *ver;
+}
+DVG_DEV bool key_matches(const Geom &G, const int *ver, const int *hdr) {
+    int k[kKeyInts];
+    key_of(G, ver, k);
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < kKeyInts; i++) ok = ok && hdr[i] == k[i];
+    return ok;
+}
+
 template <int PRE>
 DVG_DEV void color_tile(const DevScene &S, const Geom &G, int tile, const float *bg, float4 *colors, int *err,
                         uint16_t *frag = nullptr, int *pkeep = nullptr) {
+    // the key of the lists this forward keeps (err: see SceneHost::err)
+    if ((PRE ? pkeep != nullptr : frag != nullptr) && tile == 0 && threadIdx.x == 0)
+        key_of(G, err + 2, err + (PRE ? 10 : 4));
     constexpr int prefilter = PRE;
     __shared__ int s_list[kCandCap];
     __shared__ int s_w[kBlock / 64];
@@ -537,6 +563,10 @@ DVG_DEV void color_bwd_tile(const DevScene &S, const Geom &G, int tile, const fl
     float4 tb = tile_box(S, G, tx, ty, 0.f);
     // a sample whose forward kept its fragment list replays it; the others shade
     const size_t nsamp = (size_t)G.W * G.H * G.spp;
+    if (!key_matches(G, err + 2, err + (PRE ? 10 : 4))) {
+        frag = nullptr;
+        pkeep = nullptr;
+    }
     int kept = -1;
     if (!PRE && frag && valid) kept = frag[(size_t)kFragSlots * nsamp + idx];
     if (PRE && pkeep && valid) kept = pkeep[(size_t)kPFragSlots * kPFragWords * nsamp + idx];
@@ -1394,6 +1424,22 @@ static int render_empty(const dvg_render_args &a, size_t npix, hipStream_t st) {
     return 0;
 }
 
+// Makes `ptr` a kept-list buffer of at least `bytes` for scene sc: the current one when
+// it is large enough, else a new one -- unless batches hold the current one, and then
+// false (the render shades its backward samples).  `owner` is the stream that last used
+// the buffer.
+static bool keep_buffer(SceneHost &sc, void *&ptr, size_t &granted, hipStream_t owner, size_t bytes) {
+    if (ptr && granted >= bytes) return true;
+    if (sc.keep_refs > 0) return false;
+    if (ptr) {
+        if (hipStreamSynchronize(owner) != hipSuccess) return false;
+        cache_free(sc.device, ptr, granted);
+        ptr = nullptr;
+    }
+    ptr = cache_alloc(sc.device, bytes, &granted);
+    return ptr != nullptr;
+}
+
 int render_tiled(SceneHost &sc, const dvg_render_args &a) {
     hipStream_t st = (hipStream_t)a.stream;
     Geom G = make_geom(sc, a);
@@ -1425,20 +1471,6 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
         // sizes and parameter version, enqueued on the same stream) replays it.
         const uint64_t key[4] = {a.seed, (uint64_t)(uint32_t)G.W | ((uint64_t)(uint32_t)G.H << 32),
                                  (uint64_t)(uint32_t)G.nsx | ((uint64_t)(uint32_t)G.nsy << 32), sc.version};
-        // (re)allocates a kept-list buffer of `bytes`; a backward on another stream may
-        // still read the lists being replaced
-        auto keep_buffer = [&](void *&ptr, size_t &granted, hipStream_t owner, size_t bytes) -> int {
-            if (ptr && owner != st) DVG_CHECK(hipStreamSynchronize(owner));
-            if (!ptr || granted < bytes) {
-                if (ptr) {
-                    DVG_CHECK(hipStreamSynchronize(st));
-                    cache_free(sc.device, ptr, granted);
-                    ptr = nullptr;
-                }
-                ptr = cache_alloc(sc.device, bytes, &granted);
-            }
-            return 0;
-        };
         if (a.render_image) {
             uint16_t *frag = nullptr;
             int *pkeep = nullptr;
@@ -1446,17 +1478,20 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
             sc.pkeep_valid = false;
             const size_t frag_bytes = (size_t)(kFragSlots + 1) * nsamp * sizeof(uint16_t);
             const size_t pkeep_bytes = (size_t)(kPFragSlots * kPFragWords + 1) * nsamp * sizeof(int);
+            // a backward on another stream may still read the lists being replaced
             if (!pre && frag_bytes <= kFragCacheBytes && S.ng < kFragMaxGroups) {
+                if (sc.frag && sc.frag_stream != st) DVG_CHECK(hipStreamSynchronize(sc.frag_stream));
                 void *b = sc.frag;
-                if (int e = keep_buffer(b, sc.frag_granted, sc.frag_stream, frag_bytes)) return e;
+                const bool ok = keep_buffer(sc, b, sc.frag_granted, sc.frag_stream, frag_bytes);
                 sc.frag = (uint16_t *)b;
-                frag = sc.frag;
+                frag = ok ? sc.frag : nullptr;
             }
             if (pre && pkeep_bytes <= kFragCacheBytes) {
+                if (sc.pkeep && sc.pkeep_stream != st) DVG_CHECK(hipStreamSynchronize(sc.pkeep_stream));
                 void *b = sc.pkeep;
-                if (int e = keep_buffer(b, sc.pkeep_granted, sc.pkeep_stream, pkeep_bytes)) return e;
+                const bool ok = keep_buffer(sc, b, sc.pkeep_granted, sc.pkeep_stream, pkeep_bytes);
                 sc.pkeep = (int *)b;
-                pkeep = sc.pkeep;
+                pkeep = ok ? sc.pkeep : nullptr;
             }
             {
                 DVG_PROF(st, "k_color");
@@ -1661,6 +1696,8 @@ struct Job {
     float *grad;
     int grad_n;
     int ntiles, npix, nsamp;
+    uint16_t *frag;  // the scene's kept fragment lists, or null
+    int *pkeep;      // its kept prefiltered fragments, or null
 };
 
 enum { kStPix = 0, kStTile, kStSamp, kStChunk, kStSdf, kStZero, kNumStages };
@@ -1700,7 +1737,8 @@ __global__ __launch_bounds__(kBlock, DVG_WAVES_COLOR) void kb_color(const Job *j
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     if (!J.img) return;
-    color_tile<PRE>(J.S, J.G, blockIdx.x - pref[j], J.bg, J.colors, J.err);
+    color_tile<PRE>(J.S, J.G, blockIdx.x - pref[j], J.bg, J.colors, J.err, PRE ? nullptr : J.frag,
+                    PRE ? J.pkeep : nullptr);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_resolve(const Job *jobs, const int *pref, int nj) {
@@ -1715,7 +1753,8 @@ __global__ __launch_bounds__(kBlock, DVG_WAVES_BATCH_COLOR_BWD) void kb_color_bw
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     if (!J.d_img) return;
-    color_bwd_tile<PRE>(J.S, J.G, blockIdx.x - pref[j], J.bg, J.weight, J.d_img, J.d_bg, J.d_tr, J.err);
+    color_bwd_tile<PRE>(J.S, J.G, blockIdx.x - pref[j], J.bg, J.weight, J.d_img, J.d_bg, J.d_tr, J.err,
+                        PRE ? nullptr : J.frag, PRE ? J.pkeep : nullptr);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_edge_sample(const Job *jobs, const int *pref, int nj) {
@@ -1776,6 +1815,7 @@ struct BatchHost {
     void *ws = nullptr;            // per-job scratch
     size_t ws_bytes = 0, ws_granted = 0;
     hipStream_t last_stream = nullptr;
+    std::vector<SceneHost *> keep_holds;  // one entry per job holding its scene's kept lists
 };
 
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1846,6 +1886,34 @@ int batch_create(SceneHost *const *scenes, const dvg_render_args *args, int n, i
         J.nsamp = J.npix * J.G.spp;
         J.ntiles = J.G.tiles_x * J.G.tiles_y;
         if (cls[i] == 2) continue;
+        // kept fragment lists: a forward job writes them, a backward job replays them when
+        // the key the forward wrote matches its own (checked on the device)
+        {
+            const size_t ns = (size_t)J.nsamp;
+            if (!J.pre) {
+                const size_t bytes = (size_t)(kFragSlots + 1) * ns * sizeof(uint16_t);
+                if (bytes <= kFragCacheBytes && sc.dev.ng < kFragMaxGroups) {
+                    void *b = sc.frag;
+                    const bool ok = J.img ? keep_buffer(sc, b, sc.frag_granted, sc.frag_stream, bytes)
+                                          : (b && sc.frag_granted >= bytes);
+                    sc.frag = (uint16_t *)b;
+                    J.frag = ok ? sc.frag : nullptr;
+                }
+            } else {
+                const size_t bytes = (size_t)(kPFragSlots * kPFragWords + 1) * ns * sizeof(int);
+                if (bytes <= kFragCacheBytes) {
+                    void *b = sc.pkeep;
+                    const bool ok = J.img ? keep_buffer(sc, b, sc.pkeep_granted, sc.pkeep_stream, bytes)
+                                          : (b && sc.pkeep_granted >= bytes);
+                    sc.pkeep = (int *)b;
+                    J.pkeep = ok ? sc.pkeep : nullptr;
+                }
+            }
+            if (J.frag || J.pkeep) {
+                sc.keep_refs++;
+                B->keep_holds.push_back(&sc);
+            }
+        }
         o_weight[k] = ws, ws += al256((size_t)J.npix * 4);
         if (J.img) o_colors[k] = ws, ws += al256((size_t)J.nsamp * 16);
         if (J.d_img && !J.pre) {
@@ -2035,6 +2103,7 @@ void batch_destroy(BatchHost *B) {
     (void)hipStreamSynchronize(B->last_stream);
     if (B->dev_block) cache_free(B->device, B->dev_block, B->dev_granted);
     if (B->ws) cache_free(B->device, B->ws, B->ws_granted);
+    for (SceneHost *sc : B->keep_holds) sc->keep_refs--;
     if (prev >= 0) (void)hipSetDevice(prev);
     delete B;
 }
@@ -2051,7 +2120,10 @@ struct RefitArgs {
 };
 
 __global__ void k_refit_clear(int *uerr) {
-    if (threadIdx.x == 0) uerr[0] = 0;
+    if (threadIdx.x == 0) {
+        uerr[0] = 0;
+        uerr[1] += 1;  // the parameter version (err[2]): kept fragment lists go stale
+    }
 }
 
 template <class T>

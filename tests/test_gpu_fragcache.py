@@ -93,3 +93,52 @@ def test_lists_are_dropped_by_a_rebuild():
     after = a.render(w, h, 2, 2, 7, d_image=d, translation=True)
     fresh = GpuRunner(f2).render(w, h, 2, 2, 7, d_image=d, translation=True)
     _same(after, fresh, 'rebuild')
+
+
+def _batch_grads(sc, w, h, seed, d, prefilter, forward_seed=None):
+    """Batched backward on `sc` (after a batched forward with `forward_seed`, if given)."""
+    from diffvg_amd.batch import Batch
+    dev = torch.device('cuda:0')
+    if forward_seed is not None:
+        img = torch.zeros(h, w, 4, device=dev)
+        fb = Batch([(sc, dict(width=w, height=h, nsx=2, nsy=2, seed=forward_seed, render_image=img,
+                              use_prefiltering=prefilter))], zero_grads=False)
+        fb.render(None)
+    dt = torch.zeros(h, w, 2, device=dev)
+    di = torch.from_numpy(d).to(dev)
+    bb = Batch([(sc, dict(width=w, height=h, nsx=2, nsy=2, seed=seed, d_render_image=di, d_translation=dt,
+                          use_prefiltering=prefilter))], zero_grads=True)
+    bb.render(None)
+    torch.cuda.synchronize()
+    bb.status()
+    return dict(grads=sc.grads(), d_translation=dt.cpu().numpy())
+
+
+@pytest.mark.parametrize('prefilter', [False, True])
+def test_batched_backward_replays_only_matching_lists(prefilter):
+    """Batches keep lists too (graph replays included, as the batch tests check); the key
+    a forward wrote is checked on the device: another seed, or a device update of the
+    parameters after the forward, makes the backward shade."""
+    from diffvg_amd import _lib
+    print('fragcache batch', prefilter, flush=True)
+    f1 = _flat('layers_fill')
+    w, h = f1['canvas_width'], f1['canvas_height']
+    d = np.random.default_rng(9).uniform(-1, 1, (h, w, 4)).astype(np.float32)
+    fresh = _batch_grads(_lib.DeviceScene(f1, 0), w, h, 5, d, prefilter)
+    kept = _batch_grads(_lib.DeviceScene(f1, 0), w, h, 5, d, prefilter, forward_seed=5)
+    _same(kept, fresh, 'batch kept')
+    other = _batch_grads(_lib.DeviceScene(f1, 0), w, h, 5, d, prefilter, forward_seed=6)
+    _same(other, fresh, 'batch other seed')
+    # forward, then a device update, then the backward: the lists are stale
+    f2 = dict(f1)
+    f2['shape_params'] = (f1['shape_params'] * 1.15).astype(np.float32)
+    from diffvg_amd.batch import Batch
+    sc = _lib.DeviceScene(f1, 0)
+    img = torch.zeros(h, w, 4, device='cuda')
+    Batch([(sc, dict(width=w, height=h, nsx=2, nsy=2, seed=5, render_image=img, use_prefiltering=prefilter))],
+          zero_grads=False).render(None)
+    sc.update(f2['filter_radius'], None, shape_params=torch.from_numpy(f2['shape_params']).cuda())
+    sc.status()
+    after = _batch_grads(sc, w, h, 5, d, prefilter)
+    fresh2 = _batch_grads(_lib.DeviceScene(f2, 0), w, h, 5, d, prefilter)
+    _same(after, fresh2, 'batch after update')
