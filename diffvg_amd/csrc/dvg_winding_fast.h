@@ -121,9 +121,12 @@ DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float
     for (int i = 0; i + 1 < nb; i++) {
         const float fa = fv[i], fb = fv[i + 1];
         if ((fa > 0.f) == (fb > 0.f)) continue;  // no crossing on this monotone piece
-        // bisection to a bracket of width (b - a) / 65536
+        // Bisection, at most to a bracket of width (b - a) / 65536, stopping as soon as
+        // y' and x(t) - pt.x have certain signs over the bracket (checked after 3, 6, 9,
+        // 12 and 16 halvings): a crossing far from pt is decided after a few halvings.
         float lo = br[i], hi = br[i + 1];
         const bool rising = fb > 0.f;
+        int xs = 0;
         for (int it = 0; it < 16; it++) {
             float mid = 0.5f * (lo + hi);
             float fm = bern3(c0, c1, c2, c3, mid);
@@ -131,20 +134,24 @@ DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float
                 hi = mid;
             else
                 lo = mid;
+            if (!((0x8924u >> it) & 1u)) continue;  // it = 2, 5, 8, 11, 15
+            // over the bracket, each value lies within (derivative bound) x (half width)
+            // of its value at the midpoint
+            const float m = 0.5f * (lo + hi), half = 0.5f * (hi - lo);
+            const float yd = 3.f * bern2(d0, d1, d2, m);
+            if ((yd > 0.f) != rising || !(fabsf(yd) - 3.f * ddmax * half > ed)) continue;
+            const float xm = bern3(p0x, p1x, p2x, p3x, m) - ptx, xv = xdmax * half + ex;
+            if (xm - xv > 0.f) {
+                xs = 1;
+                break;
+            }
+            if (xm + xv < 0.f) {
+                xs = -1;
+                break;
+            }
         }
-        const float width = hi - lo;
-        // |y'| over the bracket: value at the ends minus the largest change across it
-        float ylo = 3.f * bern2(d0, d1, d2, lo), yhi = 3.f * bern2(d0, d1, d2, hi);
-        float var = 3.f * ddmax * width;
-        if ((ylo > 0.f) != rising || (yhi > 0.f) != rising) return false;
-        if (!(fminf(fabsf(ylo), fabsf(yhi)) - var > ed)) return false;
-        // x(t) - pt.x over the bracket
-        float xlo = bern3(p0x, p1x, p2x, p3x, lo) - ptx, xhi = bern3(p0x, p1x, p2x, p3x, hi) - ptx;
-        float xvar = xdmax * width + ex;
-        if (fminf(xlo, xhi) - xvar > 0.f)
-            acc += rising ? 1 : -1;
-        else if (!(fmaxf(xlo, xhi) + xvar < 0.f))
-            return false;
+        if (xs == 0) return false;
+        if (xs > 0) acc += rising ? 1 : -1;
     }
     w = acc;
     return true;
@@ -156,13 +163,20 @@ DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float
 // at least rmax + a rounding margin away from pt, so that test certainly rejects: the
 // curve lies within (h^2 / 8) max|B''| of its chords over 8 equal steps of t, with
 // |B''| <= 6 max|second differences of the control points| per coordinate.
-DVG_WF float seg_dist(float ax, float ay, float bx, float by, float px, float py) {
-    float dx = bx - ax, dy = by - ay;
-    float l2 = dx * dx + dy * dy;
-    float t = l2 > 0.f ? ((px - ax) * dx + (py - ay) * dy) / l2 : 0.f;
-    t = fminf(fmaxf(t, 0.f), 1.f);
-    float qx = ax + t * dx - px, qy = ay + t * dy - py;
-    return sqrtf(qx * qx + qy * qy);
+// Whether the segment ab is farther than `need` (>= 0) from p, compared in squares with
+// no division: the end points when p projects outside the segment, else the squared
+// cross product against need^2 |b - a|^2.
+DVG_WF bool seg_farther(float ax, float ay, float bx, float by, float px, float py, float need) {
+    const float dx = bx - ax, dy = by - ay, wx = px - ax, wy = py - ay;
+    const float l2 = dx * dx + dy * dy, dt = wx * dx + wy * dy;
+    const float n2 = need * need;
+    if (!(dt > 0.f) || !(l2 > 0.f)) return wx * wx + wy * wy > n2;
+    if (dt >= l2) {
+        const float ux = px - bx, uy = py - by;
+        return ux * ux + uy * uy > n2;
+    }
+    const float cr = wx * dy - wy * dx;
+    return cr * cr > n2 * l2;
 }
 
 DVG_WF bool cubic_certainly_farther(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float p3x,
@@ -179,7 +193,7 @@ DVG_WF bool cubic_certainly_farther(float p0x, float p0y, float p1x, float p1y, 
     for (int k = 1; k <= 8; k++) {
         float t = 0.125f * (float)k;
         float bx = bern3(p0x, p1x, p2x, p3x, t), by = bern3(p0y, p1y, p2y, p3y, t);
-        if (!(seg_dist(ax, ay, bx, by, ptx, pty) > need)) return false;
+        if (!seg_farther(ax, ay, bx, by, ptx, pty, need)) return false;
         ax = bx;
         ay = by;
     }
