@@ -60,6 +60,11 @@ constexpr int kPFragWords = 9;
 #ifndef DVG_WAVES_EDGE
 #define DVG_WAVES_EDGE 6
 #endif
+// the batched edge kernel (job lookup and per-job fields live across its loop) measured
+// fastest at 5 (256 uneven scenes: 31.1 ms at 5, 33.1 at 4, 46.2 at 6, 46.6 at 7)
+#ifndef DVG_WAVES_BATCH_EDGE
+#define DVG_WAVES_BATCH_EDGE 5
+#endif
 #ifndef DVG_WAVES_SDF
 #define DVG_WAVES_SDF 5
 #endif
@@ -1779,7 +1784,7 @@ __global__ __launch_bounds__(kBlock) void kb_edge_bin(const Job *jobs, const int
                  J.edge_sorted);
 }
 
-__global__ __launch_bounds__(kBlock, DVG_WAVES_EDGE) void kb_edge(const Job *jobs, const int *pref, int nj) {
+__global__ __launch_bounds__(kBlock, DVG_WAVES_BATCH_EDGE) void kb_edge(const Job *jobs, const int *pref, int nj) {
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     if (!J.d_img) return;

@@ -45,6 +45,35 @@ DVG_WF float bern2(float d0, float d1, float d2, float t) {
 
 DVG_WF float max4(float a, float b, float c, float d) { return fmaxf(fmaxf(a, b), fmaxf(c, d)); }
 
+// N halvings of the bracket [lo, hi] of a root of the cubic with control values c0..c3
+// (rising: the cubic goes from negative to positive over the bracket).
+template <int N>
+DVG_WF void halve(float c0, float c1, float c2, float c3, bool rising, float &lo, float &hi) {
+#pragma unroll
+    for (int it = 0; it < N; it++) {
+        float mid = 0.5f * (lo + hi);
+        float fm = bern3(c0, c1, c2, c3, mid);
+        if ((fm > 0.f) == rising)
+            hi = mid;
+        else
+            lo = mid;
+    }
+}
+
+// Over the bracket [lo, hi] of a crossing, each value lies within (derivative bound) x
+// (half width) of its value at the midpoint: returns +1 / -1 when y' keeps the crossing's
+// sign with margin ed and x(t) - pt.x is certainly positive / negative (margin ex), else 0.
+DVG_WF int crossing_side(float p0x, float p1x, float p2x, float p3x, float d0, float d1, float d2, float ptx,
+                         bool rising, float lo, float hi, float ddmax, float xdmax, float ed, float ex) {
+    const float m = 0.5f * (lo + hi), half = 0.5f * (hi - lo);
+    const float yd = 3.f * bern2(d0, d1, d2, m);
+    if ((yd > 0.f) != rising || !(fabsf(yd) - 3.f * ddmax * half > ed)) return 0;
+    const float xm = bern3(p0x, p1x, p2x, p3x, m) - ptx, xv = xdmax * half + ex;
+    if (xm - xv > 0.f) return 1;
+    if (xm + xv < 0.f) return -1;
+    return 0;
+}
+
 // Returns true and sets `w` when the contribution is decided.
 DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float p3x,
                                float p3y, float ptx, float pty, int &w) {
@@ -55,7 +84,11 @@ DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float
     // trigonometric branch loses precision as |b / a| grows; below |a| = 1e-6 it solves
     // the quadratic instead.
     const float ra = -p0y + 3 * p1y - 3 * p2y + p3y, rb = 3 * p0y - 6 * p1y + 3 * p2y, rc = -3 * p0y + 3 * p1y;
-    const float E = fabsf(ra) < 1e-6f ? 1e-9f : 2e-7f * (fabsf(rb) / fabsf(ra) + sqrtf(fabsf(rc) / fabsf(ra)) + 1.f);
+    float E = 1e-9f;
+    if (!(fabsf(ra) < 1e-6f)) {
+        const float ia = 1.f / fabsf(ra);
+        E = 2e-7f * (fabsf(rb) * ia + sqrtf(fabsf(rc) * ia) + 1.f);
+    }
     if (!(E < 1e-3f)) return false;
     const float ddy = fmaxf(fmaxf(fabsf(p1y - p0y), fabsf(p2y - p1y)), fabsf(p3y - p2y));
     const float ddx = fmaxf(fmaxf(fabsf(p1x - p0x), fabsf(p2x - p1x)), fabsf(p3x - p2x));
@@ -80,7 +113,10 @@ DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float
     float br[4];
     int nb = 0;
     br[nb++] = 0.f;
-    {
+    // y is monotone on [0, 1] when its control differences share a strict sign (y'/3 is
+    // their Bernstein combination): no interior critical point to find
+    const bool mono = (d0 > 0.f && d1 > 0.f && d2 > 0.f) || (d0 < 0.f && d1 < 0.f && d2 < 0.f);
+    if (!mono) {
         float r0 = -1.f, r1 = -1.f;
         const float sc = fabsf(qa) + fabsf(qb) + fabsf(qc);
         if (fabsf(qa) > 1e-6f * sc) {
@@ -121,34 +157,15 @@ DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float
     for (int i = 0; i + 1 < nb; i++) {
         const float fa = fv[i], fb = fv[i + 1];
         if ((fa > 0.f) == (fb > 0.f)) continue;  // no crossing on this monotone piece
-        // Bisection, at most to a bracket of width (b - a) / 65536, stopping as soon as
-        // y' and x(t) - pt.x have certain signs over the bracket (checked after 3, 6, 9,
-        // 12 and 16 halvings): a crossing far from pt is decided after a few halvings.
+        // Bisection to a bracket of width (b - a) / 65536 at most, in two unrolled runs:
+        // after the first 4 halvings a crossing far from pt is usually decided already.
         float lo = br[i], hi = br[i + 1];
         const bool rising = fb > 0.f;
-        int xs = 0;
-        for (int it = 0; it < 16; it++) {
-            float mid = 0.5f * (lo + hi);
-            float fm = bern3(c0, c1, c2, c3, mid);
-            if ((fm > 0.f) == rising)
-                hi = mid;
-            else
-                lo = mid;
-            if (!((0x8924u >> it) & 1u)) continue;  // it = 2, 5, 8, 11, 15
-            // over the bracket, each value lies within (derivative bound) x (half width)
-            // of its value at the midpoint
-            const float m = 0.5f * (lo + hi), half = 0.5f * (hi - lo);
-            const float yd = 3.f * bern2(d0, d1, d2, m);
-            if ((yd > 0.f) != rising || !(fabsf(yd) - 3.f * ddmax * half > ed)) continue;
-            const float xm = bern3(p0x, p1x, p2x, p3x, m) - ptx, xv = xdmax * half + ex;
-            if (xm - xv > 0.f) {
-                xs = 1;
-                break;
-            }
-            if (xm + xv < 0.f) {
-                xs = -1;
-                break;
-            }
+        halve<4>(c0, c1, c2, c3, rising, lo, hi);
+        int xs = crossing_side(p0x, p1x, p2x, p3x, d0, d1, d2, ptx, rising, lo, hi, ddmax, xdmax, ed, ex);
+        if (xs == 0) {
+            halve<12>(c0, c1, c2, c3, rising, lo, hi);
+            xs = crossing_side(p0x, p1x, p2x, p3x, d0, d1, d2, ptx, rising, lo, hi, ddmax, xdmax, ed, ex);
         }
         if (xs == 0) return false;
         if (xs > 0) acc += rising ? 1 : -1;

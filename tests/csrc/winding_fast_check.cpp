@@ -16,7 +16,8 @@ thread_local Counters *g_tc = nullptr;
 extern "C" {
 // mode 0: random curves and points; 1: points on control-point y values (ray through a
 // joint); 2: near-tangent rays; 3: large coordinates; 4: nearly flat curves; 5: points in
-// the control box (what reaches the test after the ray-box culling).
+// the control box (what reaches the test after the ray-box culling); 6: both sides of a
+// boundary sample; 7: straight segments written as cubics.
 // out[0] = decided, out[1] = mismatches, out[2] = total; first mismatch in bad[0..9]
 void wf_fuzz(int mode, int64_t n, uint64_t seed, int64_t *out, float *bad) {
     std::mt19937_64 g(seed);
@@ -51,6 +52,14 @@ void wf_fuzz(int mode, int64_t n, uint64_t seed, int64_t *out, float *bad) {
             float l = std::sqrt(dx * dx + dy * dy) + 1e-30f;
             float side = U(g) < 0.5f ? 1.f : -1.f;
             pt = orc::V2{x - side * 0.0512f * dy / l, y + side * 0.0512f * dx / l};
+        }
+        if (mode == 7) {
+            // a straight segment written as a cubic (control points at thirds), as SVG
+            // writers emit lines inside curve paths; the point anywhere near its box
+            for (int k = 1; k < 3; k++)
+                p[k] = orc::V2{p[0].x + (p[3].x - p[0].x) * (float)k / 3.f, p[0].y + (p[3].y - p[0].y) * (float)k / 3.f};
+            pt = orc::V2{std::fmin(p[0].x, p[3].x) - sp + (std::fabs(p[3].x - p[0].x) + 2.f * sp) * U(g),
+                         std::fmin(p[0].y, p[3].y) + std::fabs(p[3].y - p[0].y) * U(g)};
         }
         if (mode == 2) {
             // the y of an interior extremum of the curve, nudged by a few ulps

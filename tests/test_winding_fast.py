@@ -1,7 +1,8 @@
 """The renderer's fast cubic winding and stroke-rejection paths (diffvg_amd/csrc/dvg_winding_fast.h) against the
 oracle's restatement of the reference's double-precision solve: on random and adversarial
 segments (rays through control-point y values, near-tangent rays, large coordinates,
-nearly flat curves, points inside the control box, both sides of boundary points as the
+nearly flat curves, straight segments written as cubics, points inside the control box,
+both sides of boundary points as the
 edge kernel shades them) every decision it takes must equal the reference's.  Built for the host from the same header the GPU kernels compile."""
 import ctypes
 import os
@@ -11,7 +12,7 @@ import pytest
 SO = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'csrc', 'libwinding_fast_check.so')
 
 
-@pytest.mark.parametrize('mode, min_decided', [(0, 0.99), (1, 0.5), (2, 0.5), (3, 0.99), (4, 0.9), (5, 0.99),
+@pytest.mark.parametrize('mode, min_decided', [(0, 0.99), (1, 0.5), (2, 0.5), (3, 0.99), (4, 0.9), (5, 0.99), (7, 0.99),
                                                (6, 0.99)])
 def test_fast_cubic_winding_never_disagrees(mode, min_decided):
     if not os.path.exists(SO):
