@@ -62,16 +62,21 @@ constexpr int kPFragWords = 9;
 #endif
 // the batched edge kernel (job lookup and per-job fields live across its loop) measured
 // fastest at 5 (256 uneven scenes: 31.1 ms at 5, 33.1 at 4, 46.2 at 6, 46.6 at 7)
+// the batched forward (job lookup live across its loop) measured fastest at 4 (256 uneven
+// scenes: 12.7 ms at 4, 14.1 at 5, 24.7 at 6)
+#ifndef DVG_WAVES_BATCH_COLOR
+#define DVG_WAVES_BATCH_COLOR 4
+#endif
 #ifndef DVG_WAVES_BATCH_EDGE
 #define DVG_WAVES_BATCH_EDGE 5
 #endif
 #ifndef DVG_WAVES_SDF
 #define DVG_WAVES_SDF 5
 #endif
-// the batched backward (its job lookup adds live state) measured fastest at 2 waves
-// (256 uneven scenes: 29.1 ms at 2 waves, 38.0 ms at 5)
+// the batched backward (job lookup live across its loop; it mostly replays kept fragment
+// lists) measured fastest at 4 (256 uneven scenes: 15.0 ms at 4, 16.3 at 3, 18.1 at 2)
 #ifndef DVG_WAVES_BATCH_COLOR_BWD
-#define DVG_WAVES_BATCH_COLOR_BWD 2
+#define DVG_WAVES_BATCH_COLOR_BWD 4
 #endif
 
 // Render geometry shared by the kernels.
@@ -1738,7 +1743,7 @@ __global__ __launch_bounds__(kBlock) void kb_weight(const Job *jobs, const int *
 }
 
 template <int PRE>
-__global__ __launch_bounds__(kBlock, DVG_WAVES_COLOR) void kb_color(const Job *jobs, const int *pref, int nj) {
+__global__ __launch_bounds__(kBlock, DVG_WAVES_BATCH_COLOR) void kb_color(const Job *jobs, const int *pref, int nj) {
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     if (!J.img) return;
