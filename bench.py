@@ -567,8 +567,9 @@ def main():
         cpu_sec = cpu['fwd_s'] + cpu['bwd_s']
         cpu_baseline = {'value': round(cpu['samples'] / cpu_sec / 1e6, 4), 'unit': 'Msamples/s',
                         'cores': cpu['cores'], 'kind': cpu['kind'],
-                        'sample': 'the full %dx%d %dx%d-spp scene, one forward and one backward (edge sampling), '
-                                  'tent r=1 twin of the Gaussian filter' % (w, h, nsx, nsy)}
+                        'sample': 'the full %dx%d %dx%d-spp scene, one forward and one backward (edge sampling), %s'
+                                  % (w, h, nsx, nsy, 'tent r=1 twin of the Gaussian filter' if a.config == 2
+                                     else 'the same box r=0.5 filter')}
     out = {
         'metric': 'fwd+bwd Msamples/s on 1 GPU; p50/p99 latency, batched/served vs plain',
         'value': round(value, 4), 'unit': 'Msamples/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
