@@ -57,9 +57,10 @@ constexpr int kPFragWords = 9;
 #ifndef DVG_WAVES_COLOR_BWD
 #define DVG_WAVES_COLOR_BWD 4
 #endif
-// the prefiltered instantiations keep more state per lane: 3 waves measured fastest for
-// the single-scene kernels (300-path step with prefiltering: 6.01 / 5.93 ms against 6.2 at
-// 5 / 4), 5 for the batched forward (serving stream: 1.91 ms at 5, 2.16 at 4)
+// the prefiltered instantiations keep more state per lane: on the 300-path scene
+// k_color<1> takes 3.69 ms at 3 waves, 3.75 at 5, and k_color_bwd<1> 1.71 at 3, 1.69 at 4
+// (the same within run-to-run spread); the batched forward stays at 5 (serving stream:
+// 1.91 ms at 5, 2.16 at 4)
 #ifndef DVG_WAVES_COLOR_PRE
 #define DVG_WAVES_COLOR_PRE 3
 #endif
