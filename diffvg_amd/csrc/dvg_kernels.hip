@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <climits>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -783,6 +784,7 @@ DVG_DEV void edge_chunk(const DevScene &S, const Geom &G, int chunk, const EdgeS
                         const float *weight, const float *d_img, float *d_translation) {
     __shared__ int s_list[kCandCap];
     __shared__ int s_w[kBlock / 64];
+    __shared__ int s_perm[kEdgeChunk];
     if (chunk >= chunk_off[ntiles]) return;
     // tile = the last t with chunk_off[t] <= chunk (chunk_off is non-decreasing)
     int lo = 0, hi = ntiles - 1;
@@ -800,11 +802,32 @@ DVG_DEV void edge_chunk(const DevScene &S, const Geom &G, int chunk, const EdgeS
     float4 tb = tile_box(S, G, tx, ty, ext);
     int off = tile_off[tile];
     bool has_bg = bg != nullptr;
+    const int b0 = (chunk - chunk_off[tile]) * kEdgeChunk;
+    {
+        // The chunk's samples in order of their shape (stable): samples of one boundary
+        // share a wave, so fewer waves run the deep part of the winding test for a curve
+        // close to one of their points, and the gradient scatter combines more lanes.
+        __shared__ int s_key[kEdgeChunk];
+        const int nvalid = min(kEdgeChunk, cnt - b0);
+        if (threadIdx.x < kEdgeChunk)
+            s_key[threadIdx.x] = threadIdx.x < nvalid ? edges[sorted[off + b0 + threadIdx.x]].shape : INT_MAX;
+        __syncthreads();
+        int rank = 0;
+        if (threadIdx.x < kEdgeChunk) {
+            const int my = s_key[threadIdx.x];
+            for (int j = 0; j < kEdgeChunk; j++) {
+                const int o = s_key[j];
+                rank += (o < my || (o == my && j < (int)threadIdx.x)) ? 1 : 0;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < kEdgeChunk) s_perm[rank] = threadIdx.x;
+        __syncthreads();
+    }
     {
         // lanes 2i and 2i + 1 shade the inside and the outside point of sample i
         const int side = threadIdx.x & 1;
-        const int b0 = (chunk - chunk_off[tile]) * kEdgeChunk;
-        int k = b0 + (threadIdx.x >> 1);
+        int k = b0 + s_perm[threadIdx.x >> 1];
         bool valid = k < cnt;
         const int eidx = valid ? sorted[off + k] : 0;
         V2 p{0, 0};
