@@ -1893,6 +1893,11 @@ int batch_create(SceneHost *const *scenes, const dvg_render_args *args, int n, i
         if (color && eval) return set_error("eval_positions are supported for the SDF output only");
         cls[i] = sdf ? 2 : (a.use_prefiltering ? 1 : 0);
     }
+    // colour jobs per scene: a scene in more than one colour job of the batch keeps no
+    // fragment lists (its jobs would write one buffer at once)
+    std::map<const SceneHost *, int> color_jobs;
+    for (int i = 0; i < n; i++)
+        if (cls[i] != 2) color_jobs[scenes[i]]++;
     for (int c = 0; c < 3; c++)
         for (int i = 0; i < n; i++)
             if (cls[i] == c) B->order.push_back(i);
@@ -1934,7 +1939,7 @@ int batch_create(SceneHost *const *scenes, const dvg_render_args *args, int n, i
         if (cls[i] == 2) continue;
         // kept fragment lists: a forward job writes them, a backward job replays them when
         // the key the forward wrote matches its own (checked on the device)
-        {
+        if (color_jobs[&sc] == 1) {
             const size_t ns = (size_t)J.nsamp;
             if (!J.pre) {
                 const size_t bytes = (size_t)(kFragSlots + 1) * ns * sizeof(uint16_t);

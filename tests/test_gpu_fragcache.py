@@ -142,3 +142,23 @@ def test_batched_backward_replays_only_matching_lists(prefilter):
     after = _batch_grads(sc, w, h, 5, d, prefilter)
     fresh2 = _batch_grads(_lib.DeviceScene(f2, 0), w, h, 5, d, prefilter)
     _same(after, fresh2, 'batch after update')
+
+
+@pytest.mark.parametrize('prefilter', [False, True])
+def test_scene_twice_in_one_batch_keeps_no_lists(prefilter):
+    """Two forward jobs of one scene in a batch would write one list buffer at once: such a
+    scene keeps no lists, and a backward after that batch equals a fresh one."""
+    from diffvg_amd import _lib
+    from diffvg_amd.batch import Batch
+    print('fragcache scene twice in a batch', prefilter, flush=True)
+    f1 = _flat('layers_fill')
+    w, h = f1['canvas_width'], f1['canvas_height']
+    d = np.random.default_rng(11).uniform(-1, 1, (h, w, 4)).astype(np.float32)
+    fresh = _batch_grads(_lib.DeviceScene(f1, 0), w, h, 5, d, prefilter)
+    sc = _lib.DeviceScene(f1, 0)
+    imgs = [torch.zeros(h, w, 4, device='cuda') for _ in range(2)]
+    Batch([(sc, dict(width=w, height=h, nsx=2, nsy=2, seed=s, render_image=im, use_prefiltering=prefilter))
+           for s, im in ((6, imgs[0]), (5, imgs[1]))], zero_grads=False).render(None)
+    torch.cuda.synchronize()
+    after = _batch_grads(sc, w, h, 5, d, prefilter)
+    _same(after, fresh, 'scene twice')
