@@ -494,7 +494,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
                o_params = P.add(params), o_sb = P.add(seg_begin), o_nsegs = P.add(nsegs), o_ptoff = P.add(pt_off),
                o_npt = P.add(npt), o_thick_flag = P.add(has_thick), o_closed = P.add(closed), o_approx = P.add(approx),
                o_sbox = P.add(shape_box), o_sidx = P.add(seg_idx), o_skind = P.add(seg_kind), o_spid = P.add(seg_pid),
-               o_sbx = P.add(seg_box), o_smr = P.add(seg_maxr), o_sdfs = P.add(seg_dfs), o_scdf = P.add(seg_cdf),
+               o_sbx = P.add(seg_box), o_smr = P.add(seg_maxr), o_swf = P.add(std::vector<float4>(3 * (size_t)nseg)), o_sdfs = P.add(seg_dfs), o_scdf = P.add(seg_cdf),
                o_spmf = P.add(seg_pmf), o_pts = P.add(points), o_th = P.add(thickness), o_goff = P.add(g_off),
                o_gn = P.add(g_n), o_gids = P.add(g_ids), o_ft = P.add(fill_type), o_fo = P.add(fill_off),
                o_fn = P.add(fill_ns), o_st = P.add(stroke_type), o_so = P.add(stroke_off), o_sn = P.add(stroke_ns),
@@ -540,6 +540,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
         D.seg_pid = (const int *)(B + o_spid);
         D.seg_box = (const float4 *)(B + o_sbx);
         D.seg_maxr = (const float *)(B + o_smr);
+        D.seg_wf = (const float4 *)(B + o_swf);
         D.seg_dfs = (const int *)(B + o_sdfs);
         D.seg_cdf = (const float *)(B + o_scdf);
         D.seg_pmf = (const float *)(B + o_spmf);
@@ -587,6 +588,9 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     D.npts = d.num_points_total;
     D.filter_type = d.filter_type;
     D.filter_radius = d.filter_radius;
+    if (ns > 0 && nseg > 0) {
+        if (int e = prep_segments(sc, nullptr)) return e;
+    }
     // gradient block: params | points | thickness | stroke width | colours | xforms | filter radius
     auto al = [](size_t n) { return (n + 63) & ~size_t(63); };
     sc.g_params = 0;

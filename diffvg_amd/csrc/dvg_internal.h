@@ -19,6 +19,10 @@ int set_error(const std::string &msg);  // records the message, returns 1
 // for the caller, independent of work in flight on other streams).
 int upload(int device, void *dst, const void *src, size_t bytes);
 int clear_device(int device, void *dst, size_t bytes);
+// Fills the scene's per-segment tables that derive from its points (the cubic winding
+// prep) on stream `st` (NULL: synchronously).
+struct SceneHost;
+int prep_segments(SceneHost &sc, hipStream_t st);
 
 // Per-scene scratch reused across renders of the same or smaller size.
 struct Workspace {

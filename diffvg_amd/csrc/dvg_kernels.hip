@@ -2486,6 +2486,33 @@ __global__ __launch_bounds__(kBlock) void k_refit_cdf(RefitArgs A) {
     }
 }
 
+// The segment-only part of the fast cubic winding test (wf::cubic_prep), one lane per
+// segment, from the scene's current points.
+__global__ __launch_bounds__(kBlock) void k_seg_prep(DevScene S) {
+    const int seg = blockIdx.x * blockDim.x + threadIdx.x;
+    if (seg >= S.nseg) return;
+    float4 *dst = const_cast<float4 *>(S.seg_wf) + 3 * (size_t)seg;
+    if (S.seg_kind[seg] != 2) {
+        dst[0] = dst[1] = dst[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const int4 idx = S.seg_idx[seg];
+    const float2 a = S.points[idx.x], b = S.points[idx.y], c = S.points[idx.z], d = S.points[idx.w];
+    const wf::CubicPrep q = wf::cubic_prep(a.x, a.y, b.x, b.y, c.x, c.y, d.x, d.y);
+    dst[0] = make_float4(q.my, q.mx, q.E, q.ddy);
+    dst[1] = make_float4(q.ddx, q.ddmax, q.xdmax, q.br1);
+    dst[2] = make_float4(q.br2, 0.f, 0.f, 0.f);
+}
+
+int prep_segments(SceneHost &sc, hipStream_t st) {
+    const DevScene &D = sc.dev;
+    if (D.nseg <= 0) return 0;
+    k_seg_prep<<<(D.nseg + kBlock - 1) / kBlock, kBlock, 0, st>>>(D);
+    DVG_CHECK(hipGetLastError());
+    if (!st) DVG_CHECK(hipStreamSynchronize(nullptr));
+    return 0;
+}
+
 int update_scene(SceneHost &sc, const dvg_scene_params &p, float filter_radius, hipStream_t st) {
     sc.version++;
     sc.frag_valid = false;
@@ -2510,6 +2537,7 @@ int update_scene(SceneHost &sc, const dvg_scene_params &p, float filter_radius, 
         k_refit_shapes<<<(D.ns + kRefitWaves - 1) / kRefitWaves, kBlock, 0, st>>>(A);
         if (D.ng > 0) k_refit_groups<<<(D.ng + kBlock - 1) / kBlock, kBlock, 0, st>>>(A);
         k_refit_cdf<<<1, kBlock, 0, st>>>(A);
+        if (D.nseg > 0) k_seg_prep<<<(D.nseg + kBlock - 1) / kBlock, kBlock, 0, st>>>(D);
     }
     DVG_CHECK(hipGetLastError());
     sc.top_stale = true;

@@ -52,7 +52,8 @@ DVG_DEV V2 ceval(V2 p0, V2 p1, V2 p2, V2 p3, float t) {
 }
 
 // Winding contribution of one segment (winding_number.h:58-167)
-DVG_DEV int segment_winding(int k, const V2 *p, V2 pt) {
+// `prep`: the segment's wf::CubicPrep as three float4 (cubic segments only)
+DVG_DEV int segment_winding(int k, const V2 *p, V2 pt, const float4 *prep) {
     int w = 0;
     if (k == 0) {
         V2 p0 = p[0], p1 = p[1];
@@ -80,7 +81,10 @@ DVG_DEV int segment_winding(int k, const V2 *p, V2 pt) {
         // decided in float whenever every decision of the reference is certain
         // (dvg_winding_fast.h); otherwise the reference's double solve below
         int wfast = 0;
-        if (wf::cubic_winding_fast(p0.x, p0.y, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y, pt.x, pt.y, wfast)) return wfast;
+        const float4 q0 = prep[0], q1 = prep[1], q2 = prep[2];
+        const wf::CubicPrep q{q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x};
+        if (wf::cubic_winding_fast_prep(q, p0.x, p0.y, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y, pt.x, pt.y, wfast))
+            return wfast;
         double t[3];
         double ca = (double)(-p0.y + 3 * p1.y - 3 * p2.y + p3.y), cb = (double)(3 * p0.y - 6 * p1.y + 3 * p2.y),
                cc = (double)(-3 * p0.y + 3 * p1.y), cd = (double)(p0.y - pt.y);
@@ -116,7 +120,7 @@ DVG_DEV int winding_path(const DevScene &S, int s, V2 pt) {
         V2 p[4];
         int4 idx;
         int k = seg_load(S, seg, p, idx);
-        wn += segment_winding(k, p, pt);
+        wn += segment_winding(k, p, pt, S.seg_wf + 3 * (size_t)seg);
     }
     return wn;
 }

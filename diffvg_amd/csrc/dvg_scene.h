@@ -56,6 +56,7 @@ struct DevScene {
     const int *seg_pid;        // first point id relative to the path
     const float4 *seg_box;
     const float *seg_maxr;
+    const float4 *seg_wf;      // [3 * nseg] wf::CubicPrep of each cubic segment (from its points)
     const int *seg_dfs;        // per path (indexed from seg_begin): segment ids in visit order
     const float *seg_cdf;
     const float *seg_pmf;

@@ -74,11 +74,21 @@ DVG_WF int crossing_side(float p0x, float p1x, float p2x, float p3x, float d0, f
     return 0;
 }
 
-// Returns true and sets `w` when the contribution is decided.
-DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float p3x,
-                               float p3y, float ptx, float pty, int &w) {
-    const float M = fmaxf(max4(fabsf(p0y), fabsf(p1y), fabsf(p2y), fabsf(p3y)), fabsf(pty)) + 1.f;
-    const float Mx = fmaxf(max4(fabsf(p0x), fabsf(p1x), fabsf(p2x), fabsf(p3x)), fabsf(ptx)) + 1.f;
+// The parts of the test that depend on the segment only, computed once per segment (the
+// renderer keeps them in the scene, refreshed whenever the points change).
+struct CubicPrep {
+    float my, mx;       // largest |y|, |x| of the control points
+    float E;            // root accuracy of the reference's solver; NaN: not decidable here
+    float ddy, ddx;     // largest control-point difference per coordinate
+    float ddmax, xdmax; // |y''| / 3 and |x'| bounds
+    float br1, br2;     // interior critical points of y in (0, 1), ascending; -1 when absent;
+                        // br1 NaN: y nearly constant, leave the segment to the solver
+};
+
+DVG_WF CubicPrep cubic_prep(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float p3x, float p3y) {
+    CubicPrep q;
+    q.my = max4(fabsf(p0y), fabsf(p1y), fabsf(p2y), fabsf(p3y));
+    q.mx = max4(fabsf(p0x), fabsf(p1x), fabsf(p2x), fabsf(p3x));
     // The reference's solver works on these float coefficients (winding_number.h:134-138).
     // Its roots are accurate to about E in t: for a near-quadratic cubic (|b| >> |a|) the
     // trigonometric branch loses precision as |b / a| grows; below |a| = 1e-6 it solves
@@ -89,13 +99,63 @@ DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float
         const float ia = 1.f / fabsf(ra);
         E = 2e-7f * (fabsf(rb) * ia + sqrtf(fabsf(rc) * ia) + 1.f);
     }
+    q.E = E < 1e-3f ? E : __builtin_nanf("");
+    q.ddy = fmaxf(fmaxf(fabsf(p1y - p0y), fabsf(p2y - p1y)), fabsf(p3y - p2y));
+    q.ddx = fmaxf(fmaxf(fabsf(p1x - p0x), fabsf(p2x - p1x)), fabsf(p3x - p2x));
+    // critical points of y
+    const float d0 = p1y - p0y, d1 = p2y - p1y, d2 = p3y - p2y;
+    const float qa = d0 - 2.f * d1 + d2, qb = 2.f * (d1 - d0), qc = d0;
+    q.br1 = q.br2 = -1.f;
+    // y is monotone on [0, 1] when its control differences share a strict sign (y'/3 is
+    // their Bernstein combination): no interior critical point to find
+    const bool mono = (d0 > 0.f && d1 > 0.f && d2 > 0.f) || (d0 < 0.f && d1 < 0.f && d2 < 0.f);
+    if (!mono) {
+        float r0 = -1.f, r1 = -1.f;
+        const float sc = fabsf(qa) + fabsf(qb) + fabsf(qc);
+        bool flat = false;
+        if (fabsf(qa) > 1e-6f * sc) {
+            float disc = qb * qb - 4.f * qa * qc;
+            if (disc >= 0.f) {
+                float sq = sqrtf(disc);
+                float qq = -0.5f * (qb + (qb >= 0.f ? sq : -sq));
+                r0 = qq / qa;
+                r1 = qq != 0.f ? qc / qq : -1.f;
+            }
+        } else if (fabsf(qb) > 1e-6f * sc) {
+            r0 = -qc / qb;
+        } else {
+            flat = true;  // y nearly constant: leave it to the solver
+        }
+        if (r0 > r1) {
+            float tmp = r0;
+            r0 = r1;
+            r1 = tmp;
+        }
+        float b[2] = {-1.f, -1.f};
+        int nb = 0;
+        if (r0 > 0.f && r0 < 1.f) b[nb++] = r0;
+        if (r1 > 0.f && r1 < 1.f && r1 != r0) b[nb++] = r1;
+        q.br1 = flat ? __builtin_nanf("") : b[0];
+        q.br2 = b[1];
+    }
+    q.ddmax = 2.f * fmaxf(fabsf(d1 - d0), fabsf(d2 - d1));  // |y''| / 3 <= ddmax
+    const float dx0 = p1x - p0x, dx1 = p2x - p1x, dx2 = p3x - p2x;
+    q.xdmax = 3.f * fmaxf(fmaxf(fabsf(dx0), fabsf(dx1)), fabsf(dx2));  // |x'| bound
+    return q;
+}
+
+// Returns true and sets `w` when the contribution is decided; `q` is the segment's
+// cubic_prep.
+DVG_WF bool cubic_winding_fast_prep(const CubicPrep &q, float p0x, float p0y, float p1x, float p1y, float p2x,
+                                    float p2y, float p3x, float p3y, float ptx, float pty, int &w) {
+    const float M = fmaxf(q.my, fabsf(pty)) + 1.f;
+    const float Mx = fmaxf(q.mx, fabsf(ptx)) + 1.f;
+    const float E = q.E;
     if (!(E < 1e-3f)) return false;
-    const float ddy = fmaxf(fmaxf(fabsf(p1y - p0y), fabsf(p2y - p1y)), fabsf(p3y - p2y));
-    const float ddx = fmaxf(fmaxf(fabsf(p1x - p0x), fabsf(p2x - p1x)), fabsf(p3x - p2x));
     // value margins: the reference's coefficients are within ~2e-6 M of the curve's, and
     // its roots within E of the true ones
-    const float ey = 1e-5f * M + 3.f * E * ddy;
-    const float ex = 1e-5f * Mx + 3.f * E * ddx;
+    const float ey = 1e-5f * M + 3.f * E * q.ddy;
+    const float ex = 1e-5f * Mx + 3.f * E * q.ddx;
     const float c0 = p0y - pty, c1 = p1y - pty, c2 = p2y - pty, c3 = p3y - pty;
     // hull above or below the ray
     if ((c0 > ey && c1 > ey && c2 > ey && c3 > ey) || (c0 < -ey && c1 < -ey && c2 < -ey && c3 < -ey)) {
@@ -107,39 +167,13 @@ DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float
         w = 0;
         return true;
     }
+    if (q.br1 != q.br1) return false;  // y nearly constant
     // breakpoints: 0, critical points of y in (0, 1), 1
-    const float d0 = p1y - p0y, d1 = p2y - p1y, d2 = p3y - p2y;
-    const float qa = d0 - 2.f * d1 + d2, qb = 2.f * (d1 - d0), qc = d0;
     float br[4];
     int nb = 0;
     br[nb++] = 0.f;
-    // y is monotone on [0, 1] when its control differences share a strict sign (y'/3 is
-    // their Bernstein combination): no interior critical point to find
-    const bool mono = (d0 > 0.f && d1 > 0.f && d2 > 0.f) || (d0 < 0.f && d1 < 0.f && d2 < 0.f);
-    if (!mono) {
-        float r0 = -1.f, r1 = -1.f;
-        const float sc = fabsf(qa) + fabsf(qb) + fabsf(qc);
-        if (fabsf(qa) > 1e-6f * sc) {
-            float disc = qb * qb - 4.f * qa * qc;
-            if (disc >= 0.f) {
-                float sq = sqrtf(disc);
-                float q = -0.5f * (qb + (qb >= 0.f ? sq : -sq));
-                r0 = q / qa;
-                r1 = q != 0.f ? qc / q : -1.f;
-            }
-        } else if (fabsf(qb) > 1e-6f * sc) {
-            r0 = -qc / qb;
-        } else {
-            return false;  // y nearly constant: leave it to the solver
-        }
-        if (r0 > r1) {
-            float tmp = r0;
-            r0 = r1;
-            r1 = tmp;
-        }
-        if (r0 > 0.f && r0 < 1.f) br[nb++] = r0;
-        if (r1 > 0.f && r1 < 1.f && r1 != r0) br[nb++] = r1;
-    }
+    if (q.br1 > 0.f) br[nb++] = q.br1;
+    if (q.br2 > 0.f) br[nb++] = q.br2;
     br[nb++] = 1.f;
     float fv[4];
     for (int i = 0; i < nb; i++) {
@@ -147,12 +181,9 @@ DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float
         if (!(fabsf(fv[i]) > ey)) return false;
     }
     // bounds for the bracket tests
-    const float dmax = fmaxf(fmaxf(fabsf(d0), fabsf(d1)), fabsf(d2));       // |y'| / 3 <= dmax
-    const float ddmax = 2.f * fmaxf(fabsf(d1 - d0), fabsf(d2 - d1));        // |y''| / 3 <= ddmax
-    const float dx0 = p1x - p0x, dx1 = p2x - p1x, dx2 = p3x - p2x;
-    const float xdmax = 3.f * fmaxf(fmaxf(fabsf(dx0), fabsf(dx1)), fabsf(dx2));  // |x'| bound
-    const float ed = 1e-4f * M * 3.f + 1e-6f + 6.f * E * ddmax;              // derivative margin
-    (void)dmax;
+    const float d0 = p1y - p0y, d1 = p2y - p1y, d2 = p3y - p2y;
+    const float ddmax = q.ddmax, xdmax = q.xdmax;
+    const float ed = 1e-4f * M * 3.f + 1e-6f + 6.f * E * ddmax;  // derivative margin
     int acc = 0;
     for (int i = 0; i + 1 < nb; i++) {
         const float fa = fv[i], fb = fv[i + 1];
@@ -172,6 +203,13 @@ DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float
     }
     w = acc;
     return true;
+}
+
+// Returns true and sets `w` when the contribution is decided.
+DVG_WF bool cubic_winding_fast(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float p3x,
+                               float p3y, float ptx, float pty, int &w) {
+    return cubic_winding_fast_prep(cubic_prep(p0x, p0y, p1x, p1y, p2x, p2y, p3x, p3y), p0x, p0y, p1x, p1y, p2x,
+                                   p2y, p3x, p3y, ptx, pty, w);
 }
 
 // Stroke coverage of a cubic segment (within_distance.h:119-272) accepts only when one
