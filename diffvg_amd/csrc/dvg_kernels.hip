@@ -76,10 +76,10 @@ constexpr int kPFragWords = 9;
 #endif
 // the batched edge kernel (job lookup and per-job fields live across its loop) measured
 // fastest at 5 (256 uneven scenes: 31.1 ms at 5, 33.1 at 4, 46.2 at 6, 46.6 at 7)
-// the batched forward (job lookup live across its loop) measured fastest at 4 (256 uneven
-// scenes: 12.7 ms at 4, 14.1 at 5, 24.7 at 6)
+// the batched forward (job lookup live across its loop) measured fastest at 3 (256 uneven
+// scenes: 13.2 ms at 3, 14.3 at 4, 20.7 at 5)
 #ifndef DVG_WAVES_BATCH_COLOR
-#define DVG_WAVES_BATCH_COLOR 4
+#define DVG_WAVES_BATCH_COLOR 3
 #endif
 #ifndef DVG_WAVES_BATCH_EDGE
 #define DVG_WAVES_BATCH_EDGE 5
