@@ -494,7 +494,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
                o_params = P.add(params), o_sb = P.add(seg_begin), o_nsegs = P.add(nsegs), o_ptoff = P.add(pt_off),
                o_npt = P.add(npt), o_thick_flag = P.add(has_thick), o_closed = P.add(closed), o_approx = P.add(approx),
                o_sbox = P.add(shape_box), o_sidx = P.add(seg_idx), o_skind = P.add(seg_kind), o_spid = P.add(seg_pid),
-               o_sbx = P.add(seg_box), o_smr = P.add(seg_maxr), o_swf = P.add(std::vector<float4>(3 * (size_t)nseg)), o_sdfs = P.add(seg_dfs), o_scdf = P.add(seg_cdf),
+               o_sbx = P.add(seg_box), o_smr = P.add(seg_maxr), o_swf = P.add(std::vector<float4>(kSegPrep * (size_t)nseg)), o_sdfs = P.add(seg_dfs), o_scdf = P.add(seg_cdf),
                o_spmf = P.add(seg_pmf), o_pts = P.add(points), o_th = P.add(thickness), o_goff = P.add(g_off),
                o_gn = P.add(g_n), o_gids = P.add(g_ids), o_ft = P.add(fill_type), o_fo = P.add(fill_off),
                o_fn = P.add(fill_ns), o_st = P.add(stroke_type), o_so = P.add(stroke_off), o_sn = P.add(stroke_ns),

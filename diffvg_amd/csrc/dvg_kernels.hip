@@ -2486,14 +2486,14 @@ __global__ __launch_bounds__(kBlock) void k_refit_cdf(RefitArgs A) {
     }
 }
 
-// The segment-only part of the fast cubic winding test (wf::cubic_prep), one lane per
-// segment, from the scene's current points.
+// The segment-only parts of the fast cubic tests, one lane per segment, from the scene's
+// current points: wf::cubic_prep in float4 0-2, wf::stroke_prep in float4 3-7.
 __global__ __launch_bounds__(kBlock) void k_seg_prep(DevScene S) {
     const int seg = blockIdx.x * blockDim.x + threadIdx.x;
     if (seg >= S.nseg) return;
-    float4 *dst = const_cast<float4 *>(S.seg_wf) + 3 * (size_t)seg;
+    float4 *dst = const_cast<float4 *>(S.seg_wf) + kSegPrep * (size_t)seg;
     if (S.seg_kind[seg] != 2) {
-        dst[0] = dst[1] = dst[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = 0; i < kSegPrep; i++) dst[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
     const int4 idx = S.seg_idx[seg];
@@ -2502,6 +2502,10 @@ __global__ __launch_bounds__(kBlock) void k_seg_prep(DevScene S) {
     dst[0] = make_float4(q.my, q.mx, q.E, q.ddy);
     dst[1] = make_float4(q.ddx, q.ddmax, q.xdmax, q.br1);
     dst[2] = make_float4(q.br2, 0.f, 0.f, 0.f);
+    float poly[16], mc, dev;
+    wf::stroke_prep(a.x, a.y, b.x, b.y, c.x, c.y, d.x, d.y, poly, mc, dev);
+    for (int i = 0; i < 4; i++) dst[3 + i] = make_float4(poly[4 * i], poly[4 * i + 1], poly[4 * i + 2], poly[4 * i + 3]);
+    dst[7] = make_float4(mc, dev, 0.f, 0.f);
 }
 
 int prep_segments(SceneHost &sc, hipStream_t st) {

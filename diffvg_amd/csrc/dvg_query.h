@@ -120,7 +120,7 @@ DVG_DEV int winding_path(const DevScene &S, int s, V2 pt) {
         V2 p[4];
         int4 idx;
         int k = seg_load(S, seg, p, idx);
-        wn += segment_winding(k, p, pt, S.seg_wf + 3 * (size_t)seg);
+        wn += segment_winding(k, p, pt, S.seg_wf + kSegPrep * (size_t)seg);
     }
     return wn;
 }
@@ -304,9 +304,21 @@ DVG_DEV int segment_within(const DevScene &S, int s, int seg, V2 pt, float r) {
     if (dist_sq(ceval(p0, p1, p2, p3, 1), pt) < r3 * r3) return 1;
     // every curve point certainly farther than the largest radius magnitude (the radius
     // at t is a convex combination of r0..r3): the quintic search below would find nothing
-    if (wf::cubic_certainly_farther(p0.x, p0.y, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y, pt.x, pt.y,
-                                    fmaxf(fmaxf(fabsf(r0), fabsf(r1)), fmaxf(fabsf(r2), fabsf(r3)))))
-        return 0;
+    {
+        const float4 *q = S.seg_wf + kSegPrep * (size_t)seg + 3;
+        float poly[16];
+        for (int i = 0; i < 4; i++) {
+            const float4 v = q[i];
+            poly[4 * i] = v.x;
+            poly[4 * i + 1] = v.y;
+            poly[4 * i + 2] = v.z;
+            poly[4 * i + 3] = v.w;
+        }
+        const float4 md = q[4];
+        if (wf::cubic_certainly_farther_prep(poly, md.x, md.y, p0.x, p0.y, pt.x, pt.y,
+                                             fmaxf(fmaxf(fabsf(r0), fabsf(r1)), fmaxf(fabsf(r2), fabsf(r3)))))
+            return 0;
+    }
     Quintic q = cubic_quintic(p0, p1, p2, p3, pt);
     float iv[4];
     int niv = quintic_intervals(q, iv);

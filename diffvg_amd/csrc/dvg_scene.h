@@ -32,6 +32,9 @@ struct BNode {  // BVHNode (reference scene.h:12-16), padded to 32 bytes
     int pad;
 };
 
+// float4 per segment of DevScene::seg_wf: wf::CubicPrep (3), wf::stroke_prep (5)
+constexpr int kSegPrep = 8;
+
 struct DevScene {
     int cw, ch;
     int ns, ng, nseg, npts, num_total_shapes;
@@ -56,7 +59,7 @@ struct DevScene {
     const int *seg_pid;        // first point id relative to the path
     const float4 *seg_box;
     const float *seg_maxr;
-    const float4 *seg_wf;      // [3 * nseg] wf::CubicPrep of each cubic segment (from its points)
+    const float4 *seg_wf;      // [kSegPrep * nseg] fast-test prep of each cubic segment (from its points)
     const int *seg_dfs;        // per path (indexed from seg_begin): segment ids in visit order
     const float *seg_cdf;
     const float *seg_pmf;

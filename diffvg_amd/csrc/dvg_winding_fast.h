@@ -234,25 +234,43 @@ DVG_WF bool seg_farther(float ax, float ay, float bx, float by, float px, float 
     return cr * cr > n2 * l2;
 }
 
-DVG_WF bool cubic_certainly_farther(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float p3x,
-                                    float p3y, float ptx, float pty, float rmax) {
-    const float M = fmaxf(max4(fabsf(p0x), fabsf(p1x), fabsf(p2x), fabsf(p3x)),
-                          fmaxf(max4(fabsf(p0y), fabsf(p1y), fabsf(p2y), fabsf(p3y)), fmaxf(fabsf(ptx), fabsf(pty)))) +
-                    1.f;
-    const float margin = 1e-4f * M;
+// The segment-only part of the stroke test: the polyline points at t = k/8, k = 1..8
+// (`poly`, x/y interleaved), the largest control-point coordinate magnitude `mc` and the
+// chords' deviation bound `dev`.
+DVG_WF void stroke_prep(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float p3x, float p3y,
+                        float *poly, float &mc, float &dev) {
+    mc = fmaxf(max4(fabsf(p0x), fabsf(p1x), fabsf(p2x), fabsf(p3x)), max4(fabsf(p0y), fabsf(p1y), fabsf(p2y), fabsf(p3y)));
     const float sx = fmaxf(fabsf(p0x - 2.f * p1x + p2x), fabsf(p1x - 2.f * p2x + p3x));
     const float sy = fmaxf(fabsf(p0y - 2.f * p1y + p2y), fabsf(p1y - 2.f * p2y + p3y));
-    const float dev = (6.f / 512.f) * sqrtf(sx * sx + sy * sy) * 1.001f;
-    const float need = rmax + margin + dev;
-    float ax = p0x, ay = p0y;
+    dev = (6.f / 512.f) * sqrtf(sx * sx + sy * sy) * 1.001f;
     for (int k = 1; k <= 8; k++) {
         float t = 0.125f * (float)k;
-        float bx = bern3(p0x, p1x, p2x, p3x, t), by = bern3(p0y, p1y, p2y, p3y, t);
+        poly[2 * (k - 1)] = bern3(p0x, p1x, p2x, p3x, t);
+        poly[2 * (k - 1) + 1] = bern3(p0y, p1y, p2y, p3y, t);
+    }
+}
+
+// The test itself, given the segment's stroke_prep.
+DVG_WF bool cubic_certainly_farther_prep(const float *poly, float mc, float dev, float p0x, float p0y, float ptx,
+                                         float pty, float rmax) {
+    const float M = fmaxf(mc, fmaxf(fabsf(ptx), fabsf(pty))) + 1.f;
+    const float margin = 1e-4f * M;
+    const float need = rmax + margin + dev;
+    float ax = p0x, ay = p0y;
+    for (int k = 0; k < 8; k++) {
+        const float bx = poly[2 * k], by = poly[2 * k + 1];
         if (!seg_farther(ax, ay, bx, by, ptx, pty, need)) return false;
         ax = bx;
         ay = by;
     }
     return true;
+}
+
+DVG_WF bool cubic_certainly_farther(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float p3x,
+                                    float p3y, float ptx, float pty, float rmax) {
+    float poly[16], mc, dev;
+    stroke_prep(p0x, p0y, p1x, p1y, p2x, p2y, p3x, p3y, poly, mc, dev);
+    return cubic_certainly_farther_prep(poly, mc, dev, p0x, p0y, ptx, pty, rmax);
 }
 
 }  // namespace wf
