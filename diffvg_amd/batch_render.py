@@ -85,6 +85,10 @@ class _BatchNode(torch.autograd.Function):
                                      use_prefiltering=flat.use_prefiltering)))
         fwd = Batch(jobs, zero_grads=False)
         fwd.render(st)
+        if any(flat.use_prefiltering for flat in flats):
+            # a prefiltered stroke of a circle group, on which the reference asserts in the
+            # forward: read at the next render call or synchronize(), as RenderFunction does
+            _PENDING.append((_BatchCheck(fwd, None), None))
         ctx.spec, ctx.scenes, ctx.flats = spec, scenes, flats
         ctx.shapes = [None if t is None else t.shape for t in inputs]
         ctx.fwd = fwd
@@ -133,7 +137,8 @@ class _BatchNode(torch.autograd.Function):
 
 
 class _BatchCheck:
-    """Keeps a backward batch and its scene leases until its findings are read."""
+    """Keeps a batch (and, for a backward batch, its scene leases) until its findings are
+    read."""
 
     def __init__(self, batch, scenes):
         self.batch, self.scenes = batch, scenes

@@ -618,6 +618,34 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     return 0;
 }
 
+// The top-level BVH (scene.cpp:648-683) of a scene whose parameters dvg_scene_update
+// rewrote: its leaves are the groups' canvas boxes and radii, which the device rebuild
+// keeps current (k_refit_groups), so they are read back and the tree is built by the
+// same host code as in build_scene (std::sort by Morton code, pairwise build with carry).
+// Waits for the scene's last stream; the upload is complete when this returns.
+int rebuild_top(SceneHost &sc) {
+    const int ng = sc.num_groups;
+    if (ng > 0) {
+        if (sc.last_stream) HIP_OK(hipStreamSynchronize(sc.last_stream));
+        std::vector<float4> gb(ng);
+        std::vector<float> gr(ng);
+        HIP_OK(hipMemcpy(gb.data(), sc.dev.g_box, ng * sizeof(float4), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(gr.data(), sc.dev.g_maxr, ng * sizeof(float), hipMemcpyDeviceToHost));
+        std::vector<HNode> top(2 * (size_t)ng - 1, HNode{-1, -1, box_empty(), 0.f});
+        for (int g = 0; g < ng; g++) top[g] = HNode{g, -1, HBox{gb[g].x, gb[g].y, gb[g].z, gb[g].w}, gr[g]};
+        build_bvh(top.data(), ng, true, sc.cw, sc.ch);
+        std::vector<BNode> topd(top.size());
+        for (size_t i = 0; i < top.size(); i++)
+            topd[i] = BNode{top[i].child0, top[i].child1, top[i].box.minx, top[i].box.miny, top[i].box.maxx,
+                            top[i].box.maxy, top[i].max_radius, 0};
+        if (topd.size() * sizeof(BNode) > sc.top_bytes) return set_error("top-level BVH larger than its slot");
+        if (int e = upload(sc.device, (uint8_t *)sc.block + sc.top_offset, topd.data(), topd.size() * sizeof(BNode)))
+            return e;
+    }
+    sc.top_stale = false;
+    return 0;
+}
+
 void free_scene(SceneHost &sc) {
     if (!sc.device_set) return;
     int prev = -1;

@@ -122,5 +122,8 @@ int scene_status(SceneHost &sc);
 void free_scene(SceneHost &sc);
 int render_tiled(SceneHost &sc, const dvg_render_args &a);
 int render_plain(SceneHost &sc, const dvg_render_args &a);
+// Rebuilds the PLAIN schedule's top-level BVH from the scene's current group boxes (after
+// dvg_scene_update); synchronous.
+int rebuild_top(SceneHost &sc);
 
 }  // namespace dvg

@@ -1631,10 +1631,9 @@ int render_plain(SceneHost &sc, const dvg_render_args &a) {
     DevScene S = sc.dev;
     if (eval_mode && (a.render_image || a.d_render_image))
         return set_error("eval_positions are supported for the SDF output only (as in the reference)");
+    // the device rebuild keeps the group boxes current but not the tree over them
     if (sc.top_stale)
-        return set_error("the PLAIN schedule needs a scene built by dvg_scene_create: this scene was updated on the "
-                         "device, which does not rebuild the top-level BVH");
-    DVG_CHECK(hipMemsetAsync(sc.err, 0, sizeof(int), st));
+        if (int e = rebuild_top(sc)) return e;
     if (sc.num_shapes == 0) {
         if (int e = render_empty(a, npix, st)) return e;
         DVG_CHECK(hipStreamSynchronize(st));
@@ -1765,7 +1764,6 @@ __global__ __launch_bounds__(kBlock) void kb_zero(const Job *jobs, const int *pr
     int j = find_job(pref, nj, blockIdx.x);
     const Job &J = jobs[j];
     int b = blockIdx.x - pref[j], nb = pref[j + 1] - pref[j];
-    if (b == 0 && threadIdx.x == 0) *J.err = 0;
     if (J.d_img && J.tile_count)
         for (int i = b * kBlock + threadIdx.x; i < J.ntiles; i += nb * kBlock) J.tile_count[i] = 0;
     if (zero_grads && (J.d_img || J.d_sdf))
@@ -2132,10 +2130,14 @@ int batch_status(BatchHost &B) {
     (void)hipGetDevice(&prev);
     if (prev != B.device) (void)hipSetDevice(B.device);
     DVG_CHECK(hipStreamSynchronize(B.last_stream));
+    // render conditions accumulate in each scene's word until read: read and clear them,
+    // as dvg_scene_status does (a forward batch's findings are still there when its
+    // backward batch has run)
     int flags[2] = {0, 0};
     for (auto *sc : B.scenes) {
         int h[2] = {0, 0};
         DVG_CHECK(hipMemcpyAsync(h, sc->err, 2 * sizeof(int), hipMemcpyDeviceToHost, B.last_stream));
+        DVG_CHECK(hipMemsetAsync(sc->err, 0, sizeof(int), B.last_stream));
         DVG_CHECK(hipStreamSynchronize(B.last_stream));
         flags[0] |= h[0];
         flags[1] |= h[1];

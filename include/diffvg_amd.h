@@ -173,7 +173,8 @@ typedef struct dvg_scene_params {
  * (so results equal those of a scene created from the same values).  The gradient
  * buffers are not touched.  Conditions Scene construction rejects (non-finite points or
  * lengths, a total boundary length <= 0) are reported by dvg_scene_status.  The PLAIN
- * schedule's top-level BVH is not rebuilt: PLAIN renders of an updated scene fail.
+ * schedule's top-level BVH (scene.cpp:648-683) is rebuilt on the host from the updated
+ * group boxes by the next PLAIN render of the scene, which is synchronous anyway.
  * Batches created earlier keep the filter radius they were created with. */
 int dvg_scene_update(dvg_scene *scene, const dvg_scene_params *params, float filter_radius, void *stream);
 /* Waits for the scene's last stream and reports the conditions its last update found and

@@ -212,3 +212,47 @@ def test_batch_rejects_invalid_jobs():
         Batch([(sc, dict(width=8, height=8, nsx=17, nsy=16, seed=0, render_image=out))])
     with pytest.raises(RuntimeError, match='either colour or SDF'):
         Batch([(sc, dict(width=8, height=8, nsx=1, nsy=1, seed=0))])
+
+
+def test_batched_prefiltered_circle_stroke_is_reported():
+    """The reference's prefiltered forward asserts on a stroked circle (diffvg.cpp:868).
+    A batch flags it in the scene's condition word and the batch's status raises it: for
+    a forward-only batch, for a forward batch whose backward batch has run since (the
+    backward replays kept lists and would not flag it again), and through render_batch
+    at the next synchronize().  The status read clears the condition."""
+    import diffvg_amd as pd
+    from diffvg_amd import render_pytorch as rp
+    from diffvg_amd.batch_render import render_batch
+    c = pd.Circle(radius=scenes.T(10.0), center=scenes.T([32.0, 32.0]))
+    g = pd.ShapeGroup(shape_ids=torch.tensor([0]), fill_color=scenes.T([0.2, 0.4, 0.6, 1.0]),
+                      stroke_color=scenes.T([0.0, 0.0, 0.0, 1.0]))
+    f = scenes.flat_case(dict(canvas=(64, 64), shapes=[c], groups=[g], filter=('box', 0.5)))
+    sc = _lib.DeviceScene(f, 0)
+    img = torch.zeros(64, 64, 4, device=DEV)
+    fwd = Batch([(sc, dict(width=64, height=64, nsx=2, nsy=2, seed=0, render_image=img, use_prefiltering=True))],
+                zero_grads=False)
+    fwd.render(None)
+    with pytest.raises(RuntimeError, match='prefiltered stroke'):
+        fwd.status()
+    fwd.status()  # cleared by the read
+    d = torch.ones(64, 64, 4, device=DEV)
+    bwd = Batch([(sc, dict(width=64, height=64, nsx=2, nsy=2, seed=0, d_render_image=d, use_prefiltering=True))])
+    fwd.render(None)
+    bwd.render(None)
+    with pytest.raises(RuntimeError, match='prefiltered stroke'):
+        fwd.status()
+    bwd.status()
+    pd.set_use_gpu(True)
+    pd.set_device(DEV)
+    rp.synchronize()
+    c.radius.requires_grad_(True)
+    args = pd.RenderFunction.serialize_scene(64, 64, [c], [g], use_prefiltering=True)
+    out = render_batch([args])
+    out[0].sum().backward()
+    with pytest.raises(RuntimeError, match='prefiltered stroke'):
+        pd.synchronize()
+    pd.synchronize()
+    out = render_batch([args])  # forward only
+    with pytest.raises(RuntimeError, match='prefiltered stroke'):
+        pd.synchronize()
+    pd.synchronize()

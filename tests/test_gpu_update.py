@@ -184,9 +184,43 @@ def test_rebuild_reports_what_construction_rejects():
     fc['points'] = collapsed
     with pytest.raises(RuntimeError, match='total length'):
         _lib.DeviceScene(fc, 0)
+
+
+@pytest.mark.parametrize('case', ['overlap_eo', 'strokes', 'transformed', 'ties'])
+def test_plain_schedule_renders_updated_scenes(case):
+    """The PLAIN schedule walks the reference's top-level BVH (scene.cpp:648-683), which
+    the device rebuild does not maintain: its next PLAIN render rebuilds the tree from the
+    updated group boxes.  Renders of an updated scene then equal those of a scene created
+    from the same values, and the oracle's, in the PLAIN schedule -- also after a second
+    update and a TILED render in between."""
+    from refdrive import OracleRunner
+    print('plain after update', case, flush=True)
+    f1 = _flat(case)
+    w, h = f1['canvas_width'], f1['canvas_height']
+    d = np.random.default_rng(4).uniform(-1, 1, (h, w, 4)).astype(np.float32)
+    a = _lib.DeviceScene(f1, 0)
     a.set_mode(_lib.MODE_PLAIN)
-    with pytest.raises(RuntimeError, match='PLAIN'):
-        a.render(64, 64, 2, 2, 0, render_image=torch.zeros(64, 64, 4, device='cuda'))
+    _render(a, w, h)
+    for seed in (5, 6):
+        f2 = _perturb(f1, seed, rigid=case == 'ties')
+        a.update(f2['filter_radius'], None, **{k: _dev(f2[k]) for k in FLOAT_KEYS})
+        a.status()
+        if seed == 6:
+            a.set_mode(_lib.MODE_TILED)
+            _render(a, w, h)
+            a.set_mode(_lib.MODE_PLAIN)
+        b = _lib.DeviceScene(f2, 0, _lib.MODE_PLAIN)
+        img_a, img_b = _render(a, w, h), _render(b, w, h)
+        want = OracleRunner(f2).render(w, h, 2, 2, 3)['image']
+        # PLAIN splats with float atomics in arrival order (5e-5, as for every PLAIN render)
+        assert np.abs(img_a - img_b).max() <= 5e-5
+        assert np.abs(img_a - want).max() <= 5e-5
+        ga, gb = _render(a, w, h, d), _render(b, w, h, d)
+        for k in gb:
+            if gb[k].size == 0:
+                continue
+            scale = max(float(np.abs(gb[k]).max()), 1e-12)
+            assert np.abs(ga[k] - gb[k]).max() <= 1e-3 * scale, k
 
 
 def _leaf_scene():
@@ -241,6 +275,42 @@ def test_render_function_rebuild_rejects_non_finite_points():
         sc['shapes'][2].points[1, 0] = float('nan')
     with pytest.raises(AssertionError):
         _step(sc, target)
+    rp.clear_scene_cache()
+
+
+def test_render_function_more_than_256_spp_on_a_pooled_scene():
+    """17 x 16 samples per pixel exceed the tiled kernels' 256 lanes, so such renders run
+    the PLAIN schedule -- also on the pooled scene the second call of one layout
+    rebuilds on the device.  Both calls (forward and every parameter gradient) agree with
+    the oracle."""
+    from refdrive import OracleRunner
+    pd.set_use_gpu(True)
+    pd.set_device(torch.device('cuda:0'))
+    rp.clear_scene_cache()
+    size = 24
+    sc = scenes.overlap(size=size, n=4)
+    for s in sc['shapes']:
+        s.points.requires_grad_(True)
+    rng = np.random.default_rng(9)
+    for step in range(2):
+        if step:
+            with torch.no_grad():
+                for s in sc['shapes']:
+                    s.points += torch.from_numpy(rng.normal(0, 0.4, tuple(s.points.shape)).astype(np.float32))
+        for s in sc['shapes']:
+            s.points.grad = None
+        args = pd.RenderFunction.serialize_scene(size, size, sc['shapes'], sc['groups'])
+        img = pd.RenderFunction.apply(size, size, 17, 16, 1, None, *args)
+        d = torch.from_numpy(rng.uniform(-1, 1, (size, size, 4)).astype(np.float32)).cuda()
+        (img * d).sum().backward()
+        pd.synchronize()
+        flat = flatten(size, size, sc['shapes'], sc['groups'], 0, 0.5)
+        orc = OracleRunner(flat)
+        want = orc.render(size, size, 17, 16, 1)['image']
+        assert np.abs(img.detach().cpu().numpy() - want).max() <= 5e-5, step
+        gw = orc.render(size, size, 17, 16, 1, d_image=d.cpu().numpy())['grads']['points']
+        got = np.concatenate([s.points.grad.cpu().numpy().reshape(-1) for s in sc['shapes']])
+        assert np.abs(got - gw).max() <= 1e-3 * max(float(np.abs(gw).max()), 1e-12), step
     rp.clear_scene_cache()
 
 
