@@ -1,14 +1,22 @@
-"""Benchmark: fwd+bwd sample throughput of the render path on one MI355X.
+"""Benchmark of the render path on one MI355X; prints one JSON line.
 
-Default workload: the 300-path, 512x512, 2x2, Gaussian-filter scene (prefiltering off),
-one step = RenderFunction.apply + L2 loss + backward until the gradients are on the
-device, scene marshalling and construction included.  Prints one JSON line.
+Default line (`python bench.py`): 256 uneven scenes (path counts log-uniform in [1, 512],
+2-8 cubics per path, 10% gradient fills, 28-256 px, 2x2 spp, box filter), one forward +
+backward each with an L2 loss, rendered as one forward batch, the loss gradient and one
+backward batch replayed from a HIP graph with the scenes resident in HBM (`value`,
+Msamples/s), against the plain path (one RenderFunction call per scene on the PLAIN
+schedule, the reference's launch structure) and the user-level `render_batch` end to
+end.  Extra keys: the serving stream (Poisson arrivals at 0.8x the plain service rate,
+p50/p99 latency served and plain, an arrival-rate sweep to the rate where the served
+p99 passes 50 ms), the two 512x512 single scenes (300 closed paths with a Gaussian
+filter; 2,048 strokes) with their PLAIN ratios, the roofline of the dominant batched
+kernel and the CPU baseline.  `--config 2|3|4|5` runs one workload alone.
 
 The cpu_baseline leg runs in a subprocess: it times the reference's own CPU path
-(oracle/_ref, built from the reference sources) on the same scene with the tent-filter
-twin (the reference has no Gaussian filter) and counts the algorithmic work of each
-pass with the oracle's instrumented traversal on a strided sample of the samples; the
-counts, priced with the cost table below, give the roofline's algorithmic FLOPs.
+(oracle/_ref, built from the reference sources) on a bounded sample of the same
+workload, and counts the algorithmic work of each pass with the oracle's instrumented
+traversal on a strided sample of the samples; the counts, priced with the cost table
+below, give the roofline's algorithmic FLOPs.
 """
 import argparse
 import ctypes
@@ -39,55 +47,95 @@ COST = {
 }
 COUNTERS = list(COST)
 PEAK_FP32_TFLOPS = 157.3
+# uneven scenes timed on the host cores for the batched workload's CPU baseline
+CPU_SAMPLE_SCENES = 64
 PEAK_HBM_GBS = 8000.0
 
 
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _count_passes(lib, runner, w, h, nsx, nsy, d_img, stride):
+    """Oracle query counts of one forward and one backward on every `stride`-th sample,
+    scaled to all samples: (forward, backward interior, backward edge)."""
+    lib.orc_set_counting(1)
+    lib.orc_set_sample_stride(stride)
+    cnt = (ctypes.c_int64 * 12)()
+    runner.render(w, h, nsx, nsy, 0)
+    lib.orc_get_counters(0, ctypes.byref(cnt))
+    fwd = [int(x) * stride for x in cnt]
+    runner.render(w, h, nsx, nsy, 0, d_image=d_img)
+    lib.orc_get_counters(0, ctypes.byref(cnt))
+    bwd = [int(x) * stride for x in cnt]
+    lib.orc_get_counters(1, ctypes.byref(cnt))
+    edge = [int(x) * stride for x in cnt]
+    lib.orc_set_counting(0)
+    lib.orc_set_sample_stride(1)
+    return fwd, bwd, edge
+
+
 def cpu_baseline_main(cfg):
-    """Subprocess body of the cpu_baseline leg (no GPU use)."""
+    """Subprocess body of the cpu_baseline leg (no GPU use).  Times the reference's CPU
+    build (oracle/_ref; the oracle when that is absent) and counts the algorithmic work
+    of each pass with the oracle's instrumented traversal."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, 'tests'))
     from diffvg_amd import workloads
     from refdrive import flatten, load_ref, OracleRunner, RefRunner, load_oracle
-    c = workloads.closed_paths_300(device='cpu', filter_type='tent') if cfg == 2 else workloads.WORKLOADS[cfg](device='cpu')
-    w, h = c['canvas']
-    nsx, nsy = c['nsx'], c['nsy']
-    f = flatten(w, h, c['shapes'], c['groups'], c['filter'].type, float(c['filter'].radius))
-    out = {}
-    d_img = np.random.default_rng(0).uniform(-1, 1, (h, w, 4)).astype(np.float32)
-    if load_ref() is not None:
-        r = RefRunner(f)
+    lib = load_oracle()
+    threads = os.cpu_count() or 1
+    lib.orc_set_num_threads(threads)
+    if cfg == 4:
+        items = workloads.uneven_scenes(device='cpu')
+        scenes = []
+        for k, it in enumerate(items):
+            w, h = it['canvas']
+            f = flatten(w, h, it['shapes'], it['groups'], it['filter'].type, float(it['filter'].radius))
+            scenes.append((f, w, h, np.random.default_rng(k).uniform(-1, 1, (h, w, 4)).astype(np.float32)))
+        nsx = nsy = 2
+        timed = scenes[:CPU_SAMPLE_SCENES]
+        sample = ('the first %d of the 256 uneven scenes (%d samples), one forward and one backward (edge '
+                  'sampling) each' % (len(timed), sum(w * h * 4 for _, w, h, _ in timed)))
+        stride = 8
+    else:
+        c = (workloads.closed_paths_300(device='cpu', filter_type='tent') if cfg == 2
+             else workloads.WORKLOADS[cfg](device='cpu'))
+        w, h = c['canvas']
+        nsx, nsy = c['nsx'], c['nsy']
+        f = flatten(w, h, c['shapes'], c['groups'], c['filter'].type, float(c['filter'].radius))
+        scenes = timed = [(f, w, h, np.random.default_rng(0).uniform(-1, 1, (h, w, 4)).astype(np.float32))]
+        sample = ('the full %dx%d %dx%d-spp scene, one forward and one backward (edge sampling), %s'
+                  % (w, h, nsx, nsy, 'tent r=1 twin of the Gaussian filter' if cfg == 2 else 'the same box filter'))
+        stride = 4
+    out = {'sample': sample, 'cpu_model': cpu_model()}
+    have_ref = load_ref() is not None
+    t_fwd = t_bwd = 0.0
+    for f, w, h, d_img in timed:
+        r = RefRunner(f) if have_ref else OracleRunner(f, threads=threads)
         t0 = time.time()
         r.render(w, h, nsx, nsy, 0)
         t1 = time.time()
         r.render(w, h, nsx, nsy, 0, d_image=d_img)
         t2 = time.time()
-        out.update(kind='reference', fwd_s=t1 - t0, bwd_s=t2 - t1, cores=os.cpu_count())
-    else:
-        o = OracleRunner(f)
-        t0 = time.time()
-        o.render(w, h, nsx, nsy, 0)
-        t1 = time.time()
-        o.render(w, h, nsx, nsy, 0, d_image=d_img)
-        t2 = time.time()
-        out.update(kind='port', fwd_s=t1 - t0, bwd_s=t2 - t1, cores=load_oracle().orc_get_num_threads())
-    out['samples'] = w * h * nsx * nsy
-    # algorithmic work of each pass, counted on every `stride`-th sample and scaled
-    lib = load_oracle()
-    stride = 4
-    o = OracleRunner(f)
-    lib.orc_set_counting(1)
-    lib.orc_set_sample_stride(stride)
-    cnt = (ctypes.c_int64 * 12)()
-    o.render(w, h, nsx, nsy, 0)
-    lib.orc_get_counters(0, ctypes.byref(cnt))
-    out['counts_fwd'] = [int(x) * stride for x in cnt]
-    o.render(w, h, nsx, nsy, 0, d_image=d_img)
-    lib.orc_get_counters(0, ctypes.byref(cnt))
-    out['counts_bwd'] = [int(x) * stride for x in cnt]
-    lib.orc_get_counters(1, ctypes.byref(cnt))
-    out['counts_edge'] = [int(x) * stride for x in cnt]
-    lib.orc_set_counting(0)
-    lib.orc_set_sample_stride(1)
+        t_fwd += t1 - t0
+        t_bwd += t2 - t1
+    out.update(kind='reference' if have_ref else 'port', fwd_s=t_fwd, bwd_s=t_bwd,
+               cores=os.cpu_count() if have_ref else lib.orc_get_num_threads(),
+               samples=sum(w * h * nsx * nsy for _, w, h, _ in timed))
+    tot = [np.zeros(12, np.int64) for _ in range(3)]
+    for f, w, h, d_img in scenes:
+        o = OracleRunner(f, threads=threads)
+        for acc, c_ in zip(tot, _count_passes(lib, o, w, h, nsx, nsy, d_img, stride)):
+            acc += np.array(c_, np.int64)
+    out['counts_fwd'], out['counts_bwd'], out['counts_edge'] = [t.tolist() for t in tot]
+    out['count_stride'] = stride
     print(json.dumps(out))
 
 
@@ -148,17 +196,37 @@ def _scene_params(shapes, groups):
     return ps
 
 
-def bench_batched(a):
+METRIC = 'fwd+bwd Msamples/s on 1 GPU; p50/p99 latency, batched/served vs plain'
+DTYPE = 'f32 (f64 root solves)'
+
+
+def _profile_lib():
+    from diffvg_amd import _lib
+    L = _lib.lib()
+    names = ctypes.create_string_buffer(4096)
+    ms = (ctypes.c_double * 32)()
+    cnts = (ctypes.c_int * 32)()
+
+    def read():
+        nk = L.dvg_profile_read(names, 4096, ms, cnts, 32)
+        return {nm: {'total_ms': ms[i], 'launches': cnts[i], 'avg_ms': ms[i] / max(cnts[i], 1)}
+                for i, nm in enumerate(names.value.decode().strip().split('\n')[:nk])}
+    return L, read
+
+
+def leg_batched(steps, warmup, autotune=False, e2e_steps=4, plain_steps=1):
     """256 uneven scenes (path counts log-uniform in [1, 512], 2-8 cubics per path, 10%
     gradient fills, resolution in {28, 64, 128, 256}, 2x2 spp, box filter), one fwd+bwd
     each with an L2 loss against a per-scene target.  Three measurements:
       graph    -- scenes resident; forward batch, loss gradient and backward batch
-                  captured in one HIP graph and replayed (value);
+                  captured in one HIP graph and replayed (`steps` timed replays);
       batched  -- per step, the user-level API: serialize every scene, render_batch
                   (pooled scenes rebuilt on the device, one batched forward and one
                   batched backward), L2 loss, backward into every parameter tensor;
       plain    -- per step: one RenderFunction call per scene on the PLAIN schedule
-                  (the reference's one-scene-per-launch structure), loss, backward."""
+                  (the reference's one-scene-per-launch structure), loss, backward.
+    Per-kernel times come from HIP events on the launch stream over `steps` eager
+    renders of the same two batches right after the timed replays."""
     import numpy as np
     import torch
     import diffvg_amd as pydiffvg
@@ -190,22 +258,20 @@ def bench_batched(a):
         it = items[i]
         w, h = sizes[i]
         args = pydiffvg.RenderFunction.serialize_scene(w, h, it['shapes'], it['groups'], it['filter'])
-        fl = flatten_args(args)
-        return args, fl, _lib.DeviceScene(fl.desc, 0)
+        return _lib.DeviceScene(flatten_args(args).desc, 0)
 
     built = [build(i) for i in range(n)]
     tuned = {}
-    if a.autotune:
-        # tile width per workload signature, timed once per signature on the first scene
+    if autotune:
         from diffvg_amd.autotune import Tuner
         tuner = Tuner()
-        for i, (_, _, ds) in enumerate(built):
+        for i, ds in enumerate(built):
             tuner.tune(ds, sizes[i][0], sizes[i][1], 2, 2)
         tuned = {str(k): v for k, v in tuner.best.items()}
     fjobs = [(ds, dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0, render_image=view(img_flat, i)))
-             for i, (_, _, ds) in enumerate(built)]
+             for i, ds in enumerate(built)]
     bjobs = [(ds, dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0, d_render_image=view(d_flat, i)))
-             for i, (_, _, ds) in enumerate(built)]
+             for i, ds in enumerate(built)]
     fb, bb = Batch(fjobs, zero_grads=False), Batch(bjobs, zero_grads=True)
     stream = torch.cuda.Stream()
 
@@ -224,34 +290,28 @@ def bench_batched(a):
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
         graph_body(torch.cuda.current_stream().cuda_stream)
-    for _ in range(a.warmup):
+    for _ in range(warmup):
         g.replay()
     torch.cuda.synchronize()
-    reps = max(a.steps, 5)
-    L = _lib.lib()
+    L, read_profile = _profile_lib()
     L.dvg_profile_enable(0)
+    read_profile()
     t0 = time.perf_counter()
-    for _ in range(reps):
+    for _ in range(steps):
         g.replay()
     torch.cuda.synchronize()
-    t_graph = (time.perf_counter() - t0) / reps
+    t_graph = (time.perf_counter() - t0) / steps
     fb.status()
     bb.status()
-    # eager kernel breakdown of the same batches
-    names = ctypes.create_string_buffer(4096)
-    ms = (ctypes.c_double * 32)()
-    cnts = (ctypes.c_int * 32)()
-    L.dvg_profile_read(names, 4096, ms, cnts, 32)
+    # per-kernel times: the same batches rendered eagerly with events around each launch
     L.dvg_profile_enable(1)
-    graph_body(None)
+    for _ in range(steps):
+        graph_body(None)
     torch.cuda.synchronize()
     L.dvg_profile_enable(0)
-    nk = L.dvg_profile_read(names, 4096, ms, cnts, 32)
-    kern = {nm: round(ms[i], 4) for i, nm in enumerate(names.value.decode().strip().split('\n')[:nk])}
+    kern = read_profile()
 
-    # ---- batched end to end: the user-level autograd API over all scenes per step
-    # (serialize_scene per scene, pooled scenes rebuilt on the device, one batched
-    # forward and one batched backward, gradients delivered to every parameter tensor)
+    # ---- batched end to end through the user-level autograd API
     from diffvg_amd.batch_render import render_batch
     scene_list = [(sizes[i][0], sizes[i][1], items[i]['shapes'], items[i]['groups']) for i in range(n)]
     targets = [view(target_flat, i) for i in range(n)]
@@ -266,13 +326,14 @@ def bench_batched(a):
 
     for _ in range(2):
         batched_step()
+    pydiffvg.synchronize()
     torch.cuda.synchronize()
-    nb = max(2, a.steps // 4)
     t0 = time.perf_counter()
-    for _ in range(nb):
+    for _ in range(e2e_steps):
         batched_step()
+    pydiffvg.synchronize()
     torch.cuda.synchronize()
-    t_batched = (time.perf_counter() - t0) / nb
+    t_batched = (time.perf_counter() - t0) / e2e_steps
 
     # ---- plain: one RenderFunction call per scene (PLAIN schedule)
     pydiffvg.set_render_mode('plain')
@@ -289,41 +350,27 @@ def bench_batched(a):
 
     plain_step()
     torch.cuda.synchronize()
-    np_ = max(1, a.steps // 10)
     t0 = time.perf_counter()
-    for _ in range(np_):
+    for _ in range(plain_steps):
         plain_step()
+    pydiffvg.synchronize()
     torch.cuda.synchronize()
-    t_plain = (time.perf_counter() - t0) / np_
+    t_plain = (time.perf_counter() - t0) / plain_steps
     pydiffvg.set_render_mode('tiled')
-
-    out = {
-        'metric': 'fwd+bwd Msamples/s on 1 GPU; p50/p99 latency, batched/served vs plain',
-        'value': round(samples / t_graph / 1e6, 4), 'unit': 'Msamples/s', 'n_gpus': 1, 'steps': reps,
-        'warmup': a.warmup, 'ms_per_step': round(t_graph * 1e3, 4), 'higher_is_better': True, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': 'f32 (f64 root solves)', 'data': 'synthetic',
-        'config': {'workload': 'uneven_scenes_256', 'scenes': n, 'samples': samples,
-                   'paths': int(sum(len(it['shapes']) for it in items)), 'spp': [2, 2], 'filter': 'box r=0.5',
-                   'mode': 'batched, one HIP graph per fwd+bwd'},
-        'batched_end_to_end': {'ms_per_step': round(t_batched * 1e3, 3),
-                               'value': round(samples / t_batched / 1e6, 4)},
-        'plain': {'ms_per_step': round(t_plain * 1e3, 3), 'value': round(samples / t_plain / 1e6, 4)},
-        'speedup_graph_vs_plain': round(t_plain / t_graph, 2),
-        'speedup_batched_vs_plain': round(t_plain / t_batched, 2),
-        'kernels_ms_eager': kern,
-        'autotuned_tile_widths': tuned,
-    }
-    print(json.dumps(out))
+    return dict(samples=samples, scenes=n, paths=int(sum(len(it['shapes']) for it in items)), t_graph=t_graph,
+                t_batched=t_batched, t_plain=t_plain, kernels=kern, tuned=tuned, steps=steps,
+                e2e_steps=e2e_steps, plain_steps=plain_steps)
 
 
-def bench_serving(a):
+def leg_serving(n_req, hbm_budget_gib=64, max_batch=32, sweep=True, p99_bar_ms=50.0):
     """Serving stream: Poisson arrivals of render + gradient requests on resident scenes
     (path counts log-uniform in [1, 4096], resolution in {128, 256, 512, 1024}, colour or
     SDF, prefiltering on or off).  The arrival rate is 0.8x the plain path's sustained
     request rate measured in the same run (one request at a time, PLAIN schedule); the
     same arrival sequence is then served by the continuous-batching server and by the
     plain path, and p50/p99 latency (arrival to gradients ready) and throughput are
-    reported for both."""
+    reported for both.  The sweep then doubles the arrival rate until the served p99
+    passes `p99_bar_ms`; the highest rate that stays under it is the saturation rate."""
     import numpy as np
     import torch
     import diffvg_amd as pydiffvg
@@ -334,7 +381,6 @@ def bench_serving(a):
     pydiffvg.set_use_gpu(True)
     pydiffvg.set_device(dev)
     rng = np.random.default_rng(5)
-    n_req = a.requests
     specs = []
     for i in range(n_req):
         r = workloads.serving_request(rng, device='cpu')
@@ -349,6 +395,10 @@ def bench_serving(a):
         sc, w, h, nsx, nsy, sdf, pre, target, _ = specs[i]
         return serve.Request(i, sc, w, h, nsx, nsy, 0, sdf, pre, target, 0.0)
 
+    def server():
+        return serve.Server(device=0, hbm_budget=hbm_budget_gib << 30, max_batch=max_batch, streams=3)
+
+    samples = sum(w * h * (1 if sdf else nsx * nsy) for (_, w, h, nsx, nsy, sdf, _, _, _) in specs)
     # plain service rate, back to back
     cal = [make(i) for i in range(n_req)]
     serve.serve_plain(cal[:4], [0.0] * 4)
@@ -358,47 +408,273 @@ def bench_serving(a):
     t_plain_all = time.perf_counter() - t0
     mu = n_req / t_plain_all
     lam = 0.8 * mu
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
-    arrivals -= arrivals[0]
+    gaps = rng.exponential(1.0, n_req)  # unit-rate Poisson gaps, scaled per arrival rate
 
-    # batched server (warm-up on the first requests, then the timed stream)
-    srv = serve.Server(device=0, hbm_budget=a.hbm_budget << 30, max_batch=a.max_batch, streams=3)
-    srv.run([make(i) for i in range(min(8, n_req))], [0.0] * min(8, n_req))
+    def arrivals(rate):
+        t = np.cumsum(gaps / rate)
+        return list(t - t[0])
+
+    server().run([make(i) for i in range(min(8, n_req))], [0.0] * min(8, n_req))  # warm-up
     torch.cuda.synchronize()
-    srv = serve.Server(device=0, hbm_budget=a.hbm_budget << 30, max_batch=a.max_batch, streams=3)
-    done = srv.run([make(i) for i in range(n_req)], list(arrivals))
+    done = server().run([make(i) for i in range(n_req)], arrivals(lam))
     st_b = serve.latency_stats(done)
-    plain_done = serve.serve_plain([make(i) for i in range(n_req)], list(arrivals))
+    plain_done = serve.serve_plain([make(i) for i in range(n_req)], arrivals(lam))
     st_p = serve.latency_stats(plain_done)
-    # each request alone on an idle device (its service time without queueing), and the
-    # requests with the longest served latency
+    # each request alone on an idle device (its service time without queueing)
     solo = []
     for i in range(n_req):
-        one = serve.Server(device=0, hbm_budget=a.hbm_budget << 30, max_batch=1, streams=1)
-        r = one.run([make(i)], [0.0])[0]
+        r = serve.Server(device=0, hbm_budget=hbm_budget_gib << 30, max_batch=1, streams=1).run([make(i)], [0.0])[0]
         solo.append((r.done - r.arrival) * 1e3)
-    slowest = sorted(done, key=lambda r: r.arrival - r.done)[:5]
+    slowest = sorted(done, key=lambda r: r.arrival - r.done)[:3]
     tail = [{'request': r.rid, 'latency_ms': round((r.done - r.arrival) * 1e3, 2),
              'alone_ms': round(solo[r.rid], 2), 'size': specs[r.rid][1], 'paths': specs[r.rid][8],
              'sdf': bool(specs[r.rid][5]), 'prefilter': bool(specs[r.rid][6])} for r in slowest]
-    samples = sum(w * h * (1 if sdf else nsx * nsy) for (_, w, h, nsx, nsy, sdf, _, _, _) in specs)
     span_b = max(r.done for r in done) - min(r.arrival for r in done)
-    out = {
-        'metric': 'fwd+bwd Msamples/s on 1 GPU; p50/p99 latency, batched/served vs plain',
-        'value': round(samples / span_b / 1e6, 4), 'unit': 'Msamples/s', 'n_gpus': 1, 'steps': n_req,
-        'warmup': 8, 'ms_per_step': round(span_b / n_req * 1e3, 4), 'higher_is_better': True, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': 'f32 (f64 root solves)', 'data': 'synthetic',
-        'config': {'workload': 'serving_stream', 'requests': n_req, 'samples': samples,
-                   'mean_paths': float(np.mean([s_[8] for s_ in specs])),
-                   'sdf_fraction': float(np.mean([s_[5] for s_ in specs])),
-                   'arrival_rate_rps': round(lam, 3), 'plain_service_rate_rps': round(mu, 3),
-                   'hbm_budget_gib': a.hbm_budget, 'max_batch': a.max_batch, 'streams': 3},
-        'served': {k: round(v, 3) for k, v in st_b.items()},
-        'plain': {k: round(v, 3) for k, v in st_p.items()},
-        'alone': {'p50_ms': round(float(np.percentile(solo, 50)), 3),
-                  'p99_ms': round(float(np.percentile(solo, 99)), 3), 'max_ms': round(max(solo), 3)},
-        'slowest_served': tail,
+    sweep_rows, saturation = [], None
+    if sweep:
+        rate = 2 * lam
+        while True:
+            d = server().run([make(i) for i in range(n_req)], arrivals(rate))
+            s_ = serve.latency_stats(d)
+            span = max(r.done for r in d) - min(r.arrival for r in d)
+            row = {'arrival_rps': round(rate, 2), 'p50_ms': round(s_['p50_ms'], 2), 'p99_ms': round(s_['p99_ms'], 2),
+                   'throughput_rps': round(s_['throughput_rps'], 2), 'msamples_s': round(samples / span / 1e6, 2)}
+            sweep_rows.append(row)
+            if s_['p99_ms'] > p99_bar_ms or rate > 64 * mu:
+                break
+            saturation = row
+            rate *= 2
+        # everything at once: the server's peak throughput
+        d = server().run([make(i) for i in range(n_req)], [0.0] * n_req)
+        span = max(r.done for r in d) - min(r.arrival for r in d)
+        peak = {'throughput_rps': round(n_req / span, 2), 'msamples_s': round(samples / span / 1e6, 2)}
+    return dict(samples=samples, n_req=n_req, lam=lam, mu=mu, span_b=span_b, st_b=st_b, st_p=st_p, solo=solo,
+                tail=tail, sweep=sweep_rows, saturation=saturation, peak=peak if sweep else None,
+                mean_paths=float(np.mean([s_[8] for s_ in specs])),
+                sdf_fraction=float(np.mean([s_[5] for s_ in specs])),
+                prefilter_fraction=float(np.mean([s_[6] and not s_[5] for s_ in specs])),
+                max_size=int(max(s_[1] for s_ in specs)), max_paths=int(max(s_[8] for s_ in specs)),
+                hbm_budget_gib=hbm_budget_gib, max_batch=max_batch)
+
+
+def leg_single(cfg, steps, warmup, mode='tiled', prefilter_steps=0):
+    """One 512x512 scene (2: 300 closed paths, Gaussian filter; 3: 2,048 strokes, box
+    filter): a step is serialize_scene + RenderFunction.apply + L2 loss + backward with
+    the gradients on the device.  Per-kernel times come from the library's HIP events
+    over the same number of steps run again after the timed ones."""
+    import numpy as np
+    import torch
+    import diffvg_amd as pydiffvg
+    from diffvg_amd import workloads
+
+    dev = torch.device('cuda:0')
+    pydiffvg.set_use_gpu(True)
+    pydiffvg.set_device(dev)
+    pydiffvg.set_render_mode(mode)
+    c = workloads.WORKLOADS[cfg](device=dev)
+    w, h = c['canvas']
+    nsx, nsy = c['nsx'], c['nsy']
+    params = make_params(c, dev)
+    if 'target' in c:
+        target = c['target']
+    else:
+        target = torch.from_numpy(np.random.default_rng(cfg).uniform(0, 1, (h, w, 4)).astype(np.float32)).to(dev)
+    render = pydiffvg.RenderFunction.apply
+
+    def step(prefilter=False):
+        for p in params:
+            p.grad = None
+        args = pydiffvg.RenderFunction.serialize_scene(w, h, c['shapes'], c['groups'], c['filter'],
+                                                      use_prefiltering=prefilter)
+        img = render(w, h, nsx, nsy, 0, None, *args)
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+        return loss
+
+    for _ in range(warmup):
+        step()
+    pydiffvg.synchronize()
+    torch.cuda.synchronize()
+    L, read_profile = _profile_lib()
+    read_profile()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    pydiffvg.synchronize()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    # per-kernel times: the same steps again with the library's events around each launch
+    # (kept out of the timed steps above, whose launches they would space out)
+    L.dvg_profile_enable(1)
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    L.dvg_profile_enable(0)
+    kern = read_profile()
+    pre = None
+    if prefilter_steps:
+        for _ in range(2):
+            step(True)
+        pydiffvg.synchronize()
+        torch.cuda.synchronize()
+        tp0 = time.perf_counter()
+        for _ in range(prefilter_steps):
+            step(True)
+        pydiffvg.synchronize()
+        torch.cuda.synchronize()
+        tp = (time.perf_counter() - tp0) / prefilter_steps
+        pre = {'value': round(w * h * nsx * nsy / tp / 1e6, 4), 'ms_per_step': round(tp * 1e3, 4)}
+    pydiffvg.set_render_mode('tiled')
+    return dict(name=c['name'], w=w, h=h, nsx=nsx, nsy=nsy, paths=len(c['shapes']), samples=w * h * nsx * nsy,
+                sec=sec, kernels=kern, prefilter=pre, steps=steps, warmup=warmup, mode=mode)
+
+
+def _roofline(cpu, kern, per_kernel, workload):
+    """Roofline of the dominant kernel (most total time among those with a FLOP count)."""
+    if not cpu or 'counts_fwd' not in cpu or not kern:
+        return None
+    per_kernel_flops = {k: flops(cpu[src], adjoint=adj) for k, (src, adj) in per_kernel.items()}
+    dom = max((k for k in kern if k in per_kernel_flops), key=lambda k: kern[k]['total_ms'], default=None)
+    if dom is None:
+        return None
+    avg_s = kern[dom]['avg_ms'] / 1e3
+    achieved = per_kernel_flops[dom] / avg_s / 1e12
+    return {'bound': 'valu', 'achieved': round(achieved, 4), 'peak': PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
+            'frac': round(achieved / PEAK_FP32_TFLOPS, 6), 'traffic': load_traffic(dom, workload),
+            'kernel': dom, 'kernel_avg_ms': round(kern[dom]['avg_ms'], 4), 'kernel_launches': kern[dom]['launches'],
+            'flops_per_launch': per_kernel_flops[dom],
+            'note': 'vector FP32 peak (no MFMA on this path); FLOPs are FP32-equivalent algorithmic counts of the '
+                    'reference algorithm (oracle traversal counts x cost table); kernel time from HIP events on the '
+                    'launch stream'}
+
+
+SINGLE_KERNELS = {'k_color': ('counts_fwd', False), 'kp_render': ('counts_fwd', False),
+                  'k_color_bwd': ('counts_bwd', True), 'k_edge': ('counts_edge', False),
+                  'kp_edge': ('counts_edge', False)}
+BATCH_KERNELS = {'kb_color': ('counts_fwd', False), 'kb_color_bwd': ('counts_bwd', True),
+                 'kb_edge': ('counts_edge', False)}
+
+
+def _cpu_baseline(cpu):
+    if not cpu or 'fwd_s' not in cpu:
+        return None
+    return {'value': round(cpu['samples'] / (cpu['fwd_s'] + cpu['bwd_s']) / 1e6, 4), 'unit': 'Msamples/s',
+            'cores': cpu['cores'], 'kind': cpu['kind'], 'sample': cpu['sample'], 'cpu_model': cpu.get('cpu_model')}
+
+
+def _kernels_out(kern):
+    return {k: {'avg_ms': round(v['avg_ms'], 4), 'launches': v['launches']} for k, v in kern.items()}
+
+
+def _batched_fields(b):
+    return {
+        'batched_end_to_end': {'ms_per_step': round(b['t_batched'] * 1e3, 3),
+                               'value': round(b['samples'] / b['t_batched'] / 1e6, 4), 'steps': b['e2e_steps'],
+                               'what': 'serialize_scene + render_batch (pooled device rebuilds, one batched fwd '
+                                       'and bwd) + L2 loss + backward into every parameter tensor'},
+        'plain': {'ms_per_step': round(b['t_plain'] * 1e3, 3), 'value': round(b['samples'] / b['t_plain'] / 1e6, 4),
+                  'steps': b['plain_steps'],
+                  'what': 'one RenderFunction fwd+bwd per scene on the PLAIN schedule (reference launch structure)'},
+        'speedup_graph_vs_plain': round(b['t_plain'] / b['t_graph'], 2),
+        'speedup_batched_vs_plain': round(b['t_plain'] / b['t_batched'], 2),
+        'kernels_ms_eager': {k: round(v['avg_ms'], 4) for k, v in b['kernels'].items()},
+        'kernel_ms_per_step': round(sum(v['total_ms'] for v in b['kernels'].values()) / b['steps'], 4),
+        'autotuned_tile_widths': b['tuned'],
     }
+
+
+def _serving_fields(s):
+    import numpy as np
+    r3 = lambda d: {k: round(v, 3) for k, v in d.items()}
+    return {
+        'arrival_rate_rps': round(s['lam'], 3), 'plain_service_rate_rps': round(s['mu'], 3),
+        'requests': s['n_req'], 'mean_paths': round(s['mean_paths'], 1), 'max_paths': s['max_paths'],
+        'max_size': s['max_size'], 'sdf_fraction': round(s['sdf_fraction'], 3),
+        'prefilter_fraction': round(s['prefilter_fraction'], 3),
+        'served': r3(s['st_b']), 'plain': r3(s['st_p']),
+        'served_msamples_s': round(s['samples'] / s['span_b'] / 1e6, 4),
+        'alone': {'p50_ms': round(float(np.percentile(s['solo'], 50)), 3),
+                  'p99_ms': round(float(np.percentile(s['solo'], 99)), 3), 'max_ms': round(max(s['solo']), 3)},
+        'slowest_served': s['tail'],
+        'sweep': s['sweep'], 'saturation_under_p99_50ms': s['saturation'], 'peak_all_at_once': s['peak'],
+        'hbm_budget_gib': s['hbm_budget_gib'], 'max_batch': s['max_batch'],
+    }
+
+
+def _single_fields(t, p):
+    out = {'value': round(t['samples'] / t['sec'] / 1e6, 4), 'ms_per_step': round(t['sec'] * 1e3, 4),
+           'kernel_ms_per_step': round(sum(v['total_ms'] for v in t['kernels'].values()) / t['steps'], 4),
+           'kernels': _kernels_out(t['kernels']), 'paths': t['paths'], 'canvas': [t['w'], t['h']],
+           'spp': [t['nsx'], t['nsy']]}
+    if t['prefilter']:
+        out['prefilter_on'] = t['prefilter']
+    if p is not None:
+        out['plain'] = {'value': round(p['samples'] / p['sec'] / 1e6, 4), 'ms_per_step': round(p['sec'] * 1e3, 4),
+                        'kernel_ms_per_step': round(sum(v['total_ms'] for v in p['kernels'].values()) / p['steps'], 4)}
+        out['speedup_vs_plain'] = round(p['sec'] / t['sec'], 2)
+    return out
+
+
+def run_headline(a):
+    """The default line: the 256-scene batch (value), the serving stream and both 512x512
+    single scenes."""
+    cpu = None if a.no_cpu_baseline else run_cpu_baseline(4)
+    b = leg_batched(a.steps, a.warmup, autotune=a.autotune)
+    s = leg_serving(a.requests, a.hbm_budget, a.max_batch)
+    singles = {}
+    for cfg, (st, wu, pst) in ((2, (30, 10, 8)), (3, (12, 4, 0))):
+        t = leg_single(cfg, st, wu, 'tiled', prefilter_steps=pst)
+        p = leg_single(cfg, 3, 1, 'plain')
+        singles[t['name']] = _single_fields(t, p)
+    out = {
+        'metric': METRIC, 'value': round(b['samples'] / b['t_graph'] / 1e6, 4), 'unit': 'Msamples/s', 'n_gpus': 1,
+        'steps': a.steps, 'warmup': a.warmup, 'ms_per_step': round(b['t_graph'] * 1e3, 4), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': DTYPE, 'data': 'synthetic',
+        'config': {'workload': 'uneven_scenes_256', 'scenes': b['scenes'], 'samples': b['samples'],
+                   'paths': b['paths'], 'spp': [2, 2], 'filter': 'box r=0.5',
+                   'mode': 'batched: forward batch + loss gradient + backward batch replayed from one HIP graph, '
+                           'scenes resident in HBM'},
+    }
+    out.update(_batched_fields(b))
+    out['serving_stream'] = _serving_fields(s)
+    out['single_scene'] = singles
+    out['roofline'] = _roofline(cpu, b['kernels'], BATCH_KERNELS, 'uneven_scenes_256')
+    out['cpu_baseline'] = _cpu_baseline(cpu)
+    print(json.dumps(out))
+
+
+def run_config(a):
+    """One workload per line (profiling runs): 2, 3 single scene; 4 batch; 5 serving."""
+    base = {'metric': METRIC, 'unit': 'Msamples/s', 'n_gpus': 1, 'higher_is_better': True, 'scaling': 'weak',
+            'vs_baseline': None, 'dtype': DTYPE, 'data': 'synthetic'}
+    if a.config == 4:
+        cpu = None if a.no_cpu_baseline else run_cpu_baseline(4)
+        b = leg_batched(a.steps, a.warmup, autotune=a.autotune)
+        out = dict(base, value=round(b['samples'] / b['t_graph'] / 1e6, 4), steps=a.steps, warmup=a.warmup,
+                   ms_per_step=round(b['t_graph'] * 1e3, 4),
+                   config={'workload': 'uneven_scenes_256', 'scenes': b['scenes'], 'samples': b['samples'],
+                           'paths': b['paths'], 'spp': [2, 2], 'filter': 'box r=0.5',
+                           'mode': 'batched, one HIP graph per fwd+bwd'})
+        out.update(_batched_fields(b))
+        out['roofline'] = _roofline(cpu, b['kernels'], BATCH_KERNELS, 'uneven_scenes_256')
+        out['cpu_baseline'] = _cpu_baseline(cpu)
+    elif a.config == 5:
+        s = leg_serving(a.requests, a.hbm_budget, a.max_batch, sweep=not a.no_sweep)
+        out = dict(base, value=round(s['samples'] / s['span_b'] / 1e6, 4), steps=s['n_req'], warmup=8,
+                   ms_per_step=round(s['span_b'] / s['n_req'] * 1e3, 4),
+                   config={'workload': 'serving_stream', 'requests': s['n_req'], 'samples': s['samples']})
+        out['serving_stream'] = _serving_fields(s)
+    else:
+        cpu = None if a.no_cpu_baseline else run_cpu_baseline(a.config)
+        t = leg_single(a.config, a.steps, a.warmup, a.mode, prefilter_steps=8 if a.config == 2 else 0)
+        f = _single_fields(t, None)
+        out = dict(base, value=f['value'], steps=a.steps, warmup=a.warmup, ms_per_step=f['ms_per_step'],
+                   config={'workload': t['name'], 'canvas': [t['w'], t['h']], 'spp': [t['nsx'], t['nsy']],
+                           'filter': 'gaussian r=1.5' if a.config == 2 else 'box r=0.5', 'paths': t['paths'],
+                           'mode': a.mode, 'prefilter': False})
+        out.update({k: v for k, v in f.items() if k not in ('value', 'ms_per_step')})
+        out['roofline'] = _roofline(cpu, t['kernels'], SINGLE_KERNELS, t['name'])
+        out['cpu_baseline'] = _cpu_baseline(cpu)
     print(json.dumps(out))
 
 
@@ -435,25 +711,29 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=None,
-                    help='timed steps (default 50 for one scene, 20 for the 256-scene batch)')
-    ap.add_argument('--warmup', type=int, default=None,
-                    help='untimed steps first (default 20 for one scene: clocks and pools settle; 3 for the batch)')
-    ap.add_argument('--config', type=int, default=2)
+                    help='timed steps of the line\'s workload (default 20 graph replays of the 256-scene batch; '
+                         '50 steps of a single scene)')
+    ap.add_argument('--warmup', type=int, default=None, help='untimed steps first (default 5; 20 for a single scene)')
+    ap.add_argument('--config', default='headline', choices=['headline', '2', '3', '4', '5'],
+                    help='headline (default): the 256-scene batch with the serving stream and the single scenes '
+                         'as extra keys; 2/3: one 512x512 scene; 4: the batch alone; 5: the serving stream alone')
     ap.add_argument('--mode', default='tiled', choices=['tiled', 'plain'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-baseline-only', action='store_true')
     ap.add_argument('--copy-calibration', action='store_true', help='run a plain copy kernel of known size')
-    ap.add_argument('--requests', type=int, default=120, help='serving stream length (config 5)')
-    ap.add_argument('--autotune', action='store_true', help='pick tile widths per workload (config 4)')
-    ap.add_argument('--hbm-budget', type=int, default=64, help='GiB that requests in flight may hold (config 5)')
-    ap.add_argument('--max-batch', type=int, default=32, help='requests per batch (config 5)')
+    ap.add_argument('--requests', type=int, default=120, help='serving stream length')
+    ap.add_argument('--no-sweep', action='store_true', help='serving: skip the arrival-rate sweep')
+    ap.add_argument('--autotune', action='store_true', help='pick tile widths per workload signature (batch)')
+    ap.add_argument('--hbm-budget', type=int, default=64, help='GiB that requests in flight may hold (serving)')
+    ap.add_argument('--max-batch', type=int, default=32, help='requests per batch (serving)')
     a = ap.parse_args()
+    single = a.config in ('2', '3')
     if a.steps is None:
-        a.steps = 20 if a.config == 4 else 50
+        a.steps = 50 if single else 20
     if a.warmup is None:
-        a.warmup = 3 if a.config == 4 else 20
+        a.warmup = 20 if single else 5
     if a.cpu_baseline_only:
-        cpu_baseline_main(a.config)
+        cpu_baseline_main(4 if a.config == 'headline' else int(a.config))
         return
     if a.copy_calibration:
         copy_calibration()
@@ -462,129 +742,11 @@ def main():
         print('bench.py measures one GPU: the render path is built and measured on one MI355X, so --gpus %d '
               'is not run (how it would shard is a design note, not a measured path)' % a.gpus, file=sys.stderr)
         sys.exit(2)
-
-    if a.config == 4:
-        bench_batched(a)
-        return
-    if a.config == 5:
-        bench_serving(a)
-        return
-    cpu = None if a.no_cpu_baseline else run_cpu_baseline(a.config)
-
-    import numpy as np
-    import torch
-    import diffvg_amd as pydiffvg
-    from diffvg_amd import workloads, _lib
-
-    dev = torch.device('cuda:0')
-    pydiffvg.set_use_gpu(True)
-    pydiffvg.set_device(dev)
-    pydiffvg.set_render_mode(a.mode)
-    c = workloads.WORKLOADS[a.config](device=dev)
-    w, h = c['canvas']
-    nsx, nsy = c['nsx'], c['nsy']
-    params = make_params(c, dev)
-    if 'target' in c:
-        target = c['target']
+    if a.config == 'headline':
+        run_headline(a)
     else:
-        target = torch.from_numpy(np.random.default_rng(a.config).uniform(0, 1, (h, w, 4)).astype(np.float32)).to(dev)
-    render = pydiffvg.RenderFunction.apply
-
-    def step(prefilter=False):
-        for p in params:
-            p.grad = None
-        args = pydiffvg.RenderFunction.serialize_scene(w, h, c['shapes'], c['groups'], c['filter'],
-                                                      use_prefiltering=prefilter)
-        img = render(w, h, nsx, nsy, 0, None, *args)
-        loss = ((img - target) ** 2).sum()
-        loss.backward()
-        return loss
-
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize()
-    L = _lib.lib()
-    L.dvg_profile_enable.argtypes = [ctypes.c_int]
-    L.dvg_profile_read.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
-                                   ctypes.POINTER(ctypes.c_int), ctypes.c_int]
-    names = ctypes.create_string_buffer(4096)
-    ms = (ctypes.c_double * 32)()
-    cnts = (ctypes.c_int * 32)()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    # per-kernel times: the same steps again with the library's events around each launch
-    # (kept out of the timed steps above, whose launches they would space out)
-    L.dvg_profile_read(names, 4096, ms, cnts, 32)
-    L.dvg_profile_enable(1)
-    for _ in range(a.steps):
-        step()
-    torch.cuda.synchronize()
-    L.dvg_profile_enable(0)
-    nk = L.dvg_profile_read(names, 4096, ms, cnts, 32)
-    kern = {}
-    for i, nm in enumerate(names.value.decode().strip().split('\n')[:nk]):
-        kern[nm] = {'total_ms': ms[i], 'launches': cnts[i], 'avg_ms': ms[i] / max(cnts[i], 1)}
-    samples = w * h * nsx * nsy
-    sec = (t1 - t0) / a.steps
-    value = samples / sec / 1e6
-    kernel_ms = sum(v['total_ms'] for v in kern.values()) / a.steps
-
-    # prefiltered variant of the same scene (reported alongside)
-    pre = None
-    if a.config == 2:
-        for _ in range(2):
-            step(True)
-        torch.cuda.synchronize()
-        n_pre = max(3, a.steps // 4)
-        tp0 = time.perf_counter()
-        for _ in range(n_pre):
-            step(True)
-        torch.cuda.synchronize()
-        tp1 = time.perf_counter()
-        pre = {'value': samples / ((tp1 - tp0) / n_pre) / 1e6, 'ms_per_step': (tp1 - tp0) / n_pre * 1e3}
-
-    roofline = None
-    if cpu and 'counts_fwd' in cpu and kern:
-        per_kernel_flops = {'k_color': flops(cpu['counts_fwd']), 'kp_render': flops(cpu['counts_fwd']),
-                            'k_color_bwd': flops(cpu['counts_bwd'], adjoint=True),
-                            'k_edge': flops(cpu['counts_edge']), 'kp_edge': flops(cpu['counts_edge'])}
-        dom = max((k for k in kern if k in per_kernel_flops), key=lambda k: kern[k]['total_ms'], default=None)
-        if dom:
-            avg_s = kern[dom]['avg_ms'] / 1e3
-            achieved = per_kernel_flops[dom] / avg_s / 1e12
-            roofline = {'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': PEAK_FP32_TFLOPS,
-                        'unit': 'TFLOP/s', 'frac': round(achieved / PEAK_FP32_TFLOPS, 6),
-                        'traffic': load_traffic(dom, c['name']),
-                        'kernel': dom, 'kernel_avg_ms': round(kern[dom]['avg_ms'], 4),
-                        'flops_per_launch': per_kernel_flops[dom],
-                        'note': 'vector FP32 peak; FLOPs are FP32-equivalent algorithmic counts'}
-    cpu_baseline = None
-    if cpu and 'fwd_s' in cpu:
-        cpu_sec = cpu['fwd_s'] + cpu['bwd_s']
-        cpu_baseline = {'value': round(cpu['samples'] / cpu_sec / 1e6, 4), 'unit': 'Msamples/s',
-                        'cores': cpu['cores'], 'kind': cpu['kind'],
-                        'sample': 'the full %dx%d %dx%d-spp scene, one forward and one backward (edge sampling), %s'
-                                  % (w, h, nsx, nsy, 'tent r=1 twin of the Gaussian filter' if a.config == 2
-                                     else 'the same box r=0.5 filter')}
-    out = {
-        'metric': 'fwd+bwd Msamples/s on 1 GPU; p50/p99 latency, batched/served vs plain',
-        'value': round(value, 4), 'unit': 'Msamples/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
-        'ms_per_step': round(sec * 1e3, 4), 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'f32 (f64 root solves)', 'data': 'synthetic',
-        'config': {'workload': c['name'], 'canvas': [w, h], 'spp': [nsx, nsy],
-                   'filter': 'gaussian r=1.5' if a.config == 2 else 'box r=0.5',
-                   'paths': len(c['shapes']), 'mode': a.mode, 'prefilter': False},
-        'kernel_ms_per_step': round(kernel_ms, 4),
-        'kernels': {k: {'avg_ms': round(v['avg_ms'], 4), 'launches': v['launches']} for k, v in kern.items()},
-        'prefilter_on': pre,
-        'roofline': roofline,
-        'cpu_baseline': cpu_baseline,
-    }
-    print(json.dumps(out))
+        a.config = int(a.config)
+        run_config(a)
 
 
 if __name__ == '__main__':

@@ -116,15 +116,19 @@ def uneven_scenes(n_scenes=256, device='cpu', seed=4):
     return out
 
 
-def serving_request(rng, device='cpu'):
-    res = int(rng.choice([128, 256, 512, 1024]))
-    n = max(1, int(math.floor(math.exp(rng.uniform(0, math.log(4096))))))
+def serving_request(rng, device='cpu', res=None, n_paths=None, sdf=None, prefilter=None):
+    """One request; `res`, `n_paths`, `sdf` and `prefilter` pin what is otherwise drawn."""
+    r_res = int(rng.choice([128, 256, 512, 1024]))
+    r_n = max(1, int(math.floor(math.exp(rng.uniform(0, math.log(4096))))))
+    res = r_res if res is None else int(res)
+    n = r_n if n_paths is None else int(n_paths)
     # the path size shrinks with the path count so that the expected overlap depth stays
     # bounded (the reference renders at most 64 prefiltered / 256 plain fragments per sample)
     spread = max(1.0, 0.6 * res / math.sqrt(n))
     shapes, groups = closed_paths(rng, n, float(res), 2, 8, spread, device, stroke_frac=0.3)
-    sdf = rng.uniform() < 0.5
-    prefilter = rng.uniform() < 0.5
+    r_sdf, r_pre = rng.uniform() < 0.5, rng.uniform() < 0.5
+    sdf = r_sdf if sdf is None else bool(sdf)
+    prefilter = r_pre if prefilter is None else bool(prefilter)
     return dict(name='serving_request', canvas=(res, res), shapes=shapes, groups=groups,
                 filter=PixelFilter(type=diffvg.FilterType.box, radius=torch.tensor(0.5)),
                 nsx=1 if sdf else 2, nsy=1 if sdf else 2, sdf=sdf, prefilter=prefilter)

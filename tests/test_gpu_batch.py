@@ -256,3 +256,116 @@ def test_batched_prefiltered_circle_stroke_is_reported():
     with pytest.raises(RuntimeError, match='prefiltered stroke'):
         pd.synchronize()
     pd.synchronize()
+
+
+def _uneven_subset(items):
+    """About ten scenes of the 256 spanning the workload: every resolution, 1 to ~500
+    paths, and gradient fills (linear and radial)."""
+    from diffvg_amd.color import LinearGradient, RadialGradient
+    info = []
+    for k, it in enumerate(items):
+        lin = sum(isinstance(g.fill_color, LinearGradient) for g in it['groups'])
+        rad = sum(isinstance(g.fill_color, RadialGradient) for g in it['groups'])
+        info.append((k, it['canvas'][0], len(it['shapes']), lin, rad))
+    pick = []
+    for res in (28, 64, 128, 256):
+        at = [x for x in info if x[1] == res]
+        pick.append(min(at, key=lambda x: (x[2], x[0]))[0])              # fewest paths
+        pick.append(max(at, key=lambda x: (x[2], -x[0]))[0])             # most paths
+    pick.append(max(info, key=lambda x: (x[3], -x[0]))[0])               # most linear gradients
+    pick.append(max(info, key=lambda x: (x[4], -x[0]))[0])               # most radial gradients
+    both = [x for x in info if x[3] and x[4]]
+    pick.append(min(both, key=lambda x: (x[2], x[0]))[0])                # both kinds, fewest paths
+    mid = [x for x in info if x[1] == 256 and 40 <= x[2] <= 120]
+    pick.append(min(mid, key=lambda x: x[0])[0])                         # a mid-sized 256 px scene
+    return sorted(set(pick))
+
+
+def test_uneven_scenes_256_graph():
+    """The batched workload at full size: 256 scenes of 1-512 closed cubic paths (2-8
+    segments, 10% linear or radial gradient fills) at 28-256 px, 2x2 samples, one forward
+    batch + L2 loss gradient + backward batch captured in one HIP graph and replayed.
+    Every scene's image equals its single-scene render bitwise and its gradients agree
+    up to atomic summation order; a subset spanning resolutions, path counts and
+    gradient fills matches the oracle (forward 1e-5, gradients 1e-3).  The graph is
+    replayed three times with eager work between the replays, each replay checked."""
+    from diffvg_amd import workloads
+    import diffvg_amd as pd
+    from diffvg_amd.render_pytorch import flatten_args
+    items = workloads.uneven_scenes(device='cpu')
+    n = len(items)
+    sizes = [it['canvas'] for it in items]
+    flats, scs = [], []
+    for it in items:
+        w, h = it['canvas']
+        f = flatten_args(pd.RenderFunction.serialize_scene(w, h, it['shapes'], it['groups'], it['filter'])).desc
+        flats.append(f)
+        scs.append(_lib.DeviceScene(f, 0))
+    npix = [w * h for w, h in sizes]
+    offs = np.concatenate([[0], np.cumsum(npix)]).astype(np.int64)
+    rng = np.random.default_rng(4)
+    target = torch.from_numpy(rng.uniform(0, 1, (int(offs[-1]), 4)).astype(np.float32)).to(DEV)
+    img = torch.zeros(int(offs[-1]), 4, device=DEV)
+    d = torch.zeros(int(offs[-1]), 4, device=DEV)
+    view = lambda t, i: t[offs[i]:offs[i + 1]].view(sizes[i][1], sizes[i][0], 4)
+    fb = Batch([(scs[i], dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0,
+                              render_image=view(img, i))) for i in range(n)], zero_grads=False)
+    bb = Batch([(scs[i], dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0,
+                              d_render_image=view(d, i))) for i in range(n)], zero_grads=True)
+    stream = torch.cuda.Stream()
+
+    def body(st):
+        img.zero_()
+        fb.render(st)
+        torch.mul(img - target, 2.0, out=d)
+        bb.render(st)
+
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        body(stream.cuda_stream)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        body(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    replays = []
+    for r in range(3):
+        print('uneven_scenes_256 replay', r, flush=True)
+        g.replay()
+        torch.cuda.synchronize()
+        fb.status()
+        bb.status()
+        replays.append((img.cpu().numpy(), d.cpu().numpy(), [sc.grads() for sc in scs]))
+        junk = torch.rand(1 << 20, device=DEV)  # eager work between replays
+        (junk * 2).sum().item()
+    img0, d0, g0 = replays[0]
+    for img_r, d_r, g_r in replays[1:]:
+        assert np.array_equal(img_r, img0)
+        for i in range(n):
+            for k in GRAD_KEYS:
+                if g0[i][k].size:
+                    assert _rel(g_r[i][k], g0[i][k]) <= 1e-4, (i, k)
+    # every scene against its single-scene render
+    for i in range(n):
+        w, h = sizes[i]
+        one = torch.zeros(h, w, 4, device=DEV)
+        scs[i].render(w, h, 2, 2, 0, render_image=one)
+        scs[i].zero_grads(None)
+        scs[i].render(w, h, 2, 2, 0, d_render_image=torch.from_numpy(d0[offs[i]:offs[i + 1]].reshape(h, w, 4)).to(DEV))
+        torch.cuda.synchronize()
+        assert np.array_equal(one.cpu().numpy().reshape(-1, 4), img0[offs[i]:offs[i + 1]]), i
+        gs = scs[i].grads()
+        for k in GRAD_KEYS:
+            if gs[k].size:
+                assert _rel(g0[i][k], gs[k]) <= 1e-4, (i, k)
+    # a subset against the oracle
+    for i in _uneven_subset(items):
+        w, h = sizes[i]
+        print('uneven scene', i, 'vs oracle:', w, 'px', flats[i]['num_shapes'], 'paths', flush=True)
+        orc = OracleRunner(flats[i], threads=8)
+        want = orc.render(w, h, 2, 2, 0)['image']
+        assert np.abs(img0[offs[i]:offs[i + 1]].reshape(h, w, 4) - want).max() <= 1e-5, i
+        gw = orc.render(w, h, 2, 2, 0, d_image=d0[offs[i]:offs[i + 1]].reshape(h, w, 4))['grads']
+        for k in GRAD_KEYS:
+            if gw[k].size:
+                assert _rel(g0[i][k], gw[k]) <= 1e-3, (i, k)
