@@ -438,7 +438,9 @@ DVG_DEV void weight_pixel(const DevScene &S, const Geom &G, int pix, float *weig
             for (int sy = 0; sy < G.nsy; sy++)
                 for (int sx = 0; sx < G.nsx; sx++) {
                     int idx = ((y * G.W + x) * G.nsy + sy) * G.nsx + sx;
-                    V2 pt = sample_pos(G, x, y, sx, sy, idx, false);  // always jittered (Q1)
+                    // always jittered: the reference's weight_kernel never sets
+                    // use_prefiltering (diffvg.cpp:1525-1533)
+                    V2 pt = sample_pos(G, x, y, sx, sy, idx, false);
                     acc += filter_weight(S.filter_type, radius, xc - pt.x, yc - pt.y);
                 }
         }

@@ -119,6 +119,16 @@ void orc_set_counting(int on);
 /* Counters of the last render; phase 0 = weight and render kernels, 1 = boundary
  * sampling and edge kernel. */
 void orc_get_counters(int phase, orc_counters *out);
+/* While counting: samples of the last render whose distance queries reached the
+ * isolating-interval slot that the reference reads uninitialised when the isolator
+ * root -B/5 lies outside [0, 1] (within_distance.h:195-202, compute_distance.h:200-207).
+ * phase 0 = interior samples, 1 = edge samples that contributed a gradient.  The
+ * reference's result for such a sample depends on whatever that stack slot holds; the
+ * oracle treats the slot as absent. */
+int64_t orc_slot_samples(int phase);
+/* The phase-1 tally split by the shape each edge sample was drawn on; returns the
+ * number of entries written (min(n, number of shapes)). */
+int orc_slot_edge_samples_by_shape(int64_t *out, int n);
 /* Evaluate only every stride-th sample in each pass (for cost counting on a sample;
  * the images and gradients are then incomplete). */
 void orc_set_sample_stride(int stride);

@@ -31,6 +31,10 @@ extern thread_local Counters *g_tc;
 inline void bump(int i, int64_t v = 1) {
     if (g_tc) g_tc->c[i] += v;
 }
+// Set when a query of the current sample reaches the isolating-interval slot that the
+// reference leaves uninitialised (see quintic_intervals); read by the render loops
+// to tally the samples whose result may depend on that slot.
+extern thread_local bool t_q8_slot;
 enum {
     kCntSamples = 0,
     kCntTopBox,
@@ -279,7 +283,10 @@ inline int quintic_intervals(const Quintic &q, float *intervals) {
     double p_roots[3];
     int num_sol = solve_cubic_d(p1A, p1B, p1C, p1D, p_roots);
     int n = 0;
-    if (q_root >= 0 && q_root <= 1) intervals[n++] = (float)q_root;
+    if (q_root >= 0 && q_root <= 1)
+        intervals[n++] = (float)q_root;
+    else
+        t_q8_slot = true;
     for (int j = 0; j < num_sol; j++) intervals[n++] = (float)p_roots[j];
     for (int j = 1; j < n; j++) {
         for (int k = j; k > 0 && intervals[k - 1] > intervals[k]; k--) {

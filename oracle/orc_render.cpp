@@ -423,6 +423,18 @@ struct Frag {
 };
 static const int kMaxFrags = 256;
 static thread_local bool t_overflow = false;
+thread_local bool t_q8_slot = false;
+// Tallies (while counting) of the samples whose queries reached the interval slot that
+// the reference leaves uninitialised: [0] interior samples, [1] contributing edge
+// samples; and the contributing edge samples by the shape they were drawn on.
+static int64_t g_slot_samples[2];
+static std::vector<int64_t> g_slot_edge_by_shape;
+static void tally_slot(int phase, int shape_id) {
+    if (!g_counting || !t_q8_slot) return;
+    __atomic_fetch_add(&g_slot_samples[phase], 1, __ATOMIC_RELAXED);
+    if (shape_id >= 0 && shape_id < (int)g_slot_edge_by_shape.size())
+        __atomic_fetch_add(&g_slot_edge_by_shape[shape_id], 1, __ATOMIC_RELAXED);
+}
 
 // sample_color(scene, ...) (diffvg.cpp:525-707)
 static V4 sample_color_scene(Scene &S, const V4 *bg, V2 screen_pt, const V4 *d_color, EdgeQuery *eq, V4 *d_bg,
@@ -1180,6 +1192,7 @@ static int render(Scene &S, const orc_render_args &a) {
         parallel_for(
             [&](int idx) {
                 bump(kCntSamples);
+                t_q8_slot = false;
                 V2 pt{0, 0};
                 int x = 0, y = 0;
                 if (!eval_mode) {
@@ -1246,6 +1259,7 @@ static int render(Scene &S, const orc_render_args &a) {
                             aadd(&a.render_sdf[y * W + x], dd);
                     }
                 }
+                tally_slot(0, -1);
                 if (t_overflow) overflow = true;
             },
             num_samples);
@@ -1291,6 +1305,7 @@ static int render(Scene &S, const orc_render_args &a) {
                 V2 normal = b.normal;
                 EdgeQuery iq{b.group_id, b.shape_id, false};
                 EdgeQuery oq{b.group_id, b.shape_id, false};
+                t_q8_slot = false;
                 const V4 *bg = a.background_image ? (const V4 *)&a.background_image[4 * (by * W + bx)] : nullptr;
                 V4 ci = sample_color_scene(S, bg, b.pt - 1e-4f * normal, nullptr, &iq, nullptr, nullptr);
                 V4 co = sample_color_scene(S, bg, b.pt + 1e-4f * normal, nullptr, &oq, nullptr, nullptr);
@@ -1307,6 +1322,7 @@ static int render(Scene &S, const orc_render_args &a) {
                 d_color = vdiv(d_color, (float)(S.cw * S.ch));
                 float contrib = dot(ci - co, d_color) / b.pdf;
                 accumulate_boundary_gradient(S, b.group_id, b.shape_id, contrib, b.t, normal, b.data, b.local_pt);
+                tally_slot(1, b.shape_id);
                 if (a.d_translation) {
                     aadd(&a.d_translation[2 * (by * W + bx) + 0], normal.x * contrib);
                     aadd(&a.d_translation[2 * (by * W + bx) + 1], normal.y * contrib);
@@ -1362,7 +1378,11 @@ int orc_render(orc_scene *scene, const orc_render_args *args) {
         return 1;
     }
     t_overflow = false;
-    if (g_counting) g_counters[0] = g_counters[1] = Counters{};
+    if (g_counting) {
+        g_counters[0] = g_counters[1] = Counters{};
+        g_slot_samples[0] = g_slot_samples[1] = 0;
+        g_slot_edge_by_shape.assign(scene->s.ns, 0);
+    }
     return render(scene->s, *args);
 }
 
@@ -1399,6 +1419,12 @@ int orc_get_num_threads(void) {
 }
 void orc_set_counting(int on) { g_counting = on != 0; }
 void orc_set_sample_stride(int stride) { g_stride = stride < 1 ? 1 : stride; }
+int64_t orc_slot_samples(int phase) { return g_slot_samples[phase & 1]; }
+int orc_slot_edge_samples_by_shape(int64_t *out, int n) {
+    int m = std::min(n, (int)g_slot_edge_by_shape.size());
+    for (int i = 0; i < m; i++) out[i] = g_slot_edge_by_shape[i];
+    return m;
+}
 void orc_get_counters(int phase, orc_counters *out) {
     int64_t *o = (int64_t *)out;
     for (int i = 0; i < 12; i++) o[i] = g_counters[phase & 1].c[i];

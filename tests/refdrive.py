@@ -261,6 +261,9 @@ def load_oracle():
         lib.orc_set_counting.argtypes = [ctypes.c_int]
         lib.orc_set_sample_stride.argtypes = [ctypes.c_int]
         lib.orc_get_counters.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int64 * 12)]
+        lib.orc_slot_samples.argtypes = [ctypes.c_int]
+        lib.orc_slot_samples.restype = ctypes.c_int64
+        lib.orc_slot_edge_samples_by_shape.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.c_int]
         _oracle = lib
     return _oracle
 
@@ -342,14 +345,81 @@ class OracleRunner:
         return out
 
 
-def load_ref():
-    """Import the reference CPU build (oracle/_ref/diffvg.so) or return None."""
-    if not os.path.exists(os.path.join(REF_DIR, 'diffvg.so')):
+def load_ref(build=None):
+    """Import the reference CPU build (oracle/_ref/diffvg.so, or oracle/_ref/<build>/diffvg.so)
+    or return None.  Both builds define the same pybind11 module and classes, so one
+    process can hold only one of them: run the other with ``ref_render_in_child``."""
+    d = REF_DIR if build is None else os.path.join(REF_DIR, build)
+    if not os.path.exists(os.path.join(d, 'diffvg.so')):
         return None
-    if REF_DIR not in sys.path:
-        sys.path.insert(0, REF_DIR)
+    if d not in sys.path:
+        sys.path.insert(0, d)
     import diffvg as dv  # noqa: E402  (the reference module, compiled here)
+    if os.path.dirname(os.path.abspath(dv.__file__)) != os.path.abspath(d):
+        raise RuntimeError('another reference build is already loaded: %s' % dv.__file__)
     return dv
+
+
+def ref_build_exists(build=None):
+    d = REF_DIR if build is None else os.path.join(REF_DIR, build)
+    return os.path.exists(os.path.join(d, 'diffvg.so'))
+
+
+def ref_render_in_child(flat, build, width, height, nsx, nsy, seed, runs):
+    """Render with the reference build ``build`` (None = oracle/_ref) in a child Python
+    process.  ``runs`` is a list of keyword dicts for RefRunner.render; returns the list
+    of their outputs (image, and grads for a backward)."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        arrays = {'flat__' + k: np.asarray(v) for k, v in flat.items()}
+        for i, kw in enumerate(runs):
+            for k, v in kw.items():
+                arrays['run%d__%s' % (i, k)] = np.asarray(v)
+        np.savez(os.path.join(td, 'in.npz'), **arrays)
+        cmd = [sys.executable, os.path.abspath(__file__), os.path.join(td, 'in.npz'), os.path.join(td, 'out.npz'),
+               '' if build is None else build, str(width), str(height), str(nsx), str(nsy), str(seed), str(len(runs))]
+        subprocess.run(cmd, check=True)
+        with np.load(os.path.join(td, 'out.npz')) as z:
+            out = [dict() for _ in runs]
+            for k in z.files:
+                i, rest = k.split('__', 1)
+                i = int(i[3:])
+                if rest.startswith('grad_'):
+                    out[i].setdefault('grads', {})[rest[5:]] = z[k]
+                else:
+                    out[i][rest] = z[k]
+        return out
+
+
+def _child_main(argv):
+    src, dst, build, w, h, nsx, nsy, seed, nruns = argv
+    w, h, nsx, nsy, seed, nruns = int(w), int(h), int(nsx), int(nsy), int(seed), int(nruns)
+    with np.load(src) as z:
+        flat = {}
+        runs = [dict() for _ in range(nruns)]
+        for k in z.files:
+            head, name = k.split('__', 1)
+            v = z[k]
+            if v.ndim == 0:
+                v = v.item()
+            if head == 'flat':
+                flat[name] = v
+            else:
+                runs[int(head[3:])][name] = v
+    dv = load_ref(build or None)
+    if dv is None:
+        raise SystemExit('reference build %r not found' % build)
+    r = RefRunner(flat, dv=dv)
+    res = {}
+    for i, kw in enumerate(runs):
+        o = r.render(w, h, nsx, nsy, seed, **kw)
+        for k in ('image', 'sdf'):
+            if o.get(k) is not None:
+                res['run%d__%s' % (i, k)] = o[k]
+        for k, v in o.get('grads', {}).items():
+            res['run%d__grad_%s' % (i, k)] = v
+    np.savez(dst, **res)
 
 
 class RefRunner:
@@ -532,3 +602,7 @@ class RefRunner:
             g['shape_to_canvas'][9 * gi:9 * gi + 9] = m
         g['filter_radius'] = np.array([sc.get_d_filter_radius()], np.float32)
         return g
+
+
+if __name__ == '__main__':
+    _child_main(sys.argv[1:])

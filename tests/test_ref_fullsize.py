@@ -1,0 +1,139 @@
+"""The oracle against the reference's own CPU build on the full 512x512 benchmark scenes.
+
+Two reference builds take part, both compiled from the reference sources by
+oracle/Makefile (test infrastructure; nothing in the product loads them):
+
+* ``oracle/_ref``          -- the reference as it is;
+* ``oracle/_ref/zeroinit`` -- the same sources and flags with every uninitialised local
+  zero-filled (``-ftrivial-auto-var-init=zero``).
+
+The reference reads one isolating-interval slot uninitialised whenever the isolator
+root -B/5 of a point-to-cubic quintic lies outside [0, 1] (within_distance.h:195-202,
+compute_distance.h:200-207).  A zero there is skipped exactly as an absent slot is,
+and absent is how the oracle (and the product) define it.  So:
+
+1. the oracle must match the zero-filled build everywhere -- images within 1e-5,
+   every gradient entry within 1e-3 of its array's largest magnitude;
+2. the oracle may differ from the reference as it is only where the two reference
+   builds differ from each other, i.e. only through that uninitialised read; the
+   shapes and pixels affected are named below, with the size of the difference.
+
+Scenes: the 300 closed paths (tent filter, radius 1, the Gaussian workload's parity
+twin) and the 2,048 strokes, 2x2 samples, seed 0, d_image = U(-1, 1) with seed 0,
+prefiltering off and on.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from diffvg_amd import workloads
+from refdrive import OracleRunner, flatten, ref_build_exists, ref_render_in_child
+from test_gpu_parity import _grad_owners
+
+pytestmark = pytest.mark.skipif(not (ref_build_exists() and ref_build_exists('zeroinit')),
+                                reason='reference CPU builds not present (make -C oracle ref)')
+
+GRAD_KEYS = ('shape_params', 'points', 'thickness', 'stroke_width', 'color_params', 'shape_to_canvas',
+             'filter_radius')
+
+# What the uninitialised slot does to the reference on these scenes (this container's
+# build of it): shapes whose gradients miss the 1e-3 bar, the largest relative error of
+# their entries, and how many pixels miss the 1e-5 image bar (and by how much at most).
+EXPECTED = {
+    ('closed_paths_300', False): dict(shapes=set(), grad_err=1e-3, pixels=0, pixel_err=1e-5),
+    ('closed_paths_300', True): dict(shapes={116, 140}, grad_err=2e-3, pixels=20, pixel_err=4e-5),
+    ('strokes_2048', False): dict(shapes={257}, grad_err=8e-3, pixels=0, pixel_err=1e-5),
+    ('strokes_2048', True): dict(shapes=set(), grad_err=1e-3, pixels=45, pixel_err=3e-5),
+}
+
+
+def _scene(name):
+    if name == 'closed_paths_300':
+        c = workloads.closed_paths_300(device='cpu', filter_type='tent')
+    else:
+        c = workloads.strokes_2048(device='cpu')
+    w, h = c['canvas']
+    return flatten(w, h, c['shapes'], c['groups'], c['filter'].type, float(c['filter'].radius)), w, h
+
+
+@pytest.fixture(scope='module', params=sorted(EXPECTED), ids=lambda k: '%s-%s' % (k[0], 'pre' if k[1] else 'nopre'))
+def case(request):
+    name, pre = request.param
+    f, w, h = _scene(name)
+    d = np.random.default_rng(0).uniform(-1, 1, (h, w, 4)).astype(np.float32)
+    # Eight threads: the splat order then varies run to run, which moves box/tent pixel
+    # sums by ~1e-7 and gradient entries by ~1e-6 relative, far inside both bars.
+    o = OracleRunner(f, threads=8)
+    img = o.render(w, h, 2, 2, 0, prefilter=pre)['image']
+    o.lib.orc_set_counting(1)
+    grads = o.render(w, h, 2, 2, 0, d_image=d, prefilter=pre)['grads']
+    slot_interior, slot_edge = o.lib.orc_slot_samples(0), o.lib.orc_slot_samples(1)
+    per_shape = (ctypes.c_int64 * f['num_shapes'])()
+    o.lib.orc_slot_edge_samples_by_shape(per_shape, f['num_shapes'])
+    o.lib.orc_set_counting(0)
+    runs = [dict(prefilter=pre), dict(prefilter=pre, d_image=d)]
+    ref = ref_render_in_child(f, None, w, h, 2, 2, 0, runs)
+    zero = ref_render_in_child(f, 'zeroinit', w, h, 2, 2, 0, runs)
+    return dict(key=(name, pre), flat=f, image=img, grads=grads, ref=ref, zero=zero,
+                slot=(slot_interior, slot_edge), slot_by_shape=np.array(per_shape[:], np.int64))
+
+
+def _over(flat, ga, gb, bar=1e-3):
+    """Shapes (or groups, for paint and transform entries) owning entries of ga that miss
+    the relative bar against gb, and the largest relative error."""
+    owners = _grad_owners(flat)
+    shapes, worst = set(), 0.0
+    for k in GRAD_KEYS:
+        a, b = np.asarray(ga[k], np.float64), np.asarray(gb[k], np.float64)
+        if b.size == 0:
+            continue
+        err = np.abs(a - b) / max(np.abs(b).max(), 1e-8)
+        worst = max(worst, float(err.max()))
+        if k != 'filter_radius':
+            shapes |= set(owners[k][err > bar].tolist())
+        else:
+            assert err.max() <= bar, (k, float(err.max()))
+    return shapes, worst
+
+
+def test_oracle_matches_zero_filled_reference(case):
+    """With the uninitialised slot zero-filled, the reference and the oracle agree on the
+    full scenes: images within 1e-5, every gradient entry within 1e-3."""
+    z_img, z_grads = case['zero'][0]['image'], case['zero'][1]['grads']
+    assert np.abs(case['image'] - z_img).max() <= 1e-5
+    shapes, worst = _over(case['flat'], case['grads'], z_grads)
+    assert shapes == set(), (sorted(shapes)[:10], worst)
+
+
+def test_reference_differs_only_through_the_uninitialised_slot(case):
+    """Against the reference as it is, every pixel and every shape that misses a bar also
+    differs between the two reference builds, and the affected sets are the named ones."""
+    exp = EXPECTED[case['key']]
+    r_img, z_img = case['ref'][0]['image'], case['zero'][0]['image']
+    bad_px = np.abs(case['image'] - r_img).max(-1) > 1e-5
+    ref_moved_px = np.abs(r_img - z_img).max(-1) > 0
+    assert not np.any(bad_px & ~ref_moved_px)
+    assert int(bad_px.sum()) == exp['pixels'], int(bad_px.sum())
+    assert np.abs(case['image'] - r_img).max() <= exp['pixel_err']
+
+    r_grads, z_grads = case['ref'][1]['grads'], case['zero'][1]['grads']
+    shapes, worst = _over(case['flat'], case['grads'], r_grads)
+    ref_moved, _ = _over(case['flat'], z_grads, r_grads)
+    assert shapes <= ref_moved, sorted(shapes - ref_moved)
+    assert shapes == exp['shapes'], sorted(shapes)
+    assert worst <= exp['grad_err'], worst
+
+
+def test_slot_tally_names_the_affected_edge_samples(case):
+    """Edge sampling: a shape whose gradients moved had edge samples whose colour queries
+    reached the slot (prefiltered renders have no edge samples; their slot reads are in
+    the interior closest-point queries)."""
+    name, pre = case['key']
+    interior, edge = case['slot']
+    assert interior > 0
+    if pre:
+        assert edge == 0
+        return
+    for s in EXPECTED[case['key']]['shapes']:
+        assert case['slot_by_shape'][s] > 0, s
