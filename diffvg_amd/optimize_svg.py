@@ -1,29 +1,38 @@
-"""SVG documents as trainable scenes: ``SvgOptimizationSettings`` and ``OptimizableSvg``
-with the names, arguments and results of pydiffvg/optimize_svg.py.
+"""SVG documents as trainable scenes: ``SvgOptimizationSettings`` and ``OptimizableSvg``.
 
-An ``OptimizableSvg`` parses a document into a tree of nodes (root, groups, shapes,
-gradients kept in ``defs``).  Every node owns small torch optimizers over its own
-parameters -- transform (through a rotation/scale/shear/translation decomposition),
-solid colours, path points, gradient stops and end points -- configured per node id by
-the settings.  ``build_scene`` walks the tree, composing transforms and inherited
-appearance, into the (width, height, shapes, shape_groups) tuple that
-``RenderFunction.serialize_scene`` takes; ``render`` draws it with 2x2 samples through
-the GPU path; ``step``/``zero_grad`` drive all optimizers; ``write_xml`` writes the
-document back.
+Interface kept from pydiffvg/optimize_svg.py (so that its callers, e.g.
+apps/svg_brush.py:96-102 and apps/simple_transform_svg.py:90-104, switch over): the
+settings schema and the settings methods, ``OptimizableSvg(filename, settings,
+optimize_background, verbose, device)`` with ``build_scene``, ``render``, ``zero_grad``,
+``step`` and ``write_xml``, and ``OptimizableSvg.TransformTools.decompose/recompose``
+(apps/svg_parse_test.py:12-22).
 
-Where the reference raises on its own inputs, this module computes what the code
-plainly means instead (each case is marked "reference:" below).  Other reference
-behaviours are kept as they are because they change results: local presentation
-attributes override ``style`` declarations, style keys are not stripped, the root's own
-appearance is not inherited by its children, rects/circles/ellipses/polygons get an
-optimizer but never ``requires_grad`` (so they do not move), and ``stroke-width`` is used
-as given, not halved as ``svg_to_scene`` does.
+Design.  A document becomes a tree of ``SvgNode`` records (one class; ``kind`` is the
+SVG tag).  Every trainable tensor -- solid colours, opacities, path points, gradient
+stops and end points, transform parameters -- is registered once in a document-wide
+``ParameterBank`` together with its learning rate, optimizer family and the projection
+that keeps it valid after a step (colours clamped to [1e-4, 1]; stop offsets clamped,
+sorted and pinned to 0 and 1).  The bank owns one torch optimizer per family, with a
+parameter group per tensor, so a step is one call per family plus the projections.
+``build_scene`` walks the tree once per step, composing transforms and cascading the
+appearance, and emits the (width, height, shapes, shape_groups) tuple that
+``RenderFunction.serialize_scene`` takes.  Transforms train through a decomposition
+M = R(theta) diag(sx, sy) shearX(k) translate(t) whose free parts depend on the
+settings' transform mode.
+
+Behaviours of the reference that change results are kept: presentation attributes beat
+``style`` declarations, which beat class rules; style keys are not stripped (so a
+declaration after "; " is not recognised); the root element's own appearance is not
+inherited; rect, circle, ellipse and polygon geometry is not trained; stroke-width is
+used as given; skewX/skewY take the shear coefficient itself, not an angle; lengths use
+1 px = 0.25 mm.
 """
 import copy
 import json
 import math
 import re
-import xml.etree.ElementTree as etree
+import warnings
+import xml.etree.ElementTree as ET
 from collections import namedtuple
 from xml.dom import minidom
 
@@ -37,9 +46,10 @@ from .shape import Circle, Ellipse, Path, Polygon, Rect, ShapeGroup, from_svg_pa
 __all__ = ['SvgOptimizationSettings', 'OptimizableSvg']
 
 
+# ------------------------------------------------------------------------------ settings
 class SvgOptimizationSettings:
-    """Per-node optimisation settings keyed by SVG id, with a "default" entry
-    (optimize_svg.py:15-103).  Stored as plain nested dicts so they save to JSON."""
+    """Optimisation settings per SVG id, falling back to the "default" entry
+    (optimize_svg.py:15-103).  Plain nested dicts, so they round-trip through JSON."""
 
     default_params = {
         "optimize_color": True,
@@ -74,1338 +84,839 @@ class SvgOptimizationSettings:
         },
     }
 
-    optims = {
-        "Adam": torch.optim.Adam,
-        "SGD": torch.optim.SGD,
-        "ASGD": torch.optim.ASGD,
-    }
+    optims = {"Adam": torch.optim.Adam, "SGD": torch.optim.SGD, "ASGD": torch.optim.ASGD}
 
     def __init__(self, f=None):
-        if f is None:
-            self.store = {"default": copy.deepcopy(SvgOptimizationSettings.default_params)}
-        else:
-            self.store = json.load(f)
+        self.store = json.load(f) if f is not None else {"default": copy.deepcopy(self.default_params)}
+        self._root_name = None
 
     def default_name(self, dname):
-        """Names the entry `save` writes back as "default" (the root's id)."""
-        self.dname = dname
-        if dname not in self.store:
-            self.store[dname] = self.store["default"]
+        """Marks `dname` (the document root's id) as the entry `save` writes as "default"."""
+        self._root_name = dname
+        self.store.setdefault(dname, self.store["default"])
 
     def retrieve(self, node_id):
-        """-> (settings for node_id or the defaults, whether node_id has its own)."""
-        if node_id in self.store:
-            return self.store[node_id], True
-        return self.store["default"], False
+        """-> (the entry for node_id, or the defaults; whether node_id has its own)."""
+        own = node_id in self.store
+        return self.store[node_id if own else "default"], own
 
     def reset_to_defaults(self, node_id):
         self.store.pop(node_id, None)
         return self.store["default"]
 
     def undefault(self, node_id):
-        """Gives node_id its own copy of the defaults (to be edited) and returns it."""
+        """Gives node_id an editable copy of the defaults (if it has no entry) and returns it."""
         if node_id not in self.store:
             self.store[node_id] = copy.deepcopy(self.store["default"])
         return self.store[node_id]
 
     def override_optimizer(self, optimizer):
         if optimizer is not None:
-            for entry in self.store.values():
-                entry["optimizer"] = optimizer
+            self.global_override(["optimizer"], optimizer)
 
     def global_override(self, path, value):
-        """Sets entry[path[0]]...[path[-1]] = value in every entry."""
+        """Sets the entry at key path `path` to `value` in every stored entry."""
         for entry in self.store.values():
-            d = entry
             for key in path[:-1]:
-                d = d[key]
-            d[path[-1]] = value
+                entry = entry[key]
+            entry[path[-1]] = value
 
     def save(self, file):
-        # reference: `save` needs `default_name` to have been called first; without it
-        # the "default" entry is written as it is
-        if hasattr(self, 'dname'):
-            self.store["default"] = self.store[self.dname]
+        if self._root_name is not None:
+            self.store["default"] = self.store[self._root_name]
         json.dump(self.store, file, indent="\t")
 
 
-# ---------------------------------------------------------------------------------------
-# 2-D affine transforms as 3x3 matrices (optimize_svg.py:108-326)
-# ---------------------------------------------------------------------------------------
+# ------------------------------------------------------------------- affine 2-D algebra
 TransformDecomposition = namedtuple("TransformDecomposition", "theta scale shear translate")
-TransformProperties = namedtuple("TransformProperties",
-                                 "has_rotation has_scale has_mirror scale_uniform has_shear has_translation")
 
 
-def _eye3(device=None):
-    return torch.eye(3, device=device)
+def _affine(a, b, c, d, e, f):
+    """The SVG matrix(a b c d e f) as a 3x3 array."""
+    return np.array([[a, c, e], [b, d, f], [0.0, 0.0, 1.0]], dtype=np.float64)
+
+
+def _rotation(v):
+    r = math.radians(v[0])
+    m = _affine(math.cos(r), math.sin(r), -math.sin(r), math.cos(r), 0.0, 0.0)
+    if len(v) == 3:  # about (cx, cy): move the centre to the origin, rotate, move back
+        cx, cy = v[1], v[2]
+        m = _affine(1, 0, 0, 1, cx, cy) @ m @ _affine(1, 0, 0, 1, -cx, -cy)
+    return m
+
+
+# name -> (accepted argument counts, matrix of the arguments)
+_TRANSFORM_FUNCTIONS = {
+    "matrix": ((6,), lambda v: _affine(*v)),
+    "translate": ((1, 2), lambda v: _affine(1, 0, 0, 1, v[0], v[1] if len(v) == 2 else 0.0)),
+    "scale": ((1, 2), lambda v: _affine(v[0], 0, 0, v[-1], 0, 0)),
+    "rotate": ((1, 3), _rotation),
+    "skewX": ((1,), lambda v: _affine(1, 0, v[0], 1, 0, 0)),
+    "skewY": ((1,), lambda v: _affine(1, v[0], 0, 1, 0, 0)),
+}
+_TRANSFORM_ITEM = re.compile(r"\s*,?\s*([A-Za-z]+)\s*\(([^)]*)\)")
+_NUMBER = r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?"
+
+
+def parse_transform_list(text):
+    """An SVG transform list -> 3x3 float64 matrix (the functions multiplied left to right)."""
+    m, pos = np.eye(3), 0
+    text = text.rstrip()
+    while pos < len(text):
+        item = _TRANSFORM_ITEM.match(text, pos)
+        if item is None:
+            raise ValueError("Cannot parse transform '{}'".format(text))
+        name, args = item.group(1), [float(x) for x in re.findall(_NUMBER, item.group(2))]
+        if name not in _TRANSFORM_FUNCTIONS:
+            raise ValueError("Unknown transform type '{}'".format(name))
+        counts, fn = _TRANSFORM_FUNCTIONS[name]
+        if len(args) not in counts:
+            raise ValueError("{} takes {} arguments, got {}".format(name, counts, len(args)))
+        m = m @ fn(args)
+        pos = item.end()
+    return m
+
+
+def _fmt(x):
+    """A number as the shortest text that reads back to the same float32."""
+    return "%.9g" % float(x)
 
 
 class TransformTools:
-    """SVG transform functions as matrices, and the decomposition
-    M = [R(theta) diag(sx, sy) [[1, k], [0, 1]] | 0; 0 0 1] @ translate(tx, ty)
-    that the transform optimizers work in."""
+    """The decomposition the transform parameters train in:
+    M = [R(theta) diag(sx, sy) [[1, k], [0, 1]] | 0; 0 0 1] @ translate(tx, ty), sx > 0."""
 
     TransformDecomposition = TransformDecomposition
-    TransformProperties = TransformProperties
-
-    @staticmethod
-    def parse_matrix(vals):
-        assert len(vals) == 6
-        a, b, c, d, e, f = vals
-        return np.array([[a, c, e], [b, d, f], [0., 0., 1.]])
-
-    @staticmethod
-    def parse_translate(vals):
-        assert 1 <= len(vals) <= 2
-        m = np.eye(3)
-        m[0, 2] = vals[0]
-        if len(vals) == 2:
-            m[1, 2] = vals[1]
-        return m
-
-    @staticmethod
-    def parse_rotate(vals):
-        """rotate(deg) or rotate(deg cx cy), the latter about (cx, cy)."""
-        assert len(vals) in (1, 3)
-        rad = math.radians(vals[0])
-        c, s = math.cos(rad), math.sin(rad)
-        m = np.eye(3)
-        m[0:2, 0:2] = [[c, -s], [s, c]]
-        if len(vals) == 3:
-            # reference: calls an unqualified parse_translate here and raises NameError
-            m = TransformTools.parse_translate(vals[1:3]) @ m @ TransformTools.parse_translate([-vals[1], -vals[2]])
-        return m
-
-    @staticmethod
-    def parse_scale(vals):
-        assert 1 <= len(vals) <= 2
-        return np.diag([vals[0], vals[-1], 1.])
-
-    @staticmethod
-    def parse_skewx(vals):
-        """The coefficient itself goes into the matrix (not tan of an angle), which is
-        also how the decomposition's shear is written back."""
-        assert len(vals) == 1
-        m = np.eye(3)
-        m[0, 1] = vals[0]
-        return m
-
-    @staticmethod
-    def parse_skewy(vals):
-        assert len(vals) == 1
-        m = np.eye(3)
-        m[1, 0] = vals[0]
-        return m
-
-    @staticmethod
-    def transformPoints(pointsTensor, transform):
-        """[n, 2] points through a 3x3 matrix (affine part; no projective divide)."""
-        assert transform is not None
-        out = pointsTensor @ transform[0:2, 0:2].t() + transform[0:2, 2]
-        assert out.shape == pointsTensor.shape
-        return out.contiguous()
-
-    @staticmethod
-    def promote_numpy(M):
-        ret = np.eye(3)
-        ret[0:2, 0:2] = M
-        return ret
-
-    @staticmethod
-    def recompose_numpy(Theta, ScaleXY, ShearX, TXY):
-        c, s = math.cos(Theta), math.sin(Theta)
-        lin = np.array([[c, -s], [s, c]]) @ np.diag(ScaleXY) @ np.array([[1., ShearX], [0., 1.]])
-        T = np.eye(3)
-        T[0:2, 2] = TXY
-        return TransformTools.promote_numpy(lin) @ T
-
-    @staticmethod
-    def promote(m):
-        M = torch.eye(3, device=m.device, dtype=m.dtype)
-        M[0:2, 0:2] = m
-        return M
-
-    @staticmethod
-    def make_rot(Theta):
-        c, s = Theta.cos().squeeze(), Theta.sin().squeeze()
-        return torch.stack((torch.stack((c, -s)), torch.stack((s, c))))
-
-    @staticmethod
-    def make_scale(ScaleXY):
-        sxy = ScaleXY.squeeze()
-        if sxy.dim() == 0:
-            return torch.diag(torch.stack([sxy, sxy]))
-        return torch.diag(sxy)
-
-    @staticmethod
-    def make_shear(ShearX):
-        m = torch.eye(2, device=ShearX.device)
-        m[0, 1] = ShearX
-        return m
-
-    @staticmethod
-    def make_translate(TXY):
-        m = torch.eye(3, device=TXY.device)
-        m[0:2, 2] = TXY
-        return m
-
-    @staticmethod
-    def recompose(Theta, ScaleXY, ShearX, TXY):
-        """Differentiable inverse of `decompose`."""
-        lin = TransformTools.make_rot(Theta).mm(TransformTools.make_scale(ScaleXY)).mm(TransformTools.make_shear(ShearX))
-        return TransformTools.promote(lin).mm(TransformTools.make_translate(TXY))
-
-    @staticmethod
-    def make_named(decomp):
-        if not isinstance(decomp, TransformDecomposition):
-            decomp = TransformDecomposition(theta=decomp[0], scale=decomp[1], shear=decomp[2], translate=decomp[3])
-        return decomp
-
-    @staticmethod
-    def analyze_transform(decomp):
-        """Which parts of a decomposition differ from identity by more than 1e-3."""
-        d = TransformTools.make_named(decomp)
-        eps = 1e-3
-        scale = d.scale
-        two = (scale.squeeze().ndim > 0) if isinstance(scale, np.ndarray) else (
-            scale.squeeze().dim() > 0 if torch.is_tensor(scale) else len(scale) > 1)
-        absd = (lambda v: abs(v))
-        if two:
-            s0, s1 = scale[0], scale[1]
-            has_scale = max(abs(absd(s0) - 1), abs(absd(s1) - 1)) > eps
-            has_mirror = bool(s0 * s1 < 0)
-            uniform = bool(abs(absd(s0) - absd(s1)) < eps)
-        else:
-            has_scale = abs(absd(scale) - 1) > eps
-            has_mirror = False
-            uniform = True
-        return TransformProperties(
-            has_rotation=bool(abs(d.theta) > eps),
-            has_scale=bool(has_scale),
-            has_mirror=has_mirror,
-            scale_uniform=uniform,
-            has_shear=bool(abs(d.shear) > eps),
-            has_translation=bool(max(abs(d.translate[0]), abs(d.translate[1])) > eps))
-
-    @staticmethod
-    def check_and_decomp(M):
-        if M is None:
-            decomp = TransformDecomposition(theta=0., scale=np.array([1., 1.]), shear=0., translate=np.array([0., 0.]))
-        else:
-            decomp = TransformTools.decompose(M)
-        return decomp, TransformTools.analyze_transform(decomp)
-
-    @staticmethod
-    def tf_to_string(M):
-        return "matrix({} {} {} {} {} {})".format(M[0, 0], M[1, 0], M[0, 1], M[1, 1], M[0, 2], M[1, 2])
-
-    @staticmethod
-    def decomp_to_string(decomp):
-        """SVG transform list of the non-identity parts, in composition order."""
-        d = TransformTools.make_named(decomp)
-        props = TransformTools.analyze_transform(d)
-        out = ""
-        if props.has_rotation:
-            out += "rotate({}) ".format(math.degrees(float(d.theta)))
-        if props.has_scale:
-            if d.scale.dim() == 0:
-                out += "scale({}) ".format(d.scale.item())
-            else:
-                out += "scale({} {}) ".format(float(d.scale[0]), float(d.scale[1]))
-        if props.has_shear:
-            out += "skewX({}) ".format(float(d.shear))
-        if props.has_translation:
-            out += "translate({} {}) ".format(float(d.translate[0]), float(d.translate[1]))
-        return out
 
     @staticmethod
     def decompose(M):
-        """M (3x3, affine) -> (theta, [sx, sy], shear, [tx, ty]) with
-        M = rot(theta) diag(sx, sy) shearX(shear) translate(tx, ty) and sx > 0.
+        """3x3 affine matrix -> TransformDecomposition(theta, [sx, sy], k, [tx, ty]).
 
-        With A = M[0:2, 0:2]: the first column of A is rot(theta) (sx, 0), so
-        sx = |A[:, 0]| and theta = atan2(A10, A00); rot(-theta) A is upper triangular
-        [[sx, sx shear], [0, sy]], so sy = det(A) / sx and shear = (rot(-theta) A)01 / sx;
-        the translation is A^-1 M[0:2, 2] (it is applied first).  This is the QR
-        factorisation with signs normalised so that sx > 0, which is what
-        optimize_svg.py:287-326 computes through numpy's QR."""
-        A = np.asarray(M, dtype=np.float64)[0:2, 0:2]
-        t = np.asarray(M, dtype=np.float64)[0:2, 2]
+        The first column of the linear part A is R(theta) (sx, 0): sx is its length and
+        theta its angle.  R(-theta) A is then upper triangular [[sx, sx k], [0, sy]], so
+        sy = det(A) / sx and k = (R(-theta) A)[0, 1] / sx.  The translation is applied
+        first, so t = A^-1 M[0:2, 2]."""
+        M = np.asarray(M, dtype=np.float64)
+        A, off = M[0:2, 0:2], M[0:2, 2]
         sx = math.hypot(A[0, 0], A[1, 0])
         theta = math.atan2(A[1, 0], A[0, 0])
         c, s = math.cos(theta), math.sin(theta)
-        sy = (A[0, 0] * A[1, 1] - A[0, 1] * A[1, 0]) / sx
-        shear = (c * A[0, 1] + s * A[1, 1]) / sx
-        return TransformDecomposition(theta=theta, scale=np.array([sx, sy]), shear=shear,
-                                      translate=np.linalg.solve(A, t))
+        sy = float(np.linalg.det(A)) / sx
+        k = (c * A[0, 1] + s * A[1, 1]) / sx
+        return TransformDecomposition(theta, np.array([sx, sy]), k, np.linalg.solve(A, off))
+
+    @staticmethod
+    def _linear(theta, scale, shear, xp):
+        c, s = xp.cos(theta), xp.sin(theta)
+        sx, sy = (scale[0], scale[1]) if getattr(scale, "ndim", 0) and len(scale) == 2 else (scale, scale)
+        # R(theta) @ diag(sx, sy) @ [[1, k], [0, 1]]
+        return ((c * sx, c * sx * shear - s * sy), (s * sx, s * sx * shear + c * sy))
+
+    @staticmethod
+    def recompose(Theta, ScaleXY, ShearX, TXY):
+        """Differentiable inverse of `decompose` on torch tensors (ScaleXY may be a
+        scalar for a uniform scale)."""
+        theta, shear = Theta.reshape(()), ShearX.reshape(())
+        scale = ScaleXY.reshape(-1)
+        scale = scale if scale.numel() == 2 else scale.reshape(())
+        (a, c), (b, d) = TransformTools._linear(theta, scale, shear, torch)
+        t = TXY.reshape(2)
+        one, zero = torch.ones_like(theta), torch.zeros_like(theta)
+        rows = [torch.stack([a, c, a * t[0] + c * t[1]]), torch.stack([b, d, b * t[0] + d * t[1]]),
+                torch.stack([zero, zero, one])]
+        return torch.stack(rows)
+
+    @staticmethod
+    def recompose_numpy(Theta, ScaleXY, ShearX, TXY):
+        scale = np.asarray(ScaleXY, dtype=np.float64).reshape(-1)
+        (a, c), (b, d) = TransformTools._linear(np.float64(Theta), scale if scale.size == 2 else scale[0],
+                                                np.float64(ShearX), np)
+        tx, ty = np.asarray(TXY, dtype=np.float64).reshape(2)
+        return _affine(a, b, c, d, a * tx + c * ty, b * tx + d * ty)
 
 
-# ---------------------------------------------------------------------------------------
-# per-parameter optimizers (optimize_svg.py:328-505)
-# ---------------------------------------------------------------------------------------
-def _optim_class(params):
-    return SvgOptimizationSettings.optims[params["optimizer"]]
+def transform_features(d, eps=1e-3):
+    """Which parts of a decomposition differ from the identity."""
+    sx, sy = abs(float(d.scale[0])), abs(float(d.scale[1]))
+    return dict(rotation=abs(d.theta) > eps, scale=max(abs(sx - 1), abs(sy - 1)) > eps,
+                mirror=float(d.scale[0]) * float(d.scale[1]) < 0, uniform=abs(sx - sy) < eps,
+                shear=abs(d.shear) > eps, translation=float(np.abs(d.translate).max()) > eps)
 
 
-class ColorOptimizer:
-    """An optimizer over a tensor kept within [1e-4, 1] per entry."""
+def _matrix_svg(M):
+    M = np.asarray(M, dtype=np.float64)
+    return "matrix({})".format(" ".join(_fmt(v) for v in (M[0, 0], M[1, 0], M[0, 1], M[1, 1], M[0, 2], M[1, 2])))
 
-    def __init__(self, tensor, optim_type, lr):
-        self.tensor = tensor
-        self.optim = optim_type([tensor], lr=lr)
+
+# ------------------------------------------------------------------- trainable tensors
+def _project_colour(t):
+    t.clamp_(1e-4, 1.0)
+
+
+def _project_stops(t):
+    t.copy_(torch.sort(t.clamp(0.0, 1.0)).values)
+    t[0], t[-1] = 0.0, 1.0
+
+
+class ParameterBank:
+    """Every trainable tensor of a document with its learning rate, optimizer family and
+    post-step projection; one torch optimizer per family, a parameter group per tensor."""
+
+    def __init__(self):
+        self._entries = []
+        self._optimizers = None
+
+    def add(self, tensor, lr, family, project=None):
+        tensor.requires_grad_(True)
+        self._entries.append((tensor, float(lr), family, project))
+        self._optimizers = None
+        return tensor
+
+    def tensors(self):
+        return [e[0] for e in self._entries]
+
+    def learning_rates(self, tensor):
+        return [lr for t, lr, _, _ in self._entries if t is tensor]
+
+    def _build(self):
+        groups = {}
+        for t, lr, family, _ in self._entries:
+            groups.setdefault(family, []).append({"params": [t], "lr": lr})
+        self._optimizers = [SvgOptimizationSettings.optims[f](g) for f, g in groups.items()]
 
     def zero_grad(self):
-        self.optim.zero_grad()
+        for t in self.tensors():
+            t.grad = None
 
     def step(self):
-        self.optim.step()
-        self.tensor.data.clamp_(min=1e-4, max=1.)
+        if self._optimizers is None:
+            self._build()
+        for opt in self._optimizers:
+            opt.step()
+        with torch.no_grad():
+            for t, _, _, project in self._entries:
+                if project is not None:
+                    project(t)
 
 
-class StopOptimizer:
-    """Gradient stop offsets: clamped to [0, 1], kept sorted, ends pinned to 0 and 1."""
-
-    def __init__(self, stops, optim_type, lr):
-        self.stops = stops
-        self.optim = optim_type([stops], lr=lr)
-
-    def zero_grad(self):
-        self.optim.zero_grad()
-
-    def step(self):
-        self.optim.step()
-        self.stops.data.clamp_(min=0., max=1.)
-        self.stops.data, _ = self.stops.data.sort()
-        self.stops.data[0] = 0.
-        self.stops.data[-1] = 1.
-
-
-class GradientOptimizer:
-    """A linear gradient's end points, stop offsets, stop colours and stop alphas, each
-    optimised when its setting says so."""
-
-    def __init__(self, begin, end, offsets, stops, optim_params):
-        def own(t):
-            return t.clone().detach() if t is not None else None
-        self.begin, self.end, self.offsets = own(begin), own(end), own(offsets)
-        self.stop_colors = own(stops[:, 0:3]) if stops is not None else None
-        self.stop_alphas = own(stops[:, 3]) if stops is not None else None
-        self.optimizers = []
-        g = optim_params["gradients"]
-        opt = _optim_class(optim_params)
-        if g["optimize_stops"] and self.offsets is not None:
-            self.offsets.requires_grad_(True)
-            self.optimizers.append(StopOptimizer(self.offsets, opt, g["stop_lr"]))
-        if g["optimize_color"] and self.stop_colors is not None:
-            self.stop_colors.requires_grad_(True)
-            self.optimizers.append(ColorOptimizer(self.stop_colors, opt, g["color_lr"]))
-        if g["optimize_alpha"] and self.stop_alphas is not None:
-            self.stop_alphas.requires_grad_(True)
-            self.optimizers.append(ColorOptimizer(self.stop_alphas, opt, g["alpha_lr"]))
-        if g["optimize_location"] and self.begin is not None and self.end is not None:
-            self.begin.requires_grad_(True)
-            self.end.requires_grad_(True)
-            self.optimizers.append(opt([self.begin, self.end], lr=g["location_lr"]))
-
-    def get_vals(self):
-        stops = None
-        if self.stop_colors is not None and self.stop_alphas is not None:
-            stops = torch.cat((self.stop_colors, self.stop_alphas.unsqueeze(1)), 1)
-        return self.begin, self.end, self.offsets, stops
-
-    def zero_grad(self):
-        for o in self.optimizers:
-            o.zero_grad()
-
-    def step(self):
-        for o in self.optimizers:
-            o.step()
-
-
-class TransformOptimizer:
-    """A node's transform as trainable decomposition parameters.
-
-    transform_mode "move" trains rotation and translation, "rigid" adds a uniform scale,
-    "similarity" a uniform scale that may mirror, "affine" everything (2 scales and a
-    shear).  When the parsed transform has a part the mode cannot represent, it is kept
-    as a fixed `residual` and the trained parameters start from identity:
-    transform = residual @ recompose(parameters).  Translations train with the learning
-    rate times "translation_mult" (set to the larger canvas side by the document)."""
+class TrainableTransform:
+    """One element's transform.  Mode "move" trains rotation and translation, "rigid"
+    adds a uniform scale, "similarity" a uniform scale that keeps the parsed mirror, and
+    "affine" both scales and the shear.  A parsed transform with parts the mode cannot
+    represent is kept as a fixed `residual`, and the trained parameters start from the
+    identity: matrix() = residual @ recompose(parameters).  Translations train at the
+    transform learning rate times "translation_mult"."""
 
     MODES = ("move", "rigid", "similarity", "affine")
 
-    def __init__(self, transform, optim_params):
-        self.transform = transform
-        self.optimizes = optim_params["transforms"]["optimize_transforms"] and transform is not None
-        self.params = copy.deepcopy(optim_params)
-        self.transform_mode = optim_params["transforms"]["transform_mode"]
-        if not self.optimizes:
-            return
-        if self.transform_mode not in self.MODES:
-            raise ValueError("Unrecognized transform mode '{}'".format(self.transform_mode))
-        lr = optim_params["transforms"]["transform_lr"]
-        tmult = optim_params["transforms"]["translation_mult"]
-        dev = transform.device
-        decomp, props = TransformTools.check_and_decomp(transform.cpu().numpy())
-
-        def param(v, grad=True):
-            return torch.tensor(v, dtype=torch.float32, requires_grad=grad, device=dev)
-
-        mode = self.transform_mode
-        unsupported = {
-            "move": props.has_scale or props.has_shear or props.has_mirror,
-            "rigid": props.has_shear or props.has_mirror or not props.scale_uniform,
-            "similarity": props.has_shear or not props.scale_uniform,
-            "affine": False,
-        }[mode]
+    def __init__(self, matrix, entry, bank, device):
+        cfg = entry["transforms"]
+        self.mode = cfg["transform_mode"]
+        self.fixed = torch.tensor(matrix, dtype=torch.float32, device=device)
+        self.trained = bool(cfg["optimize_transforms"])
         self.residual = None
-        if unsupported:
-            print("Warning: set to optimize {} transforms only, but the input transform has parts "
-                  "that mode cannot represent; they are kept fixed".format(mode))
-            self.residual = transform.clone().detach().requires_grad_(False)
-            self.Theta, self.translation = param(0.), param([0., 0.])
-            if mode in ("rigid", "similarity"):
-                self.scale = param(1.)
-            if mode == "similarity":
-                self.scale_sign = param(1., grad=False)
-        else:
-            self.Theta, self.translation = param(decomp.theta), param(decomp.translate)
-            if mode in ("rigid", "similarity"):
-                self.scale = param(decomp.scale[0])
-            if mode == "similarity":
-                self.scale_sign = param(float(np.sign(decomp.scale[0] * decomp.scale[1])), grad=False)
-            if mode == "affine":
-                self.scale, self.shear = param(decomp.scale), param(decomp.shear)
-        groups = [{'params': self.Theta, 'lr': lr}]
-        if mode in ("rigid", "similarity"):
-            groups.append({'params': self.scale, 'lr': lr})
-        if mode == "affine":
-            groups += [{'params': self.scale, 'lr': lr}, {'params': self.shear, 'lr': lr}]
-        groups.append({'params': self.translation, 'lr': lr * tmult})
-        self.optimizer = _optim_class(optim_params)(groups)
+        if not self.trained:
+            return
+        if self.mode not in self.MODES:
+            raise ValueError("Unrecognized transform mode '{}'".format(self.mode))
+        d = TransformTools.decompose(matrix)
+        f = transform_features(d)
+        cannot = {"move": f["scale"] or f["shear"] or f["mirror"],
+                  "rigid": f["shear"] or f["mirror"] or not f["uniform"],
+                  "similarity": f["shear"] or not f["uniform"],
+                  "affine": False}[self.mode]
+        if cannot:
+            warnings.warn("transform mode '{}' cannot represent {}; it is kept fixed and trained on top".format(
+                self.mode, _matrix_svg(matrix)))
+            self.residual = self.fixed.clone()
+            d = TransformDecomposition(0.0, np.array([1.0, 1.0]), 0.0, np.zeros(2))
+
+        def leaf(v):
+            return torch.tensor(v, dtype=torch.float32, device=device)
+        lr, family = cfg["transform_lr"], entry["optimizer"]
+        self.theta = bank.add(leaf(d.theta), lr, family)
+        self.scale = self.shear = None
+        self.mirror = 1.0
+        if self.mode in ("rigid", "similarity"):
+            self.scale = bank.add(leaf(d.scale[0]), lr, family)
+            if self.mode == "similarity":
+                self.mirror = float(np.sign(d.scale[0] * d.scale[1]))
+        elif self.mode == "affine":
+            self.scale = bank.add(leaf(d.scale), lr, family)
+            self.shear = bank.add(leaf(d.shear), lr, family)
+        self.translate = bank.add(leaf(d.translate), lr * cfg["translation_mult"], family)
 
     def _parts(self):
-        dev = self.Theta.device
-        mode = self.transform_mode
-        if mode == "move":
-            return self.Theta, torch.tensor([1.], device=dev), torch.tensor(0., device=dev), self.translation
-        if mode == "rigid":
-            return self.Theta, self.scale, torch.tensor(0., device=dev), self.translation
-        if mode == "similarity":
-            # reference: torch.cat of two 0-d tensors, which raises
-            return self.Theta, torch.stack((self.scale, self.scale * self.scale_sign)), \
-                torch.tensor(0., device=dev), self.translation
-        return self.Theta, self.scale, self.shear, self.translation
+        dev = self.theta.device
+        if self.scale is None:
+            scale = torch.ones((), device=dev)
+        elif self.mode == "similarity":
+            scale = torch.stack([self.scale, self.scale * self.mirror])
+        else:
+            scale = self.scale
+        shear = self.shear if self.shear is not None else torch.zeros((), device=dev)
+        return self.theta, scale, shear, self.translate
 
-    def get_transform(self):
-        if not self.optimizes:
-            return self.transform
-        composed = TransformTools.recompose(*self._parts())
-        return self.residual.mm(composed) if self.residual is not None else composed
+    def matrix(self):
+        if not self.trained:
+            return self.fixed
+        m = TransformTools.recompose(*self._parts())
+        return m if self.residual is None else self.residual @ m
 
-    def tfToString(self):
-        if self.transform is None:
-            return None
-        if not self.optimizes:
-            return TransformTools.tf_to_string(self.transform)
-        parts = tuple(p.detach().cpu() for p in self._parts())
-        text = TransformTools.decomp_to_string(parts)
-        if self.transform_mode == "affine":
-            # reference: returns an undefined name here
-            return text
-        return (TransformTools.tf_to_string(self.residual) if self.residual is not None else "") + " " + text
-
-    def zero_grad(self):
-        if self.optimizes:
-            self.optimizer.zero_grad()
-
-    def step(self):
-        if self.optimizes:
-            self.optimizer.step()
+    def svg(self):
+        """The transform as SVG text: the fixed part as a matrix, then the trained
+        parts that differ from the identity, in composition order."""
+        if not self.trained:
+            return _matrix_svg(self.fixed.detach().cpu().numpy())
+        theta, scale, shear, t = (p.detach().cpu().double().numpy() for p in self._parts())
+        scale = np.broadcast_to(scale, (2,))
+        f = transform_features(TransformDecomposition(float(theta), scale, float(shear), t))
+        items = [] if self.residual is None else [_matrix_svg(self.residual.detach().cpu().numpy())]
+        if f["rotation"]:
+            items.append("rotate({})".format(_fmt(math.degrees(float(theta)))))
+        if f["scale"] or f["mirror"]:
+            items.append("scale({} {})".format(_fmt(scale[0]), _fmt(scale[1])))
+        if f["shear"]:
+            items.append("skewX({})".format(_fmt(shear)))
+        if f["translation"]:
+            items.append("translate({} {})".format(_fmt(t[0]), _fmt(t[1])))
+        return " ".join(items)
 
 
-# ---------------------------------------------------------------------------------------
-# document tree (optimize_svg.py:507-972)
-# ---------------------------------------------------------------------------------------
-def _num(v):
-    """A tensor or number as the text of a number."""
-    return str(v.item() if torch.is_tensor(v) else v)
+# --------------------------------------------------------------------- document nodes
+Paint = namedtuple("Paint", "kind rgb ref")       # kind: "none" | "solid" | "url"
+NO_PAINT = Paint("none", None, None)
+APPEARANCE_KEYS = ("fill", "fill-opacity", "fill-rule", "opacity", "stroke", "stroke-opacity", "stroke-width")
+OPACITY_KEYS = ("opacity", "fill-opacity", "stroke-opacity")
+SHAPE_TAGS = ("path", "circle", "rect", "ellipse", "polygon")
+# CSS units in pixels at 1 px = 0.25 mm (the reference's unit table: document sizes and
+# stroke widths depend on it)
+UNITS = {"px": 1.0, "mm": 4.0, "cm": 40.0, "in": 101.6, "pt": 101.6 / 72, "pc": 101.6 / 6}
 
 
 class SvgNode:
-    """A node with an optional transform, its own appearance entries and optimizers."""
+    """One element: `kind` is its tag ("svg", "g", a shape tag or "linearGradient"),
+    `transform` a TrainableTransform or None, `style` its own appearance entries and
+    `geometry` what its kind needs (paths, a parameter tensor, or gradient data)."""
 
-    def __init__(self, id, transform, appearance, settings):
-        self.id = id
+    def __init__(self, kind, node_id, transform, style, geometry=None):
+        self.kind, self.id = kind, node_id
+        self.transform, self.style = transform, style
+        self.geometry = geometry if geometry is not None else {}
         self.children = []
-        self.optimizers = []
-        self.device = settings.device
-        self.transform = torch.tensor(transform, dtype=torch.float32, device=self.device) \
-            if transform is not None else None
-        params = settings.retrieve(self.id)[0]
-        self.transform_optim = TransformOptimizer(self.transform, params)
-        self.optimizers.append(self.transform_optim)
-        self.proc_appearance(appearance, params)
 
-    def tftostring(self):
-        return self.transform_optim.tfToString()
-
-    def appearanceToString(self):
-        out = ""
-        for key, value in self.appearance.items():
-            if key in ("fill", "stroke"):
-                kind = value[0]
-                if kind == "none":
-                    out += "{}:none;".format(key)
-                elif kind == "solid":
-                    out += "{}:{};".format(key, OptimizableSvg.rgb_to_string(value[1]))
-                elif kind == "url":
-                    out += "{}:url(#{});".format(key, value[1].id)
-            elif key in ("opacity", "fill-opacity", "stroke-opacity", "stroke-width"):
-                # reference: writes the tensor's repr, which does not parse back
-                out += "{}:{};".format(key, _num(value))
-            elif key == "fill-rule":
-                out += "{}:{};".format(key, value)
-            else:
-                raise ValueError("Don't know how to write appearance parameter '{}'".format(key))
-        return out
-
-    def write_xml_common_attrib(self, node, tfname="transform"):
-        if self.transform is not None:
-            node.set(tfname, self.tftostring())
-        if len(self.appearance) > 0:
-            node.set('style', self.appearanceToString())
-        if self.id is not None:
-            node.set('id', self.id)
-
-    def proc_appearance(self, appearance, optim_params):
-        """Colours (and opacities when "optimize_alpha" is set) become trainable."""
-        self.appearance = appearance
-        opt = _optim_class(optim_params)
-        for key, value in appearance.items():
-            if key in ("fill", "stroke"):
-                if optim_params["optimize_color"] and value[0] == "solid":
-                    value[1].requires_grad_(True)
-                    self.optimizers.append(ColorOptimizer(value[1], opt, optim_params["color_lr"]))
-            elif key in ("fill-opacity", "stroke-opacity", "opacity"):
-                if optim_params["optimize_alpha"]:
-                    # reference: indexes the 0-d opacity and passes the optimizer's name,
-                    # which raises
-                    value.requires_grad_(True)
-                    self.optimizers.append(ColorOptimizer(value, opt, optim_params["alpha_lr"]))
-            elif key not in ("fill-rule", "stroke-width"):
-                raise RuntimeError("Unrecognized appearance key '{}'".format(key))
-
-    def prop_transform(self, intform):
-        return intform.matmul(self.transform_optim.get_transform()) if self.transform is not None else intform
-
-    def prop_appearance(self, inappearance):
-        """Paints, fill rule and stroke width replace the inherited ones; opacities
-        multiply them."""
-        out = copy.copy(inappearance)
-        for key, value in self.appearance.items():
-            if key in ("fill-opacity", "opacity", "stroke-opacity"):
-                out[key] = out[key] * value
-            elif key in ("fill", "fill-rule", "stroke", "stroke-width"):
-                out[key] = value
-            else:
-                raise RuntimeError("Unrecognized appearance key '{}'".format(key))
-        return out
-
-    def zero_grad(self):
-        for o in self.optimizers:
-            o.zero_grad()
-        for c in self.children:
-            c.zero_grad()
-
-    def step(self):
-        for o in self.optimizers:
-            o.step()
-        for c in self.children:
-            c.step()
-
-    def get_type(self):
-        return "Generic node"
-
-    def is_shape(self):
-        return False
-
-    def build_scene(self, shapes, shape_groups, transform, appearance):
-        raise NotImplementedError("Abstract SvgNode cannot recurse")
-
-
-class GroupNode(SvgNode):
-    def get_type(self):
-        return "Group node"
-
-    def build_scene(self, shapes, shape_groups, transform, appearance):
-        tf = self.prop_transform(transform)
-        app = self.prop_appearance(appearance)
-        for c in self.children:
-            c.build_scene(shapes, shape_groups, tf, app)
-
-    def write_xml(self, parent):
-        elm = etree.SubElement(parent, "g")
-        self.write_xml_common_attrib(elm)
-        for c in self.children:
-            c.write_xml(elm)
-
-
-class RootNode(SvgNode):
-    def write_xml(self, document):
-        elm = etree.Element('svg')
-        self.write_xml_common_attrib(elm)
-        elm.set("version", "2.0")
-        elm.set("width", str(document.canvas[0]))
-        elm.set("height", str(document.canvas[1]))
-        elm.set("xmlns", "http://www.w3.org/2000/svg")
-        elm.set("xmlns:xlink", "http://www.w3.org/1999/xlink")
-        document.write_defs(elm)
-        for c in self.children:
-            c.write_xml(elm)
-        return elm
-
-    def get_type(self):
-        return "Root node"
-
-    def build_scene(self, shapes, shape_groups, transform, appearance):
-        # the root's own appearance is not passed on (as the reference does)
-        tf = self.prop_transform(transform).to(self.device)
-        for c in self.children:
-            c.build_scene(shapes, shape_groups, tf, appearance)
-
-    @staticmethod
-    def get_default_appearance(device):
-        return {"fill": ("solid", torch.tensor([0., 0., 0.], device=device)),
-                "fill-opacity": torch.tensor([1.], device=device),
-                "fill-rule": "nonzero",
-                "opacity": torch.tensor([1.], device=device),
-                "stroke": ("none", None),
-                "stroke-opacity": torch.tensor([1.], device=device),
-                "stroke-width": torch.tensor([0.], device=device)}
-
-    @staticmethod
-    def get_default_transform():
-        return torch.eye(3)
-
-
-class ShapeNode(SvgNode):
-    def get_type(self):
-        return "Generic shape node"
-
-    def is_shape(self):
-        return True
-
-    def construct_paint(self, value, combined_opacity, transform):
-        kind = value[0]
-        if kind == "none":
-            return None
-        if kind == "solid":
-            return torch.cat([value[1], combined_opacity]).to(self.device)
-        if kind == "url":
-            return value[1].getGrad(combined_opacity, transform)
-        raise ValueError("Unknown paint value type '{}'".format(kind))
-
-    def make_shape_group(self, appearance, transform, num_shapes, num_subobjects):
-        fill = self.construct_paint(appearance["fill"], appearance["opacity"] * appearance["fill-opacity"], transform)
-        stroke = self.construct_paint(appearance["stroke"], appearance["opacity"] * appearance["stroke-opacity"],
-                                      transform)
-        return ShapeGroup(shape_ids=torch.tensor(range(num_shapes, num_shapes + num_subobjects)),
-                          fill_color=fill,
-                          use_even_odd_rule=appearance["fill-rule"] == "evenodd",
-                          stroke_color=stroke,
-                          shape_to_canvas=transform,
-                          id=self.id)
-
-    def _emit(self, shapes, shape_groups, transform, appearance, make):
-        tf = self.prop_transform(transform)
-        app = self.prop_appearance(appearance)
-        built = make(app["stroke-width"])
-        shape_groups.append(self.make_shape_group(app, tf, len(shapes), len(built)))
-        shapes.extend(built)
-
-    def _shape_optimizer(self, settings, tensor):
-        """Shapes other than paths get an optimizer under the "paths" settings, but their
-        tensor never requires grad (as in the reference), so they stay where they are."""
-        params = settings.retrieve(self.id)[0]
-        if params["paths"]["optimize_points"]:
-            self.optimizers.append(_optim_class(params)([tensor], lr=params["paths"]["shape_lr"]))
-
-
-class PathNode(ShapeNode):
-    def __init__(self, id, transform, appearance, settings, paths):
-        super().__init__(id, transform, appearance, settings)
-        self.proc_paths(paths, settings.retrieve(self.id)[0])
-
-    def proc_paths(self, paths, optim_params):
-        self.paths = paths
-        if optim_params["paths"]["optimize_points"]:
-            pts = [p.points.requires_grad_(True) for p in paths]
-            self.optimizers.append(_optim_class(optim_params)(pts, lr=optim_params["paths"]["shape_lr"]))
-
-    def get_type(self):
-        return "Path node"
-
-    def build_scene(self, shapes, shape_groups, transform, appearance):
-        self._emit(shapes, shape_groups, transform, appearance,
-                   lambda width: [Path(p.num_control_points, p.points, p.is_closed, width, p.id) for p in self.paths])
-
-    def path_to_string(self, path):
-        pts = path.points.detach()
-        n = pts.shape[0]
-
-        def xy(i):
-            return "{},{} ".format(pts[i % n][0].item(), pts[i % n][1].item())
-        out = "M " + xy(0)
-        idx = 1
-        for k in path.num_control_points.tolist():
-            out += ("L ", "Q ", "C ")[k]
-            for _ in range(k + 1):
-                out += xy(idx)
-                idx += 1
-        if path.is_closed:
-            out += "Z "
-        return out
-
-    def paths_string(self):
-        return "".join(self.path_to_string(p) for p in self.paths)
-
-    def write_xml(self, parent):
-        elm = etree.SubElement(parent, "path")
-        self.write_xml_common_attrib(elm)
-        elm.set("d", self.paths_string())
-        for c in self.children:
-            c.write_xml(elm)
-
-
-class RectNode(ShapeNode):
-    def __init__(self, id, transform, appearance, settings, rect):
-        super().__init__(id, transform, appearance, settings)
-        self.rect = torch.tensor(rect, dtype=torch.float, device=settings.device)
-        self._shape_optimizer(settings, self.rect)
-
-    def get_type(self):
-        return "Rect node"
-
-    def build_scene(self, shapes, shape_groups, transform, appearance):
-        r = self.rect
-        self._emit(shapes, shape_groups, transform, appearance,
-                   lambda width: [Rect(r[0:2], r[0:2] + r[2:4], width, self.id)])
-
-    def write_xml(self, parent):
-        # reference: writes the tensors' reprs for the four numbers
-        elm = etree.SubElement(parent, "rect")
-        self.write_xml_common_attrib(elm)
-        for name, v in zip(("x", "y", "width", "height"), self.rect):
-            elm.set(name, _num(v))
-        for c in self.children:
-            c.write_xml(elm)
-
-
-class CircleNode(ShapeNode):
-    def __init__(self, id, transform, appearance, settings, rect):
-        super().__init__(id, transform, appearance, settings)
-        self.circle = torch.tensor(rect, dtype=torch.float, device=settings.device)
-        self._shape_optimizer(settings, self.circle)
-
-    def get_type(self):
-        return "Circle node"
-
-    def build_scene(self, shapes, shape_groups, transform, appearance):
-        c = self.circle
-        self._emit(shapes, shape_groups, transform, appearance,
-                   lambda width: [Circle(c[2], c[0:2], width, self.id)])
-
-    def write_xml(self, parent):
-        elm = etree.SubElement(parent, "circle")
-        self.write_xml_common_attrib(elm)
-        for name, v in zip(("cx", "cy", "r"), self.circle):
-            elm.set(name, _num(v))
-        for c in self.children:
-            c.write_xml(elm)
-
-
-class EllipseNode(ShapeNode):
-    def __init__(self, id, transform, appearance, settings, ellipse):
-        super().__init__(id, transform, appearance, settings)
-        self.ellipse = torch.tensor(ellipse, dtype=torch.float, device=settings.device)
-        self._shape_optimizer(settings, self.ellipse)
-
-    def get_type(self):
-        return "Ellipse node"
-
-    def build_scene(self, shapes, shape_groups, transform, appearance):
-        e = self.ellipse
-        self._emit(shapes, shape_groups, transform, appearance,
-                   lambda width: [Ellipse(e[2:4], e[0:2], width, self.id)])
-
-    def write_xml(self, parent):
-        elm = etree.SubElement(parent, "ellipse")
-        self.write_xml_common_attrib(elm)
-        for name, v in zip(("cx", "cy", "rx", "ry"), self.ellipse):
-            elm.set(name, _num(v))
-        for c in self.children:
-            c.write_xml(elm)
-
-
-class PolygonNode(ShapeNode):
-    def __init__(self, id, transform, appearance, settings, points):
-        super().__init__(id, transform, appearance, settings)
-        self.points = points
-        self._shape_optimizer(settings, self.points)
-
-    def get_type(self):
-        return "Polygon node"
-
-    def build_scene(self, shapes, shape_groups, transform, appearance):
-        self._emit(shapes, shape_groups, transform, appearance,
-                   lambda width: [Polygon(self.points, True, width, self.id)])
-
-    def point_string(self):
-        return "".join("{},{} ".format(_num(p[0]), _num(p[1])) for p in self.points.detach())
-
-    def write_xml(self, parent):
-        elm = etree.SubElement(parent, "polygon")
-        self.write_xml_common_attrib(elm)
-        elm.set("points", self.point_string())
-        for c in self.children:
-            c.write_xml(elm)
-
-
-class GradientNode(SvgNode):
-    """A linearGradient from <defs>; `href` is the gradient whose stops it uses."""
-
-    def __init__(self, id, transform, settings, begin, end, offsets, stops, href):
-        super().__init__(id, transform, {}, settings)
-        self.optim = GradientOptimizer(begin, end, offsets, stops, settings.retrieve(id)[0])
-        self.optimizers.append(self.optim)
-        self.href = href
-
-    def is_ref(self):
-        return self.href is not None
-
-    def get_type(self):
-        return "Gradient node"
-
-    def get_stops(self):
-        _, _, offsets, stops = self.optim.get_vals()
-        return offsets, stops
-
-    def get_points(self):
-        begin, end, _, _ = self.optim.get_vals()
-        return begin, end
-
-    def write_xml(self, parent):
-        elm = etree.SubElement(parent, "linearGradient")
-        self.write_xml_common_attrib(elm, tfname="gradientTransform")
-        begin, end, offsets, stops = self.optim.get_vals()
-        if self.href is None:
-            for k, off in enumerate(offsets):
-                stop = etree.SubElement(elm, "stop")
-                stop.set("offset", _num(off))
-                stop.set("stop-color", OptimizableSvg.rgb_to_string(stops[k, 0:3]))
-                stop.set("stop-opacity", _num(stops[k, 3]))
-        else:
-            elm.set('xlink:href', "#{}".format(self.href.id))
-        if begin is not None and end is not None:
-            elm.set('x1', _num(begin[0]))
-            elm.set('y1', _num(begin[1]))
-            elm.set('x2', _num(end[0]))
-            elm.set('y2', _num(end[1]))
-            elm.set("gradientUnits", "userSpaceOnUse")
-        for c in self.children:
-            c.write_xml(elm)
-
-    def getGrad(self, combined_opacity, transform):
-        """-> LinearGradient in canvas space: end points through the shape's transform
-        then this gradient's own; stop alphas times the shape's combined opacity."""
-        offsets, stops = self.href.get_stops() if self.is_ref() else self.get_stops()
-        stops = stops.clone()
-        stops[:, 3] *= combined_opacity
-        begin, end = self.get_points()
-        if begin is None or end is None:
+    def local(self, ctm):
+        return ctm if self.transform is None else ctm @ self.transform.matrix()
+
+    # gradient nodes ---------------------------------------------------------------
+    def stop_data(self):
+        g = self.geometry
+        if g.get("href") is not None:
+            return g["href"].stop_data()
+        return g["offsets"], torch.cat([g["stop_rgb"], g["stop_alpha"].unsqueeze(1)], 1)
+
+    def linear_gradient(self, opacity, ctm):
+        """The gradient in canvas space for a shape drawn under `ctm`, with the stop
+        alphas scaled by the shape's combined opacity."""
+        g = self.geometry
+        if g.get("begin") is None or g.get("end") is None:
             raise ValueError("Gradient '{}' has no x1/y1/x2/y2 of its own".format(self.id))
-        tf = self.prop_transform(transform)
-        begin = TransformTools.transformPoints(begin.unsqueeze(0), tf).squeeze()
-        end = TransformTools.transformPoints(end.unsqueeze(0), tf).squeeze()
-        return LinearGradient(begin, end, offsets, stops)
+        offsets, stops = self.stop_data()
+        stops = torch.cat([stops[:, 0:3], stops[:, 3:4] * opacity.reshape(1, 1)], 1)
+        m = self.local(ctm)
+        ends = torch.stack([g["begin"], g["end"]]) @ m[0:2, 0:2].t() + m[0:2, 2]
+        return LinearGradient(ends[0], ends[1], offsets, stops)
 
 
-def _css_rules(text):
-    """'.name { a: b; c: d }' rules -> [(selector, "a: b;c: d")] (comments removed;
-    declarations joined the way cssutils serialises them once newlines are dropped)."""
-    text = re.sub(r'/\*.*?\*/', '', text or '', flags=re.S)
-    out = []
-    for m in re.finditer(r'([^{}]+)\{([^}]*)\}', text):
-        decls = []
-        for d in m.group(2).split(';'):
-            if ':' in d:
-                k, v = d.split(':', 1)
-                decls.append("{}: {}".format(k.strip().lower(), v.strip()))
-        out.append((' '.join(m.group(1).split()), ';'.join(decls)))
+def _default_style(device):
+    one = torch.ones(1, device=device)
+    return {"fill": Paint("solid", torch.zeros(3, device=device), None), "fill-opacity": one,
+            "fill-rule": "nonzero", "opacity": one, "stroke": NO_PAINT, "stroke-opacity": one,
+            "stroke-width": torch.zeros(1, device=device)}
+
+
+def _cascade(inherited, own):
+    """Paints, fill rule and stroke width replace the inherited values; opacities multiply."""
+    out = dict(inherited)
+    for key, value in own.items():
+        out[key] = out[key] * value if key in OPACITY_KEYS else value
     return out
 
 
-def _split_decls(text, keys=None):
-    """'k:v;k2:v2' -> {k: v}; keys are taken as written (not stripped), the value is
-    what lies between the first and second colon."""
+def _paint_colour(paint, opacity, ctm, device):
+    if paint.kind == "none":
+        return None
+    if paint.kind == "solid":
+        return torch.cat([paint.rgb, opacity.reshape(1)]).to(device)
+    return paint.ref.linear_gradient(opacity, ctm)
+
+
+def _emit_shapes(node, width):
+    """The diffvg shapes of one shape element at stroke width `width`."""
+    g = node.geometry
+    if node.kind == "path":
+        return [Path(p.num_control_points, p.points, p.is_closed, width, p.id) for p in g["paths"]]
+    v = g["values"]
+    if node.kind == "rect":        # x y width height
+        return [Rect(v[0:2], v[0:2] + v[2:4], width, node.id)]
+    if node.kind == "circle":      # cx cy r
+        return [Circle(v[2], v[0:2], width, node.id)]
+    if node.kind == "ellipse":     # cx cy rx ry
+        return [Ellipse(v[2:4], v[0:2], width, node.id)]
+    return [Polygon(v, True, width, node.id)]
+
+
+# ---------------------------------------------------------------------------- parsing
+def parse_length(text):
+    """'12.5mm' -> pixels; a bare number is in pixels."""
+    m = re.fullmatch(r"\s*({})\s*([a-z]*)\s*".format(_NUMBER), text)
+    if m is None:
+        raise ValueError("Cannot parse length '{}'".format(text))
+    unit = m.group(2) or "px"
+    if unit not in UNITS:
+        raise ValueError("Unknown or unsupported unit '{}'".format(unit))
+    return float(m.group(1)) * UNITS[unit]
+
+
+def parse_opacity(text):
+    """'0.5' or '50%' -> value clamped to [0, 1]."""
+    text = text.strip()
+    v = float(text[:-1]) / 100 if text.endswith("%") else float(text)
+    return min(max(v, 0.0), 1.0)
+
+
+def parse_colour(text):
+    """'#rrggbb' or '#rgb' -> RGB tensor in [0, 1] (no other colour syntax)."""
+    h = text.strip()
+    if not re.fullmatch(r"#(?:[0-9a-fA-F]{3}){1,2}", h):
+        raise ValueError("Color argument `{}` not supported".format(text))
+    h = h[1:]
+    if len(h) == 3:
+        return torch.tensor([int(c, 16) / 15.0 for c in h])
+    return torch.tensor([int(h[i:i + 2], 16) / 255.0 for i in (0, 2, 4)])
+
+
+def colour_hex(rgb):
+    """RGB in [0, 1] -> '#rrggbb', truncating each channel to a byte."""
+    b = (rgb.detach().cpu() * 255).to(torch.int).clamp(0, 255).tolist()
+    return "#" + "".join("{:02x}".format(v) for v in b)
+
+
+def _declarations(text, keys=None):
+    """'k:v;k2:v2' -> {k: v}.  Keys are taken as written, not stripped; the value is the
+    text between the first and second colon."""
     out = {}
     for item in text.split(";"):
-        if len(item) == 0:
-            continue
-        parts = item.split(":")
-        if keys is None or parts[0] in keys:
-            out[parts[0]] = parts[1]
+        if item:
+            parts = item.split(":")
+            if keys is None or parts[0] in keys:
+                out[parts[0]] = parts[1]
     return out
 
 
+def _stylesheet(text):
+    """Class rules of a text/css sheet -> {'.name': 'key: value;key: value'}."""
+    text = re.sub(r"/\*.*?\*/", "", text or "", flags=re.S)
+    out = {}
+    for selector, body in re.findall(r"([^{}]+)\{([^}]*)\}", text):
+        selector = " ".join(selector.split())
+        if len(selector) < 2 or selector[0] != ".":
+            raise ValueError("Unrecognized CSS selector {}".format(selector))
+        decls = [d.split(":", 1) for d in body.split(";") if ":" in d]
+        out[selector] = ";".join("{}: {}".format(k.strip().lower(), v.strip()) for k, v in decls)
+    return out
+
+
+def _tag(name):
+    return name.rsplit("}", 1)[-1]
+
+
+class _DocumentReader:
+    """Builds the node tree of one document, registering trainables in `bank`."""
+
+    def __init__(self, settings, bank, device, verbose):
+        self.settings, self.bank, self.device, self.verbose = settings, bank, device, verbose
+        self.gradients = {}    # id -> linearGradient node
+        self.classes = {}      # '.name' -> declarations
+        self.depth = 0
+
+    def log(self, text):
+        if self.verbose:
+            print("\t" * self.depth + text)
+
+    def entry(self, node_id):
+        return self.settings.retrieve(node_id)[0]
+
+    def transform(self, el, node_id, attr="transform"):
+        text = el.attrib.get(attr)
+        if text is None:
+            return None
+        return TrainableTransform(parse_transform_list(text), self.entry(node_id), self.bank, self.device)
+
+    def paint(self, text):
+        text = text.strip()
+        if text == "none":
+            return NO_PAINT
+        if text.startswith("#"):
+            return Paint("solid", parse_colour(text).to(self.device), None)
+        m = re.fullmatch(r"url\(\s*['\"]?#?([^'\")]+)['\"]?\s*\)", text)
+        if m is None:
+            raise ValueError("Unrecognized paint string: '{}'".format(text))
+        if m.group(1) not in self.gradients:
+            raise ValueError("Paint-type attribute referencing an unknown object with ID '#{}'".format(m.group(1)))
+        return Paint("url", None, self.gradients[m.group(1)])
+
+    def appearance(self, el, node_id, trainable=True):
+        raw = {}
+        cls = el.attrib.get("class")
+        if cls is not None and "." + cls in self.classes:
+            raw.update(_declarations(self.classes["." + cls], APPEARANCE_KEYS))
+        raw.update(_declarations(el.attrib.get("style", ""), APPEARANCE_KEYS))
+        raw.update({k: v for k, v in el.attrib.items() if k in APPEARANCE_KEYS})
+        entry = self.entry(node_id)
+        style = {}
+        for key, text in raw.items():
+            if key in ("fill", "stroke"):
+                p = self.paint(text)
+                if trainable and p.kind == "solid" and entry["optimize_color"]:
+                    self.bank.add(p.rgb, entry["color_lr"], entry["optimizer"], _project_colour)
+                style[key] = p
+            elif key in OPACITY_KEYS:
+                v = torch.tensor([parse_opacity(text)], dtype=torch.float32, device=self.device)
+                if trainable and entry["optimize_alpha"]:
+                    self.bank.add(v, entry["alpha_lr"], entry["optimizer"], _project_colour)
+                style[key] = v
+            elif key == "stroke-width":
+                style[key] = torch.tensor([parse_length(text)], dtype=torch.float32, device=self.device)
+            else:
+                style[key] = text.strip()
+        return style
+
+    def skipped(self, el, known, children=True):
+        if not self.verbose:
+            return
+        attrs = sorted(k for k in el.attrib if not k.startswith("{") and k not in known)
+        if attrs:
+            self.log("Warning: skipping attributes of '{}': {}".format(_tag(el.tag), ", ".join(attrs)))
+        if children and len(el):
+            self.log("Warning: skipping children of '{}': {}".format(
+                _tag(el.tag), ", ".join(_tag(c.tag) for c in el)))
+
+    # elements ------------------------------------------------------------------------
+    def document(self, el):
+        canvas = self.canvas(el)
+        self.settings.global_override(["transforms", "translation_mult"], int(max(canvas)))
+        if el.attrib.get("version", "<unknown version>") != "2.0":
+            self.log("Warning: Version {} is not 2.0, strange things may happen".format(
+                el.attrib.get("version", "<unknown version>")))
+        rid = el.attrib.get("id")
+        root = SvgNode("svg", rid, self.transform(el, rid), self.appearance(el, rid, trainable=False))
+        self.skipped(el, ("width", "height", "id", "transform", "version", "style", "viewBox") + APPEARANCE_KEYS,
+                     children=False)
+        self.children(el, root)
+        return root, canvas
+
+    @staticmethod
+    def canvas(el):
+        def px(v):
+            return int(math.ceil(parse_length(v)))
+        if "width" in el.attrib and "height" in el.attrib:
+            return np.array([px(el.attrib["width"]), px(el.attrib["height"])])
+        if "viewBox" in el.attrib:
+            vb = el.attrib["viewBox"].replace(",", " ").split()
+            return np.array([px(vb[2]), px(vb[3])])
+        raise ValueError("Size information is missing from document definition")
+
+    def children(self, el, parent):
+        self.depth += 1
+        for child in el:
+            tag = _tag(child.tag)
+            if tag in SHAPE_TAGS:
+                parent.children.append(self.shape(child, tag))
+            elif tag == "g":
+                parent.children.append(self.group(child))
+            elif tag == "defs":
+                self.defs(child)
+            elif tag == "style":
+                self.stylesheet(child)
+            else:
+                self.log("Warning: skipping element '{}'".format(tag))
+        self.depth -= 1
+
+    def group(self, el):
+        gid = el.attrib.get("id", "<No ID>")
+        self.log("Parsing g#{}".format(gid))
+        node = SvgNode("g", gid, self.transform(el, gid), self.appearance(el, gid))
+        self.skipped(el, ("id", "transform", "style") + APPEARANCE_KEYS, children=False)
+        self.children(el, node)
+        return node
+
+    def shape(self, el, tag):
+        sid = el.attrib.get("id", "")
+        self.log("Parsing {}#{}".format(tag, sid or "<No ID>"))
+        a = el.attrib
+        if tag == "path":
+            paths = from_svg_path(a["d"])
+            entry = self.entry(sid)
+            for k, p in enumerate(paths):
+                p.points = p.points.to(self.device)
+                p.num_control_points = p.num_control_points.to(self.device)
+                p.id = "{}-{}".format(sid, k) if len(paths) > 1 else sid
+                if entry["paths"]["optimize_points"]:
+                    self.bank.add(p.points, entry["paths"]["shape_lr"], entry["optimizer"])
+            geometry, used = {"paths": paths}, ("d",)
+        elif tag == "polygon":
+            pts = [[float(x) for x in item.split(",")] for item in a["points"].split(" ") if item]
+            if any(len(p) != 2 for p in pts):
+                raise ValueError("Malformed polygon points '{}'".format(a["points"]))
+            geometry, used = {"values": torch.tensor(pts, dtype=torch.float32, device=self.device)}, ("points",)
+        else:
+            used = {"rect": ("x", "y", "width", "height"), "circle": ("cx", "cy", "r"),
+                    "ellipse": ("cx", "cy", "rx", "ry")}[tag]
+            optional = used[:2]    # the position defaults to 0; sizes are required
+            vals = [float(a[n]) if (n in a or n not in optional) else 0.0 for n in used]
+            geometry = {"values": torch.tensor(vals, dtype=torch.float32, device=self.device)}
+        node = SvgNode(tag, sid, self.transform(el, sid), self.appearance(el, sid), geometry)
+        self.skipped(el, ("id", "transform", "style", "class") + used + APPEARANCE_KEYS)
+        return node
+
+    def stylesheet(self, el):
+        if el.attrib.get("type") != "text/css":
+            raise ValueError("Only text/css style recognized, got {}".format(el.attrib.get("type")))
+        if len(el):
+            raise ValueError("Style node should not have children (has {})".format(len(el)))
+        self.classes.update(_stylesheet(el.text))
+
+    def defs(self, el):
+        self.depth += 1
+        for child in el:
+            tag = _tag(child.tag)
+            if tag == "linearGradient":
+                node = self.gradient(child)
+                self.gradients[node.id] = node
+            elif tag in SHAPE_TAGS:
+                raise NotImplementedError("Definition/instantiation of shapes not supported")
+            elif tag in ("defs", "g"):
+                raise NotImplementedError("'{}' within definitions not supported".format(tag))
+            else:
+                self.log("Warning: skipping definition '{}'".format(tag))
+        self.depth -= 1
+
+    def gradient(self, el):
+        gid = el.attrib.get("id", "<No ID>")
+        self.log("Parsing linearGradient#{}".format(gid))
+        a = el.attrib
+        href = next((v for k, v in a.items() if _tag(k) == "href"), None)
+        stops = []
+        for s in el:
+            if _tag(s.tag) != "stop":
+                continue
+            attrs = {k: v for k, v in s.attrib.items() if k in ("offset", "stop-color", "stop-opacity")}
+            attrs.update(_declarations(s.attrib.get("style", "")))
+            stops.append((parse_opacity(attrs["offset"]), parse_colour(attrs["stop-color"]).tolist(),
+                          parse_opacity(attrs["stop-opacity"]) if "stop-opacity" in attrs else 1.0))
+        if not stops and href is None:
+            raise ValueError("Gradient {} has neither stops nor a href link to them".format(gid))
+        entry = self.entry(gid)
+        cfg, family = entry["gradients"], entry["optimizer"]
+
+        def tensor(v):
+            return torch.tensor(v, dtype=torch.float32, device=self.device)
+
+        def point(x, y):
+            return tensor([float(a.get(x, 0.0)), float(a.get(y, 0.0))]) if (x in a or y in a) else None
+        g = {"begin": point("x1", "y1"), "end": point("x2", "y2"),
+             "href": self.gradients[href.lstrip("#")] if href is not None else None}
+        if stops:
+            stops.sort(key=lambda s: s[0])
+            g["offsets"] = tensor([s[0] for s in stops])
+            g["stop_rgb"] = tensor([s[1] for s in stops])
+            g["stop_alpha"] = tensor([s[2] for s in stops])
+            if cfg["optimize_stops"]:
+                self.bank.add(g["offsets"], cfg["stop_lr"], family, _project_stops)
+            if cfg["optimize_color"]:
+                self.bank.add(g["stop_rgb"], cfg["color_lr"], family, _project_colour)
+            if cfg["optimize_alpha"]:
+                self.bank.add(g["stop_alpha"], cfg["alpha_lr"], family, _project_colour)
+        if cfg["optimize_location"] and g["begin"] is not None and g["end"] is not None:
+            self.bank.add(g["begin"], cfg["location_lr"], family)
+            self.bank.add(g["end"], cfg["location_lr"], family)
+        return SvgNode("linearGradient", gid, self.transform(el, gid, "gradientTransform"), {}, g)
+
+
+# ---------------------------------------------------------------------------- writing
+def _style_text(style):
+    items = []
+    for key, value in style.items():
+        if isinstance(value, Paint):
+            text = {"none": lambda p: "none", "solid": lambda p: colour_hex(p.rgb),
+                    "url": lambda p: "url(#{})".format(p.ref.id)}[value.kind](value)
+        elif torch.is_tensor(value):
+            text = _fmt(value.reshape(-1)[0])
+        else:
+            text = value
+        items.append("{}:{}".format(key, text))
+    return ";".join(items) + (";" if items else "")
+
+
+def _path_data(path):
+    """One Path as SVG path data: every segment written out (a closed path's last
+    segment ends on its first point), then Z for a closed path."""
+    pts = path.points.detach().cpu().double().numpy()
+    n = len(pts)
+    words = ["M", _fmt(pts[0, 0]), _fmt(pts[0, 1])]
+    i = 1
+    for k in path.num_control_points.tolist():
+        words.append("LQC"[k])
+        for j in range(i, i + k + 1):
+            words += [_fmt(pts[j % n, 0]), _fmt(pts[j % n, 1])]
+        i += k + 1
+    if path.is_closed:
+        words.append("Z")
+    return " ".join(words)
+
+
+_GEOMETRY_ATTRS = {"rect": ("x", "y", "width", "height"), "circle": ("cx", "cy", "r"),
+                   "ellipse": ("cx", "cy", "rx", "ry")}
+
+
+def _write_node(parent, node):
+    el = ET.SubElement(parent, node.kind)
+    if node.transform is not None:
+        el.set("gradientTransform" if node.kind == "linearGradient" else "transform", node.transform.svg())
+    if node.style:
+        el.set("style", _style_text(node.style))
+    if node.id is not None:
+        el.set("id", node.id)
+    g = node.geometry
+    if node.kind == "path":
+        el.set("d", " ".join(_path_data(p) for p in g["paths"]))
+    elif node.kind == "polygon":
+        el.set("points", " ".join("{},{}".format(_fmt(x), _fmt(y)) for x, y in g["values"].tolist()))
+    elif node.kind in _GEOMETRY_ATTRS:
+        for name, v in zip(_GEOMETRY_ATTRS[node.kind], g["values"].tolist()):
+            el.set(name, _fmt(v))
+    elif node.kind == "linearGradient":
+        if g["href"] is not None:
+            el.set("xlink:href", "#" + g["href"].id)
+        else:
+            for off, rgb, alpha in zip(g["offsets"].tolist(), g["stop_rgb"].detach(), g["stop_alpha"].tolist()):
+                ET.SubElement(el, "stop", {"offset": _fmt(off), "stop-color": colour_hex(rgb),
+                                           "stop-opacity": _fmt(alpha)})
+        if g["begin"] is not None and g["end"] is not None:
+            for name, v in zip(("x1", "y1", "x2", "y2"), torch.cat([g["begin"], g["end"]]).tolist()):
+                el.set(name, _fmt(v))
+            el.set("gradientUnits", "userSpaceOnUse")
+    for c in node.children:
+        _write_node(el, c)
+
+
+# --------------------------------------------------------------------------- document
 class OptimizableSvg:
     """An SVG document whose parameters train through the differentiable renderer
-    (optimize_svg.py:106-1607)."""
+    (optimize_svg.py:974-1046 for the interface)."""
 
     TransformTools = TransformTools
-    ColorOptimizer = ColorOptimizer
-    StopOptimizer = StopOptimizer
-    GradientOptimizer = GradientOptimizer
-    TransformOptimizer = TransformOptimizer
-    SvgNode = SvgNode
-    GroupNode = GroupNode
-    RootNode = RootNode
-    ShapeNode = ShapeNode
-    PathNode = PathNode
-    RectNode = RectNode
-    CircleNode = CircleNode
-    EllipseNode = EllipseNode
-    PolygonNode = PolygonNode
-    GradientNode = GradientNode
-
-    recognised_shapes = ["path", "circle", "rect", "ellipse", "polygon"]
-    appearance_keys = ["fill", "fill-opacity", "fill-rule", "opacity", "stroke", "stroke-opacity", "stroke-width"]
-    # CSS units in pixels, with 1 px = 0.25 mm
-    unit_dict = {"px": 1, "mm": 4, "cm": 40, "in": 25.4 * 4, "pt": 25.4 * 4 / 72, "pc": 25.4 * 4 / 6}
 
     def __init__(self, filename, settings=None, optimize_background=False, verbose=False,
                  device=torch.device("cpu")):
         self.settings = settings if settings is not None else SvgOptimizationSettings()
-        self.verbose = verbose
         self.device = device
-        self.settings.device = device
-        root = etree.parse(filename).getroot()
-        self.optimizers = []
-        # kept for interface parity: `render` composites over nothing, like the reference
-        self.background = torch.tensor([1., 1., 1.], dtype=torch.float32, requires_grad=optimize_background,
-                                       device=self.device)
+        self.bank = ParameterBank()
+        # kept for the interface: `render` composites over nothing
+        self.background = torch.tensor([1.0, 1.0, 1.0], device=device)
         if optimize_background:
             p = self.settings.retrieve("default")[0]
-            self.optimizers.append(ColorOptimizer(self.background, _optim_class(p), p["color_lr"]))
-        self.defs = {}
-        self.depth = 0
-        self.dirty = True
-        self.scene = None
-        self.parseRoot(root)
+            self.bank.add(self.background, p["color_lr"], p["optimizer"], _project_colour)
+        reader = _DocumentReader(self.settings, self.bank, device, verbose)
+        self.root, self.canvas = reader.document(ET.parse(filename).getroot())
+        self.gradients, self.classes = reader.gradients, reader.classes
+        self._scene = None
 
-    # ---- core ---------------------------------------------------------------------------
-    def build_scene(self):
-        """-> (width, height, shapes, shape_groups); rebuilt after each `step`."""
-        if self.dirty:
-            shapes, groups = [], []
-            self.root.build_scene(shapes, groups, RootNode.get_default_transform().to(self.device),
-                                  RootNode.get_default_appearance(self.device))
-            self.scene = (self.canvas[0], self.canvas[1], shapes, groups)
-            self.dirty = False
-        return self.scene
-
-    def _def_nodes(self):
-        return [v for v in self.defs.values() if isinstance(v, SvgNode)]
-
-    def zero_grad(self):
-        self.root.zero_grad()
-        for o in self.optimizers:
-            o.zero_grad()
-        for n in self._def_nodes():
-            n.zero_grad()
-
-    def render(self, scale=None, seed=0):
-        """The scene at 2x2 samples; `scale` scales the output size only (the scene is
-        not scaled, so the image shows the top-left part of the canvas at 1:1)."""
-        scene = self.build_scene()
-        args = RenderFunction.serialize_scene(*scene)
-        w, h = (scene[0], scene[1]) if scale is None else (int(scene[0] * scale), int(scene[1] * scale))
-        return RenderFunction.apply(w, h, 2, 2, seed, None, *args)
-
-    def step(self):
-        self.dirty = True
-        self.root.step()
-        for o in self.optimizers:
-            o.step()
-        for n in self._def_nodes():
-            n.step()
-
-    # ---- reporting ------------------------------------------------------------------------
-    def offset_str(self, s):
-        return ("\t" * self.depth) + s
-
-    def _label(self, node):
-        tag = OptimizableSvg.remove_namespace(node.tag)
-        return tag if "id" not in node.attrib else "{}#{}".format(tag, node.attrib["id"])
-
-    def reportSkippedAttribs(self, node, non_skipped=[]):
-        skipped = set(k for k in node.attrib if not OptimizableSvg.is_namespace(k)) - set(non_skipped)
-        if skipped:
-            print(self.offset_str("Warning: Skipping the following attributes of node '{}': {}".format(
-                self._label(node), ", ".join("'{}'".format(a) for a in skipped))))
-
-    def reportSkippedChildren(self, node, skipped):
-        if skipped:
-            names = ["{}#{}".format(e.tag, e.attrib["id"]) if "id" in e.attrib else e.tag for e in skipped]
-            print(self.offset_str("Warning: Skipping the following children of node '{}': {}".format(
-                self._label(node), ", ".join("'{}'".format(n) for n in names))))
-
-    # ---- parsing ---------------------------------------------------------------------------
-    @staticmethod
-    def remove_namespace(s):
-        """'{uri}tag' -> 'tag'."""
-        return re.sub('{.*}', '', s)
-
-    @staticmethod
-    def is_namespace(s):
-        return re.match('{.*}', s) is not None
-
-    @staticmethod
-    def parseTransform(node):
-        """transform (or gradientTransform) attribute -> 3x3 numpy matrix, or None."""
-        text = node.attrib.get("transform", node.attrib.get("gradientTransform"))
-        if text is None:
-            return None
-        parsers = {"matrix": TransformTools.parse_matrix, "translate": TransformTools.parse_translate,
-                   "rotate": TransformTools.parse_rotate, "scale": TransformTools.parse_scale,
-                   "skewX": TransformTools.parse_skewx, "skewY": TransformTools.parse_skewy}
-        mat = np.eye(3)
-        for item in text.split(")")[:-1]:
-            # reference: a space before the function name or inside the parentheses
-            # makes it raise
-            kind, _, arg_text = item.partition("(")
-            kind = kind.strip().lstrip(",").strip()
-            if kind not in parsers:
-                raise ValueError("Unknown transform type '{}'".format(kind))
-            args = [float(v) for v in re.split("[, ]+", arg_text.strip())]
-            mat = mat @ parsers[kind](args)
-        return mat
-
-    @staticmethod
-    def parseLength(s):
-        """'12.5mm' -> pixels (a bare number is in pixels)."""
-        s = s.strip()
-        m = re.fullmatch(r'([-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?)\s*([a-z]*)', s)
-        if m is None:
-            raise ValueError("Cannot parse length '{}'".format(s))
-        val, unit = float(m.group(1)), m.group(2)
-        if unit == "":
-            return val
-        if unit not in OptimizableSvg.unit_dict:
-            raise ValueError("Unknown or unsupported unit '{}' encountered while parsing".format(unit))
-        return val * OptimizableSvg.unit_dict[unit]
-
-    @staticmethod
-    def parseOpacity(s):
-        s = s.strip()
-        pct = s.endswith("%")
-        val = float(s.rstrip("%"))
-        return np.clip(val / 100 if pct else val, 0., 1.)
-
-    @staticmethod
-    def parse_color(s):
-        """'#rrggbb' or '#rgb' -> RGB tensor in [0, 1]; nothing else is accepted."""
-        s = s.strip()
-        if not s.startswith('#'):
-            raise ValueError("Color argument `{}` not supported".format(s))
-        h = s[1:]
-        if len(h) == 6:
-            return torch.tensor([int(h[i:i + 2], 16) / 255.0 for i in (0, 2, 4)])
-        if len(h) == 3:
-            return torch.tensor([int(h[i], 16) / 15.0 for i in (0, 1, 2)])
-        raise ValueError("Color argument `{}` not supported".format(s))
-
-    @staticmethod
-    def rgb_to_string(val):
-        """RGB in [0, 1] -> '#rrggbb', truncating to bytes."""
-        b = (val.clone().detach() * 255).type(torch.int).clamp_(min=0, max=255).tolist()
-        return "#{:02x}{:02x}{:02x}".format(*b)
-
-    @staticmethod
-    def parsePaint(paintStr, defs, device):
-        """-> ("none", None) | ("solid", rgb) | ("url", gradient node)."""
-        p = paintStr.strip()
-        if p == "none":
-            return ("none", None)
-        if p.startswith("#"):
-            return ("solid", OptimizableSvg.parse_color(p).to(device))
-        if p.startswith("url"):
-            ref = re.sub(r'^url\(|\)$', '', p).strip().strip("'\"").lstrip("#")
-            if ref not in defs:
-                raise ValueError("Paint-type attribute referencing an unknown object with ID '#{}'".format(ref))
-            return ("url", defs[ref])
-        raise ValueError("Unrecognized paint string: '{}'".format(paintStr))
-
-    @staticmethod
-    def parseAppearance(node, defs, device):
-        """Appearance entries of a node from its CSS class, then its `style`, then its
-        presentation attributes (each overriding the one before)."""
-        keys = OptimizableSvg.appearance_keys
-        merged = {}
-        cls = node.attrib.get("class")
-        if cls is not None and "." + cls in defs:
-            merged.update(_split_decls(defs["." + cls], keys))
-        if "style" in node.attrib:
-            merged.update(_split_decls(node.attrib["style"], keys))
-        merged.update({k: v for k, v in node.attrib.items() if k in keys})
-        out = {}
-        for key, value in merged.items():
-            if key in ("fill", "stroke"):
-                out[key] = OptimizableSvg.parsePaint(value, defs, device)
-            elif key in ("fill-opacity", "opacity", "stroke-opacity"):
-                out[key] = torch.tensor(OptimizableSvg.parseOpacity(value), device=device)
-            elif key == "fill-rule":
-                out[key] = value
-            elif key == "stroke-width":
-                out[key] = torch.tensor(OptimizableSvg.parseLength(value), device=device)
+    def nodes(self, kind=None):
+        """The elements of the tree in document order (optionally of one kind)."""
+        out, todo = [], [self.root]
+        while todo:
+            n = todo.pop()
+            if kind is None or n.kind == kind:
+                out.append(n)
+            todo.extend(reversed(n.children))
         return out
 
-    def _parse_children(self, node, parent):
-        skipped = []
-        for child in node:
-            tag = OptimizableSvg.remove_namespace(child.tag)
-            if tag in OptimizableSvg.recognised_shapes:
-                self.parseShape(child, parent)
-            elif tag == "defs":
-                self.parseDefs(child)
-            elif tag == "style":
-                self.parseStyle(child)
-            elif tag == "g":
-                self.parseGroup(child, parent)
-            else:
-                skipped.append(child)
-        if self.verbose:
-            self.reportSkippedChildren(node, skipped)
+    def node(self, node_id):
+        for n in self.nodes() + list(self.gradients.values()):
+            if n.id == node_id:
+                return n
+        raise KeyError(node_id)
 
-    def parseRoot(self, root):
-        if self.verbose:
-            print(self.offset_str("Parsing root"))
-        self.depth += 1
-        self.parseViewport(root)
-        self.settings.global_override(["transforms", "translation_mult"], int(np.max(self.canvas)))
-        rid = root.attrib.get("id")
-        transform = OptimizableSvg.parseTransform(root)
-        appearance = OptimizableSvg.parseAppearance(root, self.defs, self.device)
-        version = root.attrib.get("version", "<unknown version>")
-        if version != "2.0":
-            print(self.offset_str("Warning: Version {} is not 2.0, strange things may happen".format(version)))
-        self.root = RootNode(rid, transform, appearance, self.settings)
-        if self.verbose:
-            self.reportSkippedAttribs(root, ["width", "height", "id", "transform", "version", "style"]
-                                      + OptimizableSvg.appearance_keys)
-        self._parse_children(root, self.root)
-        self.depth -= 1
+    def build_scene(self):
+        """-> (width, height, shapes, shape_groups), rebuilt after each `step`."""
+        if self._scene is None:
+            shapes, groups = [], []
+            ctm = self.root.local(torch.eye(3, device=self.device))
+            style = _default_style(self.device)   # the root's own appearance is not inherited
+            for c in self.root.children:
+                self._emit(c, ctm, style, shapes, groups)
+            self._scene = (self.canvas[0], self.canvas[1], shapes, groups)
+        return self._scene
 
-    def parseShape(self, shape, parent):
-        tag = OptimizableSvg.remove_namespace(shape.tag)
-        if self.verbose:
-            print(self.offset_str("Parsing {}#{}".format(tag, shape.attrib.get("id", "<No ID>"))))
-        self.depth += 1
-        parse = {"path": self.parsePath, "circle": self.parseCircle, "rect": self.parseRect,
-                 "ellipse": self.parseEllipse, "polygon": self.parsePolygon}.get(tag)
-        if parse is None:
-            raise ValueError("Encountered unknown shape type '{}'".format(tag))
-        parse(shape, parent)
-        self.depth -= 1
+    def _emit(self, node, ctm, style, shapes, groups):
+        ctm, style = node.local(ctm), _cascade(style, node.style)
+        if node.kind == "g":
+            for c in node.children:
+                self._emit(c, ctm, style, shapes, groups)
+            return
+        built = _emit_shapes(node, style["stroke-width"])
+        fill = _paint_colour(style["fill"], style["opacity"] * style["fill-opacity"], ctm, self.device)
+        stroke = _paint_colour(style["stroke"], style["opacity"] * style["stroke-opacity"], ctm, self.device)
+        groups.append(ShapeGroup(shape_ids=torch.arange(len(shapes), len(shapes) + len(built)),
+                                 fill_color=fill, use_even_odd_rule=style["fill-rule"] == "evenodd",
+                                 stroke_color=stroke, shape_to_canvas=ctm, id=node.id))
+        shapes.extend(built)
 
-    def _attach(self, shape, parent, node_class, payload, known):
-        name = shape.attrib.get('id', '')
-        transform = OptimizableSvg.parseTransform(shape)
-        appearance = OptimizableSvg.parseAppearance(shape, self.defs, self.device)
-        parent.children.append(node_class(name, transform, appearance, self.settings, payload))
-        if self.verbose:
-            self.reportSkippedAttribs(shape, ["id", "transform", "style"] + known + OptimizableSvg.appearance_keys)
-            self.reportSkippedChildren(shape, list(shape))
+    def render(self, scale=None, seed=0):
+        """The scene at 2x2 samples; `scale` scales the output size only (the scene keeps
+        its canvas, so the image shows its top-left part at 1:1 when scale > 1)."""
+        w, h, shapes, groups = self.build_scene()
+        args = RenderFunction.serialize_scene(w, h, shapes, groups)
+        if scale is not None:
+            w, h = int(w * scale), int(h * scale)
+        return RenderFunction.apply(w, h, 2, 2, seed, None, *args)
 
-    def parsePath(self, shape, parent):
-        name = shape.attrib.get('id', '')
-        paths = from_svg_path(shape.attrib['d'])
-        for k, p in enumerate(paths):
-            p.stroke_width = torch.tensor([0.], device=self.device)
-            p.num_control_points = p.num_control_points.to(self.device)
-            p.points = p.points.to(self.device)
-            p.source_id = name
-            p.id = "{}-{}".format(name, k) if len(paths) > 1 else name
-        self._attach(shape, parent, PathNode, paths, ["d"])
+    def zero_grad(self):
+        self.bank.zero_grad()
 
-    @staticmethod
-    def _floats(shape, names, optional=()):
-        return tuple(float(shape.attrib[n]) if n in shape.attrib or n not in optional else 0.
-                     for n in names)
+    def step(self):
+        self.bank.step()
+        self._scene = None
 
-    def parseEllipse(self, shape, parent):
-        self._attach(shape, parent, EllipseNode, self._floats(shape, ("cx", "cy", "rx", "ry"), ("cx", "cy")),
-                     ["cx", "cy", "rx", "ry"])
-
-    def parseCircle(self, shape, parent):
-        self._attach(shape, parent, CircleNode, self._floats(shape, ("cx", "cy", "r"), ("cx", "cy")),
-                     ["cx", "cy", "r"])
-
-    def parseRect(self, shape, parent):
-        self._attach(shape, parent, RectNode, self._floats(shape, ("x", "y", "width", "height"), ("x", "y")),
-                     ["x", "y", "width", "height"])
-
-    def parsePolygon(self, shape, parent):
-        pts = []
-        for item in shape.attrib['points'].split(" "):
-            if len(item) == 0:
-                continue
-            xy = item.split(",")
-            assert len(xy) == 2
-            pts.append([float(xy[0]), float(xy[1])])
-        self._attach(shape, parent, PolygonNode, torch.tensor(pts, dtype=torch.float, device=self.device),
-                     ["points"])
-
-    def parseGroup(self, group, parent):
-        gid = group.attrib.get("id", "<No ID>")
-        if self.verbose:
-            print(self.offset_str("Parsing {}#{}".format(OptimizableSvg.remove_namespace(group.tag), gid)))
-        self.depth += 1
-        node = GroupNode(gid, self.parseTransform(group), OptimizableSvg.parseAppearance(group, self.defs, self.device),
-                         self.settings)
-        parent.children.append(node)
-        if self.verbose:
-            self.reportSkippedAttribs(group, ["id", "transform", "style"] + OptimizableSvg.appearance_keys)
-        self._parse_children(group, node)
-        self.depth -= 1
-
-    def parseStyle(self, style_node):
-        """A text/css stylesheet of class rules -> defs['.name'] = declarations."""
-        if self.verbose:
-            print(self.offset_str("Parsing {}#{}".format(OptimizableSvg.remove_namespace(style_node.tag),
-                                                         style_node.attrib.get("id", "<No ID>"))))
-        if style_node.attrib.get("type") != "text/css":
-            raise ValueError("Only text/css style recognized, got {}".format(style_node.attrib.get("type")))
-        if len(style_node) > 0:
-            raise ValueError("Style node should not have children (has {})".format(len(style_node)))
-        for selector, decls in _css_rules(style_node.text):
-            if len(selector) >= 2 and selector[0] == '.':
-                self.defs[selector] = decls
-            else:
-                raise ValueError("Unrecognized CSS selector {}".format(selector))
-
-    def parseDefs(self, def_node):
-        """Linear gradients go into defs by id; radial gradients and anything else are
-        skipped; shapes, groups and nested defs are not supported."""
-        if self.verbose:
-            print(self.offset_str("Parsing {}#{}".format(OptimizableSvg.remove_namespace(def_node.tag),
-                                                         def_node.attrib.get("id", "<No ID>"))))
-        self.depth += 1
-        holder = SvgNode(def_node.attrib.get("id", "<No ID>"), None, {}, self.settings)
-        if self.verbose:
-            self.reportSkippedAttribs(def_node, ["id"])
-        skipped = []
-        for child in def_node:
-            tag = OptimizableSvg.remove_namespace(child.tag)
-            if tag == "linearGradient":
-                self.parseGradient(child, holder)
-            elif tag in OptimizableSvg.recognised_shapes:
-                raise NotImplementedError("Definition/instantiation of shapes not supported")
-            elif tag == "defs":
-                raise NotImplementedError("Definition within definition not supported")
-            elif tag == "g":
-                raise NotImplementedError("Groups within definition not supported")
-            else:
-                skipped.append(child)
-            if holder.children:
-                node = holder.children.pop()
-                self.defs[node.id] = node
-        if self.verbose:
-            self.reportSkippedChildren(def_node, skipped)
-        self.depth -= 1
-
-    def parseGradientStop(self, stop):
-        """-> (offset, rgb, opacity); `style` entries override attributes."""
-        params = {k: v for k, v in stop.attrib.items() if k in ("id", "offset", "stop-color", "stop-opacity")}
-        if "style" in stop.attrib:
-            params.update(_split_decls(stop.attrib["style"]))
-        offset = OptimizableSvg.parseOpacity(params["offset"])
-        color = OptimizableSvg.parse_color(params["stop-color"])
-        opacity = OptimizableSvg.parseOpacity(params["stop-opacity"]) if "stop-opacity" in params else 1.
-        return offset, color, opacity
-
-    def parseGradient(self, gradient_node, parent):
-        gid = gradient_node.attrib.get("id", "<No ID>")
-        if self.verbose:
-            print(self.offset_str("Parsing {}#{}".format(OptimizableSvg.remove_namespace(gradient_node.tag), gid)))
-        self.depth += 1
-        children = list(gradient_node)
-        has_stops = any(OptimizableSvg.remove_namespace(c.tag) == "stop" for c in children)
-        href_key = next((v for k, v in gradient_node.attrib.items()
-                         if OptimizableSvg.remove_namespace(k) == "href"), None)
-        if not has_stops and href_key is None:
-            raise ValueError("Gradient {} has neither stops nor a href link to them".format(gid))
-        a = gradient_node.attrib
-
-        def point(xk, yk):
-            if xk not in a and yk not in a:
-                return None
-            return torch.tensor([float(a.get(xk, 0.)), float(a.get(yk, 0.))], dtype=torch.float32,
-                                device=self.device)
-        begin, end = point("x1", "y1"), point("x2", "y2")
-        offsets, stops = None, None
-        stop_nodes = [c for c in children if OptimizableSvg.remove_namespace(c.tag) == "stop"]
-        if stop_nodes:
-            parsed = sorted((self.parseGradientStop(s) for s in stop_nodes), key=lambda t: t[0])
-            offsets = torch.tensor([float(o) for o, _, _ in parsed], dtype=torch.float32, device=self.device)
-            stops = torch.tensor([[*c.tolist(), float(op)] for _, c, op in parsed], dtype=torch.float32,
-                                 device=self.device)
-        href = self.defs[href_key.lstrip("#")] if href_key is not None else None
-        parent.children.append(GradientNode(gid, self.parseTransform(gradient_node), self.settings,
-                                            begin, end, offsets, stops, href))
-        self.depth -= 1
-
-    def parseViewport(self, root):
-        """Canvas size = ceil(width), ceil(height), else the viewBox's size."""
-        def size(v):
-            try:
-                return int(math.ceil(float(v)))
-            except ValueError:
-                return int(math.ceil(OptimizableSvg.parseLength(v)))
-        if "width" in root.attrib and "height" in root.attrib:
-            self.canvas = np.array([size(root.attrib["width"]), size(root.attrib["height"])])
-        elif "viewBox" in root.attrib:
-            vb = root.attrib["viewBox"].replace(",", " ").split()
-            self.canvas = np.array([size(vb[2]), size(vb[3])])
-        else:
-            raise ValueError("Size information is missing from document definition")
-
-    # ---- writing --------------------------------------------------------------------------
     def write_xml(self):
         """The document with the current parameter values, as pretty-printed text."""
-        tree = self.root.write_xml(self)
-        return minidom.parseString(etree.tostring(tree, 'utf-8')).toprettyxml(indent="  ")
+        el = ET.Element("svg")
+        if self.root.transform is not None:
+            el.set("transform", self.root.transform.svg())
+        if self.root.style:
+            el.set("style", _style_text(self.root.style))
+        if self.root.id is not None:
+            el.set("id", self.root.id)
+        for k, v in (("version", "2.0"), ("width", str(self.canvas[0])), ("height", str(self.canvas[1])),
+                     ("xmlns", "http://www.w3.org/2000/svg"), ("xmlns:xlink", "http://www.w3.org/1999/xlink")):
+            el.set(k, v)
+        if self.gradients or self.classes:
+            defs = ET.SubElement(el, "defs")
+            written = set()
 
-    def write_defs(self, root):
-        if len(self.defs) == 0:
-            return
-        defnode = etree.SubElement(root, 'defs')
-        stylenode = etree.SubElement(root, 'style')
-        stylenode.set('type', 'text/css')
-        stylenode.text = ""
-        # gradients that others link to go first
-        pending = dict(self.defs)
-        while pending:
-            done = []
-            for key, value in pending.items():
-                if isinstance(value, SvgNode):
-                    if value.href is not None and any(value.href is v for v in pending.values()):
-                        continue
-                    value.write_xml(defnode)
-                else:
-                    stylenode.text += key + " {" + value + "}\n"
-                done.append(key)
-            for key in done:
-                del pending[key]
+            def put(node):     # a gradient after the one it links to (parsing needs that order)
+                if node.id in written:
+                    return
+                if node.geometry["href"] is not None:
+                    put(node.geometry["href"])
+                _write_node(defs, node)
+                written.add(node.id)
+            for node in self.gradients.values():
+                put(node)
+            sheet = ET.SubElement(el, "style", {"type": "text/css"})
+            sheet.text = "".join("{} {{{}}}\n".format(k, v) for k, v in self.classes.items())
+        for c in self.root.children:
+            _write_node(el, c)
+        return minidom.parseString(ET.tostring(el, "utf-8")).toprettyxml(indent="  ")

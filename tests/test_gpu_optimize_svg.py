@@ -104,7 +104,7 @@ def test_training_recovers_rotation_and_translation(tmp_path):
     s.global_override(["paths", "optimize_points"], False)
     s.global_override(["transforms", "transform_lr"], 1e-2)
     o = pd.OptimizableSvg(str(src), s)
-    node = o.root.children[0].transform_optim
+    node = o.node('g').transform
     losses = []
     for it in range(150):
         o.zero_grad()
@@ -114,12 +114,12 @@ def test_training_recovers_rotation_and_translation(tmp_path):
         losses.append(float(loss))
         o.step()
         if it % 25 == 0:
-            print(it, losses[-1], float(node.Theta), float(node.scale), node.translation.tolist(), flush=True)
+            print(it, losses[-1], float(node.theta), float(node.scale), node.translate.tolist(), flush=True)
     pd.synchronize()
-    theta = float(node.Theta)
+    theta = float(node.theta)
     # the group's translation is applied before its rotation and scale: compare the
     # composed matrix's offset with the target's canvas offset
-    M = node.get_transform().detach().cpu().numpy()
+    M = node.matrix().detach().cpu().numpy()
     print('final', losses[-1], theta, M.tolist(), flush=True)
     assert np.mean(losses[-5:]) < 0.1 * np.mean(losses[:3]), losses
     assert abs(theta - angle) < 0.1 * angle, theta

@@ -19,7 +19,10 @@ import pytest
 import torch
 
 import diffvg_amd as pydiffvg
-from diffvg_amd.optimize_svg import OptimizableSvg, SvgOptimizationSettings, TransformTools
+from diffvg_amd import optimize_svg as osv
+from diffvg_amd.optimize_svg import (OptimizableSvg, ParameterBank, SvgOptimizationSettings, TransformTools,
+                                     colour_hex, parse_colour, parse_length, parse_opacity,
+                                     parse_transform_list, transform_features)
 
 pytestmark = pytest.mark.filterwarnings('ignore')
 
@@ -93,14 +96,14 @@ def test_decompose_closed_forms(M, theta, scale, shear, tr):
     np.testing.assert_allclose(d.translate, tr, atol=1e-12)
 
 
-def test_analyze_transform():
-    p = TransformTools.analyze_transform(TransformTools.decompose(_rot(10) @ _sc(2, 2) @ _tr(1, 0)))
-    assert p.has_rotation and p.has_scale and not p.has_mirror and p.scale_uniform
-    assert not p.has_shear and p.has_translation
-    p = TransformTools.analyze_transform(TransformTools.decompose(_sc(1, -2)))
-    assert p.has_mirror and not p.scale_uniform and not p.has_rotation
-    p = TransformTools.analyze_transform(TransformTools.decompose(np.eye(3)))
-    assert p.scale_uniform and not any(p._replace(scale_uniform=False))
+def test_transform_features():
+    p = transform_features(TransformTools.decompose(_rot(10) @ _sc(2, 2) @ _tr(1, 0)))
+    assert p['rotation'] and p['scale'] and not p['mirror'] and p['uniform']
+    assert not p['shear'] and p['translation']
+    p = transform_features(TransformTools.decompose(_sc(1, -2)))
+    assert p['mirror'] and not p['uniform'] and not p['rotation']
+    p = transform_features(TransformTools.decompose(np.eye(3)))
+    assert p['uniform'] and not any(v for k, v in p.items() if k != 'uniform')
 
 
 @pytest.mark.parametrize('text, M', [
@@ -117,34 +120,37 @@ def test_analyze_transform():
     ('translate(1,2)rotate(45)', _tr(1, 2) @ _rot(45)),
 ])
 def test_parse_transform(text, M):
-    class N:
-        attrib = {'transform': text}
-    np.testing.assert_allclose(OptimizableSvg.parseTransform(N), M, atol=1e-12)
+    np.testing.assert_allclose(parse_transform_list(text), M, atol=1e-12)
+
+
+def test_parse_transform_errors():
+    for bad in ('spin(3)', 'translate(1 2 3)', 'rotate(1 2)', 'scale 2'):
+        with pytest.raises(ValueError):
+            parse_transform_list(bad)
 
 
 def test_rotate_about_centre_keeps_it_fixed():
-    class N:
-        attrib = {'transform': 'rotate(37 12 -5)'}
-    M = OptimizableSvg.parseTransform(N)
+    M = parse_transform_list('rotate(37 12 -5)')
     np.testing.assert_allclose(M @ [12, -5, 1], [12, -5, 1], atol=1e-12)
 
 
 def test_lengths_opacities_colours():
-    assert OptimizableSvg.parseLength('3') == 3.0
-    assert OptimizableSvg.parseLength('2.5px') == 2.5
-    assert OptimizableSvg.parseLength('2mm') == 8.0
-    assert OptimizableSvg.parseLength('1in') == pytest.approx(101.6)
-    assert OptimizableSvg.parseLength('72pt') == pytest.approx(101.6)
+    assert parse_length('3') == 3.0
+    assert parse_length('2.5px') == 2.5
+    assert parse_length('2mm') == 8.0
+    assert parse_length('1in') == pytest.approx(101.6)
+    assert parse_length('72pt') == pytest.approx(101.6)
     with pytest.raises(ValueError):
-        OptimizableSvg.parseLength('3em')
-    assert OptimizableSvg.parseOpacity('50%') == 0.5
-    assert OptimizableSvg.parseOpacity('1.5') == 1.0
-    assert OptimizableSvg.parseOpacity('-1') == 0.0
-    np.testing.assert_allclose(OptimizableSvg.parse_color('#ff8000').numpy(), [1, 128 / 255, 0], rtol=1e-6)
-    np.testing.assert_allclose(OptimizableSvg.parse_color('#f80').numpy(), [1, 8 / 15, 0], rtol=1e-6)
-    with pytest.raises(ValueError):
-        OptimizableSvg.parse_color('red')
-    assert OptimizableSvg.rgb_to_string(torch.tensor([1.0, 0.5, 1.2])) == '#ff7fff'
+        parse_length('3em')
+    assert parse_opacity('50%') == 0.5
+    assert parse_opacity('1.5') == 1.0
+    assert parse_opacity('-1') == 0.0
+    np.testing.assert_allclose(parse_colour('#ff8000').numpy(), [1, 128 / 255, 0], rtol=1e-6)
+    np.testing.assert_allclose(parse_colour('#f80').numpy(), [1, 8 / 15, 0], rtol=1e-6)
+    for bad in ('red', '#12345', '#ggg'):
+        with pytest.raises(ValueError):
+            parse_colour(bad)
+    assert colour_hex(torch.tensor([1.0, 0.5, 1.2])) == '#ff7fff'
 
 
 # ---- documents ---------------------------------------------------------------------------
@@ -292,20 +298,22 @@ def test_multi_subpath_ids(tmp_path):
 
 def test_trainable_parameters(doc):
     o = OptimizableSvg(doc)
-    path_node = o.root.children[0].children[0].children[0]
-    assert path_node.paths[0].points.requires_grad
-    # shapes other than paths get an optimizer but do not require grad (kept as is)
-    rect_node = o.root.children[0].children[1]
-    assert not rect_node.rect.requires_grad
+    assert o.node('tri').geometry['paths'][0].points.requires_grad
+    # shapes other than paths keep their geometry (not trained)
+    assert not o.node('box').geometry['values'].requires_grad
     # transforms (rigid mode by default) train from their decomposition
-    outer = o.root.children[0]
-    assert outer.transform_optim.optimizes and outer.transform_optim.residual is None
-    np.testing.assert_allclose(_mat(outer.transform_optim.get_transform()), _tr(10, 5), atol=1e-6)
-    # translation learning rate is scaled by the larger canvas side
-    lrs = [g['lr'] for g in outer.transform_optim.optimizer.param_groups]
-    assert lrs == pytest.approx([2e-3, 2e-3, 2e-3 * 64])
+    outer = o.node('outer').transform
+    assert outer.trained and outer.residual is None
+    np.testing.assert_allclose(_mat(outer.matrix()), _tr(10, 5), atol=1e-6)
+    # the translation's learning rate is scaled by the larger canvas side
+    assert o.bank.learning_rates(outer.theta) == pytest.approx([2e-3])
+    assert o.bank.learning_rates(outer.scale) == pytest.approx([2e-3])
+    assert o.bank.learning_rates(outer.translate) == pytest.approx([2e-3 * 64])
     # colours
-    assert o.root.children[2].appearance['fill'][1].requires_grad
+    assert o.node('ell').style['fill'].rgb.requires_grad
+    # every registered tensor is a leaf that requires grad, registered once
+    ts = o.bank.tensors()
+    assert all(t.is_leaf and t.requires_grad for t in ts) and len(set(map(id, ts))) == len(ts)
 
 
 @pytest.mark.parametrize('mode, text, residual', [
@@ -316,6 +324,7 @@ def test_trainable_parameters(doc):
     ('similarity', 'scale(2 -2) rotate(10)', False),
     ('similarity', 'scale(1 3)', True),
     ('affine', 'matrix(1 0.2 -0.4 1.5 3 -7)', False),
+    ('affine', 'scale(1 -1)', False),
 ])
 def test_transform_modes_start_at_the_parsed_transform(tmp_path, mode, text, residual):
     f = tmp_path / 'a.svg'
@@ -324,40 +333,60 @@ def test_transform_modes_start_at_the_parsed_transform(tmp_path, mode, text, res
     s = SvgOptimizationSettings()
     s.global_override(["transforms", "transform_mode"], mode)
     o = OptimizableSvg(str(f), s)
-    node = o.root.children[0]
-    to = node.transform_optim
+    to = o.node('g').transform
     assert (to.residual is not None) == residual
-    M = o.parseTransform(type('N', (), {'attrib': {'transform': text}}))
-    T = to.get_transform()
-    np.testing.assert_allclose(_mat(T), M, atol=1e-5)
+    T = to.matrix()
+    np.testing.assert_allclose(_mat(T), parse_transform_list(text), atol=1e-5)
     # gradients reach the parameters
     T.sum().backward()
-    assert to.Theta.grad is not None and to.translation.grad is not None
-    # the written transform parses back to the current one after a step
+    assert to.theta.grad is not None and to.translate.grad is not None
+    # the written transform parses back to the current one after the parameters move
     with torch.no_grad():
-        to.Theta += 0.1
-        to.translation += torch.tensor([0.5, -0.25])
-    back = o.parseTransform(type('N', (), {'attrib': {'transform': node.tftostring()}}))
-    np.testing.assert_allclose(back, _mat(to.get_transform()), atol=1e-5)
+        to.theta += 0.1
+        to.translate += torch.tensor([0.5, -0.25])
+    np.testing.assert_allclose(parse_transform_list(to.svg()), _mat(to.matrix()), atol=1e-5)
 
 
-def test_color_and_stop_optimizers_keep_ranges():
-    c = torch.tensor([0.5, 0.0, 1.0], requires_grad=True)
-    co = OptimizableSvg.ColorOptimizer(c, torch.optim.SGD, 1.0)
+def test_projections_keep_ranges():
+    """Colours stay in [1e-4, 1]; stop offsets are clamped to [0, 1], sorted, and their
+    ends pinned to 0 and 1 after every step."""
+    c = torch.tensor([0.5, 0.0, 1.0])
+    s = torch.tensor([0.0, 0.3, 0.6, 1.0])
+    bank = ParameterBank()
+    bank.add(c, 1.0, 'SGD', osv._project_colour)
+    bank.add(s, 1.0, 'SGD', osv._project_stops)
     c.grad = torch.tensor([1.0, 1.0, -1.0])
-    co.step()
-    np.testing.assert_allclose(c.detach().numpy(), [1e-4, 1e-4, 1.0])
-    s = torch.tensor([0.0, 0.3, 0.6, 1.0], requires_grad=True)
-    so = OptimizableSvg.StopOptimizer(s, torch.optim.SGD, 1.0)
     s.grad = torch.tensor([-0.5, 0.5, -0.2, 0.0])
-    so.step()
+    bank.step()
+    np.testing.assert_allclose(c.detach().numpy(), [1e-4, 1e-4, 1.0])
     # after the step [0.5, -0.2, 0.8, 1.0]; clamped [0.5, 0, 0.8, 1]; sorted
     # [0, 0.5, 0.8, 1]; ends pinned to 0 and 1
     np.testing.assert_allclose(s.detach().numpy(), [0.0, 0.5, 0.8, 1.0], atol=1e-7)
     # [0, 0.5, 0.8, 0.5] sorts to [0, 0.5, 0.5, 0.8], then the last is pinned to 1
+    bank.zero_grad()
+    c.grad = torch.zeros(3)
     s.grad = torch.tensor([0.0, 0.0, 0.0, 0.5])
-    so.step()
+    bank.step()
     np.testing.assert_allclose(s.detach().numpy(), [0.0, 0.5, 0.5, 1.0], atol=1e-7)
+
+
+def test_one_bank_matches_separate_optimizers():
+    """Adam over parameter groups in one optimizer steps each tensor exactly as a
+    separate Adam per tensor does."""
+    torch.manual_seed(0)
+    a, b = torch.randn(5), torch.randn(3)
+    a2, b2 = a.clone().requires_grad_(True), b.clone().requires_grad_(True)
+    bank = ParameterBank()
+    bank.add(a, 0.1, 'Adam')
+    bank.add(b, 0.01, 'Adam')
+    oa, ob = torch.optim.Adam([a2], lr=0.1), torch.optim.Adam([b2], lr=0.01)
+    for _ in range(4):
+        ga, gb = torch.randn(5), torch.randn(3)
+        a.grad, b.grad, a2.grad, b2.grad = ga.clone(), gb.clone(), ga.clone(), gb.clone()
+        bank.step()
+        oa.step()
+        ob.step()
+    assert torch.equal(a, a2) and torch.equal(b, b2)
 
 
 def test_step_moves_points_and_rebuilds(doc):
@@ -385,7 +414,7 @@ def test_optimize_alpha_trains_opacity(tmp_path):
     _, _, _, (g,) = o.build_scene()
     g.fill_color[3].backward()
     o.step()
-    assert float(o.root.children[0].appearance['fill-opacity']) == pytest.approx(0.4)
+    assert float(o.nodes('circle')[0].style['fill-opacity']) == pytest.approx(0.4)
 
 
 # ---- writing -------------------------------------------------------------------------------
@@ -457,7 +486,7 @@ def test_written_document_text(doc):
     text = OptimizableSvg(doc).write_xml()
     assert '<defs>' in text and '<style type="text/css">.warm {fill: #ff0000;fill-opacity: 0.5}' in text
     assert 'xlink:href="#base"' in text and 'gradientUnits="userSpaceOnUse"' in text
-    assert 'd="M 0.0,0.0 L 8.0,0.0 L 4.0,6.0 L 0.0,0.0 Z "' in text
+    assert 'd="M 0 0 L 8 0 L 4 6 L 0 0 Z"' in text
     assert 'opacity:0.5' in text and 'tensor' not in text
 
 
