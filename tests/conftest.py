@@ -9,10 +9,6 @@ for p in (ROOT, os.path.join(ROOT, 'tests')):
         sys.path.insert(0, p)
 
 
-def pytest_configure(config):
-    config.addinivalue_line('markers', 'gpu: needs an MI355X GPU')
-
-
 def gpu_available():
     try:
         import torch
