@@ -80,6 +80,7 @@ def lib():
     L.dvg_scene_create.argtypes = [ctypes.POINTER(SceneDesc), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
     L.dvg_scene_destroy.argtypes = [ctypes.c_void_p]
     L.dvg_render.argtypes = [ctypes.c_void_p, ctypes.POINTER(RenderArgs)]
+    L.dvg_render_noalloc.argtypes = [ctypes.c_void_p, ctypes.POINTER(RenderArgs)]
     L.dvg_scene_get_grads.argtypes = [ctypes.c_void_p, ctypes.POINTER(Grads)]
     L.dvg_scene_zero_grads.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.dvg_scene_set_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]

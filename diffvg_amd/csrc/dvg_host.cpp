@@ -740,6 +740,21 @@ int dvg_render(dvg_scene *scene, const dvg_render_args *args) {
     return rc;
 }
 
+int dvg_render_noalloc(dvg_scene *scene, const dvg_render_args *args) {
+    if (!scene || !args) return set_error("null argument");
+    const dvg_render_args &a = *args;
+    if (a.width <= 0 || a.height <= 0 || a.num_samples_x <= 0 || a.num_samples_y <= 0)
+        return set_error("width, height and sample counts must be positive");
+    if (a.num_eval_positions < 0) return set_error("negative eval position count");
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    if (prev != scene->h.device) (void)hipSetDevice(scene->h.device);
+    int rc = render_noalloc(scene->h, a);
+    scene->h.last_stream = (hipStream_t)a.stream;
+    if (prev >= 0 && prev != scene->h.device) (void)hipSetDevice(prev);
+    return rc;
+}
+
 struct dvg_batch {
     dvg::BatchHost *b;
 };

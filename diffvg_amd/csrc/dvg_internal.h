@@ -120,7 +120,11 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc);
 int update_scene(SceneHost &sc, const dvg_scene_params &p, float filter_radius, hipStream_t st);
 int scene_status(SceneHost &sc);
 void free_scene(SceneHost &sc);
-int render_tiled(SceneHost &sc, const dvg_render_args &a);
+// `fixed`: a workspace the caller carved (the allocation-free entry point); the render
+// then neither grows a workspace nor keeps fragment lists for its backward.
+int render_tiled(SceneHost &sc, const dvg_render_args &a, Workspace *fixed = nullptr);
+// dvg_render_noalloc: the TILED render with its workspace in a fixed device array.
+int render_noalloc(SceneHost &sc, const dvg_render_args &a);
 int render_plain(SceneHost &sc, const dvg_render_args &a);
 // Rebuilds the PLAIN schedule's top-level BVH from the scene's current group boxes (after
 // dvg_scene_update); synchronous.

@@ -1488,7 +1488,7 @@ static bool keep_buffer(SceneHost &sc, void *&ptr, size_t &granted, hipStream_t 
     return ptr != nullptr;
 }
 
-int render_tiled(SceneHost &sc, const dvg_render_args &a) {
+int render_tiled(SceneHost &sc, const dvg_render_args &a, Workspace *fixed) {
     hipStream_t st = (hipStream_t)a.stream;
     Geom G = make_geom(sc, a);
     const bool eval_mode = a.eval_positions && a.num_eval_positions > 0;
@@ -1503,13 +1503,21 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
     const bool sdf_out = a.render_sdf || a.d_render_sdf;
     int ntiles = G.tiles_x * G.tiles_y;
     if (color_out) {
-        Workspace &ws = acquire_workspace(sc.device, st);
+        Workspace &ws = fixed ? *fixed : acquire_workspace(sc.device, st);
         struct Release {
             int d;
             hipStream_t s;
-            ~Release() { release_workspace(d, s); }
-        } release_guard{sc.device, st};
-        if (int e = ws.reserve(npix, nsamp, ntiles)) return e;
+            bool on;
+            ~Release() {
+                if (on) release_workspace(d, s);
+            }
+        } release_guard{sc.device, st, fixed == nullptr};
+        if (fixed) {
+            if (npix > ws.npix_cap || nsamp > ws.nsamp_cap || (size_t)ntiles > ws.ntiles_cap)
+                return set_error("render larger than the fixed workspace");
+        } else if (int e = ws.reserve(npix, nsamp, ntiles)) {
+            return e;
+        }
         {
             DVG_PROF(st, "k_weight");
             k_weight<<<(unsigned)((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(S, G, ws.weight);
@@ -1526,15 +1534,16 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
             sc.pkeep_valid = false;
             const size_t frag_bytes = (size_t)(kFragSlots + 1) * nsamp * sizeof(uint16_t);
             const size_t pkeep_bytes = (size_t)(kPFragSlots * kPFragWords + 1) * nsamp * sizeof(int);
-            // a backward on another stream may still read the lists being replaced
-            if (!pre && frag_bytes <= kFragCacheBytes && S.ng < kFragMaxGroups) {
+            // a backward on another stream may still read the lists being replaced; with
+            // a fixed workspace nothing is kept (keeping may allocate)
+            if (!fixed && !pre && frag_bytes <= kFragCacheBytes && S.ng < kFragMaxGroups) {
                 if (sc.frag && sc.frag_stream != st) DVG_CHECK(hipStreamSynchronize(sc.frag_stream));
                 void *b = sc.frag;
                 const bool ok = keep_buffer(sc, b, sc.frag_granted, sc.frag_stream, frag_bytes);
                 sc.frag = (uint16_t *)b;
                 frag = ok ? sc.frag : nullptr;
             }
-            if (pre && pkeep_bytes <= kFragCacheBytes) {
+            if (!fixed && pre && pkeep_bytes <= kFragCacheBytes) {
                 if (sc.pkeep && sc.pkeep_stream != st) DVG_CHECK(hipStreamSynchronize(sc.pkeep_stream));
                 void *b = sc.pkeep;
                 const bool ok = keep_buffer(sc, b, sc.pkeep_granted, sc.pkeep_stream, pkeep_bytes);
@@ -1621,6 +1630,76 @@ int render_tiled(SceneHost &sc, const dvg_render_args &a) {
     DVG_CHECK(hipGetLastError());
     // nothing waits here: conditions the kernels flag are reported by scene_status
     return 0;
+}
+
+// ---------------------------------------------------------------- allocation-free render
+// dvg_render_noalloc's workspace: one file-scope device array per GPU, carved like
+// Workspace for at most kNaMaxPixels pixels and kNaMaxSamples samples (54 MiB), held by
+// every GPU the library runs on until the process ends.  Host code never uses the
+// array's name as an address: na_base() asks the runtime for the device address, once per
+// device, with that device current.
+namespace {
+constexpr size_t kNaMaxPixels = size_t(1) << 19;
+constexpr size_t kNaMaxSamples = size_t(1) << 19;
+constexpr size_t kNaMaxTiles = kNaMaxPixels;  // every tile holds at least one pixel
+constexpr size_t na_align(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t kNaWeight = 0;
+constexpr size_t kNaColors = kNaWeight + na_align(kNaMaxPixels * sizeof(float));
+constexpr size_t kNaEdges = kNaColors + na_align(kNaMaxSamples * sizeof(float4));
+constexpr size_t kNaEdgeTile = kNaEdges + na_align(kNaMaxSamples * sizeof(EdgeSample));
+constexpr size_t kNaEdgeSorted = kNaEdgeTile + na_align(kNaMaxSamples * sizeof(int));
+constexpr size_t kNaTileCount = kNaEdgeSorted + na_align(kNaMaxSamples * sizeof(int));
+constexpr size_t kNaTileOff = kNaTileCount + na_align(kNaMaxTiles * sizeof(int));
+constexpr size_t kNaTileCursor = kNaTileOff + na_align(kNaMaxTiles * sizeof(int));
+constexpr size_t kNaChunkOff = kNaTileCursor + na_align(kNaMaxTiles * sizeof(int));
+constexpr size_t kNaBytes = kNaChunkOff + na_align((kNaMaxTiles + 1) * sizeof(int));
+constexpr int kNaMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+static std::mutex na_mu;
+static unsigned char *na_ptr[kNaMaxDevices];
+
+// The device address of `scratch` on `device` (the current device).
+static int na_base(int device, unsigned char **out) {
+    if (device < 0 || device >= kNaMaxDevices) return set_error("device index out of range");
+    std::lock_guard<std::mutex> lk(na_mu);
+    if (!na_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        na_ptr[device] = (unsigned char *)p;
+    }
+    *out = na_ptr[device];
+    return 0;
+}
+
+int render_noalloc(SceneHost &sc, const dvg_render_args &a) {
+    if ((int64_t)a.num_samples_x * a.num_samples_y > 256)
+        return set_error("dvg_render_noalloc takes at most 256 samples per pixel");
+    const size_t npix = (size_t)a.width * a.height;
+    const size_t nsamp = npix * a.num_samples_x * a.num_samples_y;
+    if (npix > kNaMaxPixels || nsamp > kNaMaxSamples)
+        return set_error("dvg_render_noalloc renders at most " + std::to_string(kNaMaxPixels) + " pixels and " +
+                         std::to_string(kNaMaxSamples) + " samples");
+    unsigned char *base = nullptr;
+    if (int e = na_base(sc.device, &base)) return e;
+    Workspace ws;
+    ws.npix_cap = kNaMaxPixels;
+    ws.nsamp_cap = kNaMaxSamples;
+    ws.ntiles_cap = kNaMaxTiles;
+    ws.weight = (float *)(base + kNaWeight);
+    ws.colors = (float4 *)(base + kNaColors);
+    ws.edges = (EdgeSample *)(base + kNaEdges);
+    ws.edge_tile = (int *)(base + kNaEdgeTile);
+    ws.edge_sorted = (int *)(base + kNaEdgeSorted);
+    ws.tile_count = (int *)(base + kNaTileCount);
+    ws.tile_off = (int *)(base + kNaTileOff);
+    ws.tile_cursor = (int *)(base + kNaTileCursor);
+    ws.chunk_off = (int *)(base + kNaChunkOff);
+    return render_tiled(sc, a, &ws);
 }
 
 int render_plain(SceneHost &sc, const dvg_render_args &a) {

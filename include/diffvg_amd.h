@@ -131,6 +131,17 @@ int dvg_scene_destroy(dvg_scene *scene);
  * like the reference. */
 int dvg_render(dvg_scene *scene, const dvg_render_args *args);
 
+/* dvg_render in DVG_MODE_TILED (whatever the scene's mode) that allocates no device
+ * memory: no reference counterpart.  For callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops).  Its workspace is a fixed array in device memory,
+ * one per GPU, held from the first call on that GPU until the process ends; renders of
+ * at most 524,288 pixels and 524,288 samples (at most 256 per pixel), larger ones fail
+ * with an error.  Every call on a GPU shares that GPU's array: calls that overlap in
+ * time on one GPU (on different streams, or with dvg_render_noalloc calls of other
+ * threads) race and must not be made.  The forward keeps no fragment lists for its
+ * backward (the backward shades its samples again); results equal dvg_render's. */
+int dvg_render_noalloc(dvg_scene *scene, const dvg_render_args *args);
+
 /* Copies the accumulated gradients to host memory (synchronises the scene's last
  * stream first and fails with the conditions dvg_scene_status reports). */
 int dvg_scene_get_grads(dvg_scene *scene, dvg_grads *out);
