@@ -52,6 +52,12 @@ CPU_SAMPLE_SCENES = 64
 PEAK_HBM_GBS = 8000.0
 
 
+def progress(msg):
+    """One line per finished part of the run, on stderr (the JSON line is stdout's only
+    output)."""
+    print('[bench %s] %s' % (time.strftime('%H:%M:%S'), msg), file=sys.stderr, flush=True)
+
+
 def cpu_model():
     try:
         for line in open('/proc/cpuinfo'):
@@ -147,6 +153,7 @@ def flops(counts, adjoint=False):
 
 
 def run_cpu_baseline(cfg):
+    progress('cpu baseline (subprocess)')
     try:
         p = subprocess.run([sys.executable, os.path.abspath(__file__), '--cpu-baseline-only', '--config', str(cfg)],
                            capture_output=True, text=True, timeout=600)
@@ -261,25 +268,46 @@ def leg_batched(steps, warmup, autotune=False, e2e_steps=4, plain_steps=1):
         return _lib.DeviceScene(flatten_args(args).desc, 0)
 
     built = [build(i) for i in range(n)]
-    tuned = {}
-    if autotune:
-        from diffvg_amd.autotune import Tuner
-        tuner = Tuner()
-        for i, ds in enumerate(built):
-            tuner.tune(ds, sizes[i][0], sizes[i][1], 2, 2)
-        tuned = {str(k): v for k, v in tuner.best.items()}
+    progress('batch: %d scenes built' % n)
     fjobs = [(ds, dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0, render_image=view(img_flat, i)))
              for i, ds in enumerate(built)]
     bjobs = [(ds, dict(width=sizes[i][0], height=sizes[i][1], nsx=2, nsy=2, seed=0, d_render_image=view(d_flat, i)))
              for i, ds in enumerate(built)]
-    fb, bb = Batch(fjobs, zero_grads=False), Batch(bjobs, zero_grads=True)
+
+    def make_batches(persistent):
+        return (Batch(fjobs, zero_grads=False, persistent=persistent),
+                Batch(bjobs, zero_grads=True, persistent=persistent))
+
+    def eager_step(batches, st=None):
+        img_flat.zero_()
+        batches[0].render(st)
+        torch.mul(img_flat - target_flat, 2.0, out=d_flat)
+        batches[1].render(st)
+
+    tuned = None
+    persistent = False
+    if autotune:
+        # per scene signature: tile width and edge samples per wave; then the batch's
+        # launch structure (one launch per stage, or one persistent launch per pass)
+        from diffvg_amd.autotune import Tuner
+        tuner = Tuner()
+        t_tune = time.perf_counter()
+        for i, ds in enumerate(built):
+            tuner.tune(ds, sizes[i][0], sizes[i][1], 2, 2)
+        persistent, _ = tuner.tune_batch(make_batches, eager_step)
+        torch.cuda.synchronize()
+        progress('batch: autotuned in %.1f s' % (time.perf_counter() - t_tune))
+        tuned = {'choices': {str(k): list(v) for k, v in tuner.best.items()},
+                 'schedule': 'persistent' if persistent else 'staged',
+                 'schedule_ms': {k: (None if v is None else round(v, 3)) for k, v in tuner.batch_timings.items()},
+                 'tune_s': round(time.perf_counter() - t_tune, 2)}
+    fb, bb = make_batches(persistent)
+    # per-kernel timing (below) runs the one-launch-per-stage batches
+    fb_k, bb_k = (fb, bb) if not persistent else make_batches(False)
     stream = torch.cuda.Stream()
 
     def graph_body(st):
-        img_flat.zero_()
-        fb.render(st)
-        torch.mul(img_flat - target_flat, 2.0, out=d_flat)
-        bb.render(st)
+        eager_step((fb, bb), st)
 
     stream.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(stream):
@@ -301,15 +329,19 @@ def leg_batched(steps, warmup, autotune=False, e2e_steps=4, plain_steps=1):
         g.replay()
     torch.cuda.synchronize()
     t_graph = (time.perf_counter() - t0) / steps
+    progress('batch: graph replay %.2f ms' % (t_graph * 1e3))
     fb.status()
     bb.status()
-    # per-kernel times: the same batches rendered eagerly with events around each launch
+    # per-kernel times: the same jobs rendered eagerly, one launch per stage, with events
+    # around each launch
     L.dvg_profile_enable(1)
     for _ in range(steps):
-        graph_body(None)
+        eager_step((fb_k, bb_k))
     torch.cuda.synchronize()
     L.dvg_profile_enable(0)
     kern = read_profile()
+    fb_k.status()
+    bb_k.status()
 
     # ---- batched end to end through the user-level autograd API
     from diffvg_amd.batch_render import render_batch
@@ -324,16 +356,19 @@ def leg_batched(steps, warmup, autotune=False, e2e_steps=4, plain_steps=1):
         loss = sum(((im - t) ** 2).sum() for im, t in zip(imgs, targets))
         loss.backward()
 
-    for _ in range(2):
-        batched_step()
-    pydiffvg.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(e2e_steps):
-        batched_step()
-    pydiffvg.synchronize()
-    torch.cuda.synchronize()
-    t_batched = (time.perf_counter() - t0) / e2e_steps
+    t_batched = None
+    if e2e_steps:
+        for _ in range(2):
+            batched_step()
+        pydiffvg.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(e2e_steps):
+            batched_step()
+        pydiffvg.synchronize()
+        torch.cuda.synchronize()
+        t_batched = (time.perf_counter() - t0) / e2e_steps
+        progress('batch: render_batch end to end %.1f ms' % (t_batched * 1e3))
 
     # ---- plain: one RenderFunction call per scene (PLAIN schedule)
     pydiffvg.set_render_mode('plain')
@@ -348,14 +383,17 @@ def leg_batched(steps, warmup, autotune=False, e2e_steps=4, plain_steps=1):
             img = pydiffvg.RenderFunction.apply(w, h, 2, 2, 0, None, *args)
             ((img - view(target_flat, i)) ** 2).sum().backward()
 
-    plain_step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(plain_steps):
+    t_plain = None
+    if plain_steps:
         plain_step()
-    pydiffvg.synchronize()
-    torch.cuda.synchronize()
-    t_plain = (time.perf_counter() - t0) / plain_steps
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(plain_steps):
+            plain_step()
+        pydiffvg.synchronize()
+        torch.cuda.synchronize()
+        t_plain = (time.perf_counter() - t0) / plain_steps
+        progress('batch: plain %.1f ms' % (t_plain * 1e3))
     pydiffvg.set_render_mode('tiled')
     return dict(samples=samples, scenes=n, paths=int(sum(len(it['shapes']) for it in items)), t_graph=t_graph,
                 t_batched=t_batched, t_plain=t_plain, kernels=kern, tuned=tuned, steps=steps,
@@ -407,6 +445,7 @@ def leg_serving(n_req, hbm_budget_gib=64, max_batch=32, sweep=True, p99_bar_ms=5
     serve.serve_plain(cal, [0.0] * n_req)
     t_plain_all = time.perf_counter() - t0
     mu = n_req / t_plain_all
+    progress('serving: plain service rate %.2f requests/s' % mu)
     lam = 0.8 * mu
     gaps = rng.exponential(1.0, n_req)  # unit-rate Poisson gaps, scaled per arrival rate
 
@@ -418,6 +457,7 @@ def leg_serving(n_req, hbm_budget_gib=64, max_batch=32, sweep=True, p99_bar_ms=5
     torch.cuda.synchronize()
     done = server().run([make(i) for i in range(n_req)], arrivals(lam))
     st_b = serve.latency_stats(done)
+    progress('serving: served p99 %.1f ms' % st_b['p99_ms'])
     plain_done = serve.serve_plain([make(i) for i in range(n_req)], arrivals(lam))
     st_p = serve.latency_stats(plain_done)
     # each request alone on an idle device (its service time without queueing)
@@ -440,6 +480,7 @@ def leg_serving(n_req, hbm_budget_gib=64, max_batch=32, sweep=True, p99_bar_ms=5
             row = {'arrival_rps': round(rate, 2), 'p50_ms': round(s_['p50_ms'], 2), 'p99_ms': round(s_['p99_ms'], 2),
                    'throughput_rps': round(s_['throughput_rps'], 2), 'msamples_s': round(samples / span / 1e6, 2)}
             sweep_rows.append(row)
+            progress('serving: sweep %s' % row)
             if s_['p99_ms'] > p99_bar_ms or rate > 64 * mu:
                 break
             saturation = row
@@ -503,6 +544,7 @@ def leg_single(cfg, steps, warmup, mode='tiled', prefilter_steps=0):
     pydiffvg.synchronize()
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / steps
+    progress('single %s (%s): %.2f ms per step' % (c['name'], mode, sec * 1e3))
     # per-kernel times: the same steps again with the library's events around each launch
     # (kept out of the timed steps above, whose launches they would space out)
     L.dvg_profile_enable(1)
@@ -567,6 +609,10 @@ def _kernels_out(kern):
 
 
 def _batched_fields(b):
+    if b['t_batched'] is None or b['t_plain'] is None:  # the graph replay alone (profiling runs)
+        return {'kernels_ms_eager': {k: round(v['avg_ms'], 4) for k, v in b['kernels'].items()},
+                'kernel_ms_per_step': round(sum(v['total_ms'] for v in b['kernels'].values()) / b['steps'], 4),
+                'autotune': b['tuned']}
     return {
         'batched_end_to_end': {'ms_per_step': round(b['t_batched'] * 1e3, 3),
                                'value': round(b['samples'] / b['t_batched'] / 1e6, 4), 'steps': b['e2e_steps'],
@@ -579,7 +625,7 @@ def _batched_fields(b):
         'speedup_batched_vs_plain': round(b['t_plain'] / b['t_batched'], 2),
         'kernels_ms_eager': {k: round(v['avg_ms'], 4) for k, v in b['kernels'].items()},
         'kernel_ms_per_step': round(sum(v['total_ms'] for v in b['kernels'].values()) / b['steps'], 4),
-        'autotuned_tile_widths': b['tuned'],
+        'autotune': b['tuned'],
     }
 
 
@@ -619,7 +665,7 @@ def run_headline(a):
     """The default line: the 256-scene batch (value), the serving stream and both 512x512
     single scenes."""
     cpu = None if a.no_cpu_baseline else run_cpu_baseline(4)
-    b = leg_batched(a.steps, a.warmup, autotune=a.autotune)
+    b = leg_batched(a.steps, a.warmup, autotune=not a.no_autotune)
     s = leg_serving(a.requests, a.hbm_budget, a.max_batch)
     singles = {}
     for cfg, (st, wu, pst) in ((2, (30, 10, 8)), (3, (12, 4, 0))):
@@ -633,7 +679,8 @@ def run_headline(a):
         'config': {'workload': 'uneven_scenes_256', 'scenes': b['scenes'], 'samples': b['samples'],
                    'paths': b['paths'], 'spp': [2, 2], 'filter': 'box r=0.5',
                    'mode': 'batched: forward batch + loss gradient + backward batch replayed from one HIP graph, '
-                           'scenes resident in HBM'},
+                           'scenes resident in HBM',
+                   'schedule': (b['tuned'] or {}).get('schedule', 'staged')},
     }
     out.update(_batched_fields(b))
     out['serving_stream'] = _serving_fields(s)
@@ -649,12 +696,14 @@ def run_config(a):
             'vs_baseline': None, 'dtype': DTYPE, 'data': 'synthetic'}
     if a.config == 4:
         cpu = None if a.no_cpu_baseline else run_cpu_baseline(4)
-        b = leg_batched(a.steps, a.warmup, autotune=a.autotune)
+        b = leg_batched(a.steps, a.warmup, autotune=not a.no_autotune, e2e_steps=0 if a.graph_only else 4,
+                        plain_steps=0 if a.graph_only else 1)
         out = dict(base, value=round(b['samples'] / b['t_graph'] / 1e6, 4), steps=a.steps, warmup=a.warmup,
                    ms_per_step=round(b['t_graph'] * 1e3, 4),
                    config={'workload': 'uneven_scenes_256', 'scenes': b['scenes'], 'samples': b['samples'],
                            'paths': b['paths'], 'spp': [2, 2], 'filter': 'box r=0.5',
-                           'mode': 'batched, one HIP graph per fwd+bwd'})
+                           'mode': 'batched, one HIP graph per fwd+bwd',
+                           'schedule': (b['tuned'] or {}).get('schedule', 'staged')})
         out.update(_batched_fields(b))
         out['roofline'] = _roofline(cpu, b['kernels'], BATCH_KERNELS, 'uneven_scenes_256')
         out['cpu_baseline'] = _cpu_baseline(cpu)
@@ -723,7 +772,11 @@ def main():
     ap.add_argument('--copy-calibration', action='store_true', help='run a plain copy kernel of known size')
     ap.add_argument('--requests', type=int, default=120, help='serving stream length')
     ap.add_argument('--no-sweep', action='store_true', help='serving: skip the arrival-rate sweep')
-    ap.add_argument('--autotune', action='store_true', help='pick tile widths per workload signature (batch)')
+    ap.add_argument('--no-autotune', action='store_true',
+                    help='batch: keep the default tile shape, samples per wave and launch structure instead of '
+                         'timing the candidates per workload signature before the timed replays')
+    ap.add_argument('--graph-only', action='store_true',
+                    help='config 4: time the graph replay only (no end-to-end or PLAIN legs; for profiling runs)')
     ap.add_argument('--hbm-budget', type=int, default=64, help='GiB that requests in flight may hold (serving)')
     ap.add_argument('--max-batch', type=int, default=32, help='requests per batch (serving)')
     a = ap.parse_args()

@@ -85,6 +85,7 @@ def lib():
     L.dvg_scene_zero_grads.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.dvg_scene_set_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.dvg_scene_set_tiling.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.dvg_scene_set_samples_per_wave.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.dvg_scene_grad_layout.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64 * 7),
                                         ctypes.POINTER(ctypes.c_int64)]
     L.dvg_scene_copy_grads.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -149,6 +150,9 @@ class DeviceScene:
 
     def set_tiling(self, tile_width):
         check(self._lib.dvg_scene_set_tiling(self.handle, int(tile_width)))
+
+    def set_samples_per_wave(self, edge_samples_per_wave):
+        check(self._lib.dvg_scene_set_samples_per_wave(self.handle, int(edge_samples_per_wave)))
 
     @staticmethod
     def make_args(width, height, nsx, nsy, seed, background=None, render_image=None, render_sdf=None,

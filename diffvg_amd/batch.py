@@ -16,10 +16,15 @@ import ctypes
 from . import _lib
 
 ZERO_GRADS = 1
+PERSISTENT = 2
 
 
 class Batch:
-    def __init__(self, jobs, zero_grads=True):
+    """`persistent`: the whole render is one kernel launch whose workgroups take the
+    batch's work items from a device queue (DVG_BATCH_PERSISTENT), instead of one launch
+    per stage."""
+
+    def __init__(self, jobs, zero_grads=True, persistent=False):
         L = _lib.lib()
         self._lib = L
         n = len(jobs)
@@ -35,9 +40,11 @@ class Batch:
             keep.extend(v for v in kw.values() if hasattr(v, 'data_ptr'))
         self._keep = keep
         h = ctypes.c_void_p()
-        _lib.check(L.dvg_batch_create(scenes, args, n, ZERO_GRADS if zero_grads else 0, ctypes.byref(h)))
+        flags = (ZERO_GRADS if zero_grads else 0) | (PERSISTENT if persistent else 0)
+        _lib.check(L.dvg_batch_create(scenes, args, n, flags, ctypes.byref(h)))
         self.handle = h
         self.n = n
+        self.persistent = bool(persistent)
 
     def render(self, stream=None):
         """Enqueues the batch on `stream` (a raw HIP stream handle; None = null stream)."""

@@ -714,6 +714,14 @@ int dvg_scene_set_tiling(dvg_scene *scene, int tile_width) {
     return 0;
 }
 
+int dvg_scene_set_samples_per_wave(dvg_scene *scene, int edge_samples_per_wave) {
+    if (!scene) return set_error("null scene");
+    if (edge_samples_per_wave != 0 && edge_samples_per_wave != 32 && edge_samples_per_wave != 64)
+        return set_error("edge samples per wave must be 0 (default), 32 or 64");
+    scene->h.edge_spw = edge_samples_per_wave == 64 ? 64 : 32;
+    return 0;
+}
+
 int dvg_scene_set_mode(dvg_scene *scene, int mode) {
     if (!scene) return set_error("null scene");
     if (mode != DVG_MODE_TILED && mode != DVG_MODE_PLAIN) return set_error("unknown render mode");
@@ -768,7 +776,7 @@ int dvg_batch_create(dvg_scene *const *scenes, const dvg_render_args *args, int 
         hs[i] = &scenes[i]->h;
     }
     dvg::BatchHost *b = nullptr;
-    if (int rc = dvg::batch_create(hs.data(), args, n, (flags & DVG_BATCH_ZERO_GRADS) ? 1 : 0, &b)) return rc;
+    if (int rc = dvg::batch_create(hs.data(), args, n, flags, &b)) return rc;
     *out = new dvg_batch{b};
     return 0;
 }

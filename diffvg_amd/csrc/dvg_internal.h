@@ -34,7 +34,7 @@ struct Workspace {
     int *edge_sorted = nullptr;
     int *tile_count = nullptr;
     int *tile_off = nullptr;
-    int *tile_cursor = nullptr;
+    int *edge_rank = nullptr;   // [nsamp] position of each boundary sample in its tile's bin
     int *chunk_off = nullptr;  // [ntiles + 1] boundary-sample chunk offsets per tile
     int reserve(size_t npix, size_t nsamp, size_t ntiles);
     void release();
@@ -72,6 +72,7 @@ struct SceneHost {
     bool device_set = false;      // a device was chosen and memory may be held on it
     int mode = DVG_MODE_TILED;
     int tile_w = 0;               // pixel tile width of the TILED schedule (0 = square default)
+    int edge_spw = 32;            // boundary samples per wave of the edge kernel: 32 or 64
     int cw = 0, ch = 0, num_shapes = 0, num_groups = 0;
     int num_shape_params = 0, num_points_total = 0, num_color_params = 0;
     void *block = nullptr;        // scene arrays
@@ -110,7 +111,8 @@ struct SceneHost {
 
 // Batched renders (one kernel per stage over the jobs of many scenes).
 struct BatchHost;
-int batch_create(SceneHost *const *scenes, const dvg_render_args *args, int n, int zero_grads, BatchHost **out);
+// flags: DVG_BATCH_ZERO_GRADS, DVG_BATCH_PERSISTENT
+int batch_create(SceneHost *const *scenes, const dvg_render_args *args, int n, int flags, BatchHost **out);
 int batch_render(BatchHost &b, hipStream_t st);
 int batch_status(BatchHost &b);
 void batch_destroy(BatchHost *b);

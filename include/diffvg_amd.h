@@ -160,6 +160,11 @@ int dvg_scene_set_mode(dvg_scene *scene, int mode);
  * square default).  Picked by the autotuner; results do not depend on it beyond float
  * summation order.  No reference counterpart (the reference has no tiles). */
 int dvg_scene_set_tiling(dvg_scene *scene, int tile_width);
+/* Boundary samples per wavefront of the TILED edge kernel for later renders and batches
+ * of this scene: 32 (each lane shades one of the two colour queries of a sample; the
+ * default, also chosen by 0) or 64 (each lane shades both).  Picked by the autotuner;
+ * results do not depend on it beyond float summation order.  No reference counterpart. */
+int dvg_scene_set_samples_per_wave(dvg_scene *scene, int edge_samples_per_wave);
 
 /* Device-resident parameter values of a scene: DEVICE pointers on the scene's GPU, in the
  * layout of the float arrays of dvg_scene_desc (points as x,y pairs; thickness 0 where a
@@ -217,6 +222,9 @@ int dvg_scene_debug_block(dvg_scene *scene, void *dst, int64_t cap, int64_t *byt
  * Jobs of one batch that share a scene, or renders of the batch's scenes outside it,
  * must not overlap in time. */
 #define DVG_BATCH_ZERO_GRADS 1
+/* DVG_BATCH_PERSISTENT (one kernel launch per render whose workgroups take the work
+ * items from a device queue) is disabled in this build: dvg_batch_create fails with it. */
+#define DVG_BATCH_PERSISTENT 2
 typedef struct dvg_batch dvg_batch;
 int dvg_batch_create(dvg_scene *const *scenes, const dvg_render_args *args, int n, int flags, dvg_batch **out);
 int dvg_batch_render(dvg_batch *batch, void *stream);

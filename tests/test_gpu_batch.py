@@ -100,7 +100,7 @@ def _single(flats):
     return res
 
 
-def _batched(flats, graph=False, replays=1):
+def _batched(flats, graph=False, replays=1, persistent=False):
     scs = [_lib.DeviceScene(f, 0) for f in flats]
     outs, fj, bj, keep = [], [], [], []
     for k, (f, (_, _, opt)) in enumerate(zip(flats, SPECS)):
@@ -111,7 +111,8 @@ def _batched(flats, graph=False, replays=1):
         bj.append((scs[k], kw2))
         outs.append((out, d_tr, d_bg))
         keep += [kp, kp2]
-    fb, bb = Batch(fj, zero_grads=False), Batch(bj, zero_grads=True)
+    fb = Batch(fj, zero_grads=False, persistent=persistent)
+    bb = Batch(bj, zero_grads=True, persistent=persistent)
 
     def run(stream):
         for out, d_tr, d_bg in outs:
@@ -204,10 +205,68 @@ def test_batch_graph_replay():
             _same(a, b, ('replay', r, k), exact=True)
 
 
+@pytest.mark.skip(reason='the persistent batch kernel did not finish on the GPU (cause not known); the path is disabled')
+def test_persistent_batch_matches_single_scene_renders():
+    """The same jobs as one persistent launch per batch (every stage of every job taken
+    from a device queue, each task waiting for its own job's earlier stages)."""
+    flats = _flats()
+    single = _single(flats)
+    batched = _batched(flats, persistent=True)[0]
+    for k, (a, b) in enumerate(zip(batched, single)):
+        print('persistent job', k, SPECS[k][0], SPECS[k][2], flush=True)
+        _same(a, b, ('persistent', k, SPECS[k][0]))
+
+
+@pytest.mark.skip(reason='the persistent batch kernel did not finish on the GPU (cause not known); the path is disabled')
+def test_persistent_batch_graph_replay():
+    """Persistent forward and backward launches captured in one graph and replayed three
+    times (the last workgroup of each launch resets the task counters for the next):
+    every replay equals the eager staged batch."""
+    flats = _flats()
+    eager = _batched(flats)[0]
+    reps = _batched(flats, graph=True, replays=3, persistent=True)
+    for r, res in enumerate(reps):
+        for k, (a, b) in enumerate(zip(res, eager)):
+            _same(a, b, ('persistent replay', r, k), exact=True)
+
+
+@pytest.mark.parametrize('persistent', [False, pytest.param(True, marks=pytest.mark.skip(
+    reason='the persistent batch kernel did not finish on the GPU (cause not known); the path is disabled'))])
+def test_batch_job_with_forward_and_backward(persistent):
+    """Jobs that render and differentiate in one call (render_image and d_render_image
+    both set, as dvg_render allows): the backward replays the fragment lists its own
+    forward keeps in the same launch, so it must wait for that forward."""
+    cases = ['overlap_eo', 'strokes', 'gradients', 'mixed_segments']
+    flats = [scenes.flat_case(c) for c in cases]
+    jobs, outs, want = [], [], []
+    for k, f in enumerate(flats):
+        w, h = f['canvas_width'], f['canvas_height']
+        d = torch.from_numpy(np.random.default_rng(30 + k).uniform(-1, 1, (h, w, 4)).astype(np.float32)).to(DEV)
+        one = _lib.DeviceScene(f, 0)
+        img1 = torch.zeros(h, w, 4, device=DEV)
+        one.render(w, h, 2, 2, k, render_image=img1, d_render_image=d)
+        torch.cuda.synchronize()
+        want.append((img1.cpu().numpy(), one.grads()))
+        sc = _lib.DeviceScene(f, 0)
+        img = torch.zeros(h, w, 4, device=DEV)
+        jobs.append((sc, dict(width=w, height=h, nsx=2, nsy=2, seed=k, render_image=img, d_render_image=d)))
+        outs.append((sc, img, d))
+    b = Batch(jobs, zero_grads=True, persistent=persistent)
+    b.render(None)
+    b.status()
+    for k, ((sc, img, _), (w_img, w_g)) in enumerate(zip(outs, want)):
+        assert np.array_equal(img.cpu().numpy(), w_img), k
+        g = sc.grads()
+        for key in GRAD_KEYS:
+            assert _rel(g[key], w_g[key]) <= 1e-4, (k, key)
+
+
 def test_batch_rejects_invalid_jobs():
     f = scenes.flat_case('single_circle_64')
     sc = _lib.DeviceScene(f, 0)
     out = torch.zeros(8, 8, 4, device=DEV)
+    with pytest.raises(RuntimeError, match='PERSISTENT is disabled'):
+        Batch([(sc, dict(width=8, height=8, nsx=1, nsy=1, seed=0, render_image=out))], persistent=True)
     with pytest.raises(RuntimeError, match='256 samples'):
         Batch([(sc, dict(width=8, height=8, nsx=17, nsy=16, seed=0, render_image=out))])
     with pytest.raises(RuntimeError, match='either colour or SDF'):

@@ -196,3 +196,54 @@ def test_render_batch_reports_non_finite_grad_at_sync():
     with pytest.raises(AssertionError):
         pd.synchronize()
     pd.synchronize()
+
+
+@pytest.mark.parametrize('kind', ['line', 'bezier_color'])
+def test_generative_front_end_matches_oracle(kind):
+    """line_render / bezier_render against the oracle: each item's scene (as the front end
+    returns it) rendered by the oracle at the seed the front end drew for it, put through
+    the same alpha product gives the batched output (within 2e-5: products of values within
+    1e-5); the oracle's backward for that item's image gradient, chained through
+    points = 0.5 (p + 1) canvas_size + jitter and colour = (rgb, alpha), gives the
+    gradients of the generative inputs (within 1e-3 relative)."""
+    from refdrive import OracleRunner, flatten
+    print('generative vs oracle', kind, flush=True)
+    bs, n, cs = 4, 5, 32
+    g = torch.Generator(device='cuda').manual_seed(3)
+    npts = 2 if kind == 'line' else 7
+    pts = (torch.rand(bs, n, npts, 2, device='cuda', generator=g) * 1.6 - 0.8).requires_grad_(True)
+    widths = (torch.rand(bs, n, device='cuda', generator=g) * 1.5 + 0.5).requires_grad_(True)
+    alphas = (torch.rand(bs, n, device='cuda', generator=g) * 0.8 + 0.2).requires_grad_(True)
+    colors = (torch.rand(bs, n, 3, device='cuda', generator=g).requires_grad_(True)
+              if kind == 'bezier_color' else None)
+    torch.manual_seed(5)
+    random.seed(7)
+    fn = generative.line_render if kind == 'line' else generative.bezier_render
+    out, items = fn(pts, widths, alphas, canvas_size=cs, colors=colors)
+    weight = torch.linspace(0, 1, out.numel(), device='cuda').view(out.shape)
+    (out * weight).sum().backward()
+    random.seed(7)
+    seeds = [random.randint(0, 1000000) for _ in range(bs)]  # one per item, as the front end draws them
+    want_pts = np.zeros((bs, n, npts, 2), np.float32)
+    want_w, want_a = np.zeros((bs, n), np.float32), np.zeros((bs, n), np.float32)
+    want_c = np.zeros((bs, n, 3), np.float32)
+    for k in range(bs):
+        w, h, shapes, groups = items[k]
+        f = flatten(w, h, shapes, groups, 0, 0.5)
+        o = OracleRunner(f)
+        raster = torch.from_numpy(o.render(w, h, 2, 2, seeds[k])['image']).requires_grad_(True)
+        r = raster.permute(2, 0, 1).reshape(4, cs, cs)
+        alpha = r[3:4]
+        img = r[:3] * alpha.repeat(3, 1, 1) if colors is not None else r[:1] * alpha
+        assert float((img.detach() - out[k].detach().cpu()).abs().max()) <= 2e-5, k
+        (img * weight[k].cpu()).sum().backward()
+        gr = o.render(w, h, 2, 2, seeds[k], d_image=raster.grad.numpy())['grads']
+        want_pts[k] = gr['points'].reshape(n, npts, 2) * (0.5 * cs)
+        want_w[k] = gr['stroke_width']
+        col = np.stack([gr['color_params'][f['stroke_offset'][i]:f['stroke_offset'][i] + 4] for i in range(n)])
+        want_a[k] = col[:, 3]
+        want_c[k] = col[:, :3]
+    got = [pts.grad, widths.grad, alphas.grad] + ([] if colors is None else [colors.grad])
+    want = [want_pts, want_w, want_a] + ([] if colors is None else [want_c])
+    for name, a, b in zip(('points', 'widths', 'alphas', 'colors'), got, want):
+        assert _rel(a.cpu(), torch.from_numpy(b)) <= 1e-3, name
