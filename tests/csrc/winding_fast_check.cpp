@@ -102,7 +102,10 @@ void wf_fuzz(int mode, int64_t n, uint64_t seed, int64_t *out, float *bad) {
 
 extern "C" {
 // Stroke coverage: whenever the fast rejection fires, the oracle's within-distance test
-// of a one-segment open path with that stroke width must reject.  out as wf_fuzz.
+// of a one-segment open path with that stroke width must reject.  Mode 3 checks the
+// closest-point use of the same test: whenever it fires for radius r, the oracle's
+// closest point of the segment (compute_distance with max_radius r) must find nothing
+// closer than r, so a search that skips the segment keeps its result.  out as wf_fuzz.
 void wf_within_fuzz(int mode, int64_t n, uint64_t seed, int64_t *out, float *bad) {
     std::mt19937_64 g(seed);
     std::uniform_real_distribution<float> U(0.f, 1.f);
@@ -174,7 +177,15 @@ void wf_within_fuzz(int mode, int64_t n, uint64_t seed, int64_t *out, float *bad
         d.filter_radius = 0.5f;
         orc_scene *sc = nullptr;
         if (orc_scene_create(&d, &sc) != 0) continue;
-        int within = orc_within_distance(sc, 0, ptx, pty);
+        int within;
+        if (mode == 3) {
+            float dd = 0.f, cxy[2];
+            int32_t sid, pinfo[2];
+            float tr;
+            within = orc_compute_distance(sc, 0, ptx, pty, r, &dd, cxy, &sid, pinfo, &tr);
+        } else {
+            within = orc_within_distance(sc, 0, ptx, pty);
+        }
         orc_scene_destroy(sc);
         if (within) {
             if (mism == 0) {
