@@ -11,6 +11,7 @@ from .color import *  # noqa: F401,F403
 from .pixel_filter import *  # noqa: F401,F403
 from . import diffvg  # noqa: F401
 from .render_pytorch import (RenderFunction, OutputType, set_print_timing, set_render_mode,  # noqa: F401
+                             set_direct_leaf_gradients,
                              get_render_mode, synchronize)
 from .batch import Batch  # noqa: F401
 from .parse_svg import svg_to_scene  # noqa: F401

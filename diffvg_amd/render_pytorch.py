@@ -16,6 +16,7 @@ Differences from the reference, all deliberate:
 * ``render_grad`` with a background image works (the reference reads an undefined
   variable there, render_pytorch.py:638).
 """
+import operator
 import time
 import warnings
 from enum import IntEnum
@@ -50,6 +51,28 @@ def set_render_mode(mode):
 
 def get_render_mode():
     return _render_mode
+
+
+_direct_leaf_grads = False
+
+
+def set_direct_leaf_gradients(enabled):
+    """Opt-in delivery of scene gradients straight into the ``.grad`` of the argument
+    tensors, bypassing the autograd engine's per-tensor accumulation.
+
+    With it on, a render whose float argument tensors that require a gradient are all
+    leaves (no grad_fn), each used once, float32, on the render device and without
+    gradient hooks is recorded as one autograd node without edges to those tensors; its
+    backward writes each tensor's gradient (a view of the scene's gradient block) into
+    ``.grad``, adding to a gradient already there, as ``loss.backward()`` would.  The
+    saving is the engine's per-tensor cost, which dominates the host time of scenes of
+    thousands of shapes.  What changes: ``torch.autograd.grad`` and
+    ``backward(inputs=...)`` cannot name those tensors (they are not in the graph and
+    the engine says so), and the gradients are not taken when a render's argument
+    tensors fail the conditions above -- such a render goes through the engine as
+    usual.  Off by default."""
+    global _direct_leaf_grads
+    _direct_leaf_grads = bool(enabled)
 
 
 class OutputType(IntEnum):
@@ -319,6 +342,25 @@ def flatten_args(args):
     return f
 
 
+def _picker(pos):
+    """args -> tuple of args[i] for i in pos, in one call (scenes hold thousands)."""
+    if not pos:
+        return lambda args: ()
+    if len(pos) == 1:
+        i = pos[0]
+        return lambda args: (args[i],)
+    return operator.itemgetter(*pos)
+
+
+_size = torch.Tensor.size
+_version = operator.attrgetter('_version')
+_requires_grad = operator.attrgetter('requires_grad')
+_grad_fn = operator.attrgetter('grad_fn')
+_hooks = operator.attrgetter('_backward_hooks')
+_device = operator.attrgetter('device')
+_dtype = operator.attrgetter('dtype')
+
+
 class _SceneTemplate:
     """Topology of one serialize_scene argument list: everything but the float values
     (shape kinds, segment counts, point counts, group members, paint kinds, flags,
@@ -336,7 +378,9 @@ class _SceneTemplate:
         tensor_pos = set(self.float_pos) | set(flat.pos['ncp']) | set(flat.pos['ids'])
         self.scalar_pos = [i for i in range(self.nargs) if i != 6 and i not in tensor_pos]
         self.scalar_vals = tuple(args[i] for i in self.scalar_pos)
-        self.float_shapes = [args[i].shape for i in self.float_pos]
+        self._get_scalars = _picker(self.scalar_pos)
+        self._get_floats = _picker(self.float_pos)
+        self.float_shapes = list(map(_size, self._get_floats(args)))
         # how each kind's tensors concatenate (their ranks are part of the topology)
         self.cat_mode = {}
         for kind in ('params', 'points', 'thick', 'sw', 'color', 'xform'):
@@ -354,34 +398,37 @@ class _SceneTemplate:
                 mode = 'flatten_each'
             self.cat_mode[kind] = mode
         self.int_pos = flat.pos['ncp'] + flat.pos['ids']
+        self._get_ints = _picker(self.int_pos)
+        self._get_kind = {kind: _picker(flat.pos[kind]) for kind in ('params', 'points', 'thick', 'sw', 'color',
+                                                                     'xform')}
         self.int_bytes = flat.int_bytes
         # the integer topology tensors last seen with these contents, and their versions:
         # the same tensors, unmodified, need no comparison of contents
-        self._int_seen = [args[i] for i in self.int_pos]
-        self._int_versions = [t._version for t in self._int_seen]
+        self._int_seen = self._get_ints(args)
+        self._int_versions = list(map(_version, self._int_seen))
         self.idle = []
         self._dev_tables = {}
+        self._runs = None
 
     def matches(self, args):
         if len(args) != self.nargs:
             return False
         try:
-            if tuple([args[i] for i in self.scalar_pos]) != self.scalar_vals:
+            if self._get_scalars(args) != self.scalar_vals:
                 return False
-            if [args[i].shape for i in self.float_pos] != self.float_shapes:
+            if list(map(_size, self._get_floats(args))) != self.float_shapes:
                 return False
         except (RuntimeError, TypeError, AttributeError):
             return False
-        cur = [args[i] for i in self.int_pos]
-        if all(a is b for a, b in zip(cur, self._int_seen)) and \
-                [t._version for t in cur] == self._int_versions:
+        cur = self._get_ints(args)
+        if all(map(operator.is_, cur, self._int_seen)) and list(map(_version, cur)) == self._int_versions:
             return True
         ncp = _cat_int([args[i] for i in self.flat.pos['ncp']])
         gids = _cat_int([args[i] for i in self.flat.pos['ids']])
         if ncp.tobytes() + b'|' + gids.tobytes() != self.int_bytes:
             return False
         self._int_seen = cur
-        self._int_versions = [t._version for t in cur]
+        self._int_versions = list(map(_version, cur))
         return True
 
     def _tables(self, dev):
@@ -409,7 +456,7 @@ class _SceneTemplate:
                 mode = self.cat_mode[kind]
                 if mode is None:
                     return None
-                srcs = [args[i] for i in f.pos[kind]]
+                srcs = self._get_kind[kind](args)
                 for attempt in range(2):
                     try:
                         if mode == 'stack':
@@ -438,6 +485,81 @@ class _SceneTemplate:
                 sw = torch.where(T['sw_zero'], torch.zeros((), device=dev), sw[T['sw_first']]).contiguous()
             return dict(shape_params=params, points=points, thickness=thickness, stroke_width=sw,
                         color_params=cat('color'), shape_to_canvas=cat('xform'))
+
+    def _direct_runs(self):
+        """The gradient slots of the float arguments (the map from argument positions to
+        gradient slices) grouped into runs that one view + unbind of the gradient block
+        covers: consecutive slots of one kind, shape and size.  False when some argument's
+        slot is not the whole tensor (a stroke width tensor of several elements)."""
+        if self._runs is None:
+            runs = []
+            # the filter radius is an input of the render node itself
+            slots = [e for e in self.flat.grad_map if e[1] != 'filter_radius']
+            for pos, kind, off, cnt, shape in sorted(slots, key=lambda e: (e[1], e[2])):
+                if int(np.prod(shape, dtype=np.int64)) != cnt:
+                    self._runs = False
+                    return False
+                r = runs[-1] if runs else None
+                if r and r[0] == kind and r[2] == cnt and r[3] == shape and r[1] + r[4] * cnt == off:
+                    r[4] += 1
+                    r[5].append(pos)
+                else:
+                    runs.append([kind, off, cnt, shape, 1, [pos]])
+            run_pos = [p for r in runs for p in r[5]]
+            self._run_get = _picker(run_pos)
+            # float arguments without a gradient slot (the stroke width of a path with
+            # per-point thickness) receive zeros, as the engine gives them
+            self._zero_pos = sorted(set(self.float_pos) - set(run_pos))
+            self._zero_get = _picker(self._zero_pos)
+            self._runs = runs
+        return self._runs
+
+    def direct_leaves(self, args, dev):
+        """(tensors in run order, which of them require a gradient, zero-gradient tensors,
+        whether a tensor appears more than once) when gradients of `args` can be delivered
+        directly, else None."""
+        if self._direct_runs() is False:
+            return None
+        ts = self._run_get(args)
+        zs = self._zero_get(args)
+        every = ts + zs
+        rg = list(map(_requires_grad, ts))
+        if not any(rg):
+            return None
+        if any(map(_grad_fn, every)) or any(map(_hooks, every)):
+            return None
+        if dev.type == 'cuda' and dev.index is None:
+            dev = torch.device('cuda', torch.cuda.current_device())
+        if set(map(_device, ts)) != {dev} or set(map(_dtype, ts)) != {torch.float32}:
+            return None
+        dup = len(set(map(id, every))) != len(every)
+        return ts, rg, [z for z in zs if z.requires_grad], dup
+
+    def deliver(self, block, layout, plan):
+        """Writes the gradients of a direct-delivery plan into the tensors' .grad."""
+        ts, rg, zeros, dup = plan
+        views = []
+        for kind, off, cnt, shape, m, _pos in self._runs:
+            o = layout[kind] + off
+            views.extend(block[o:o + m * cnt].view((m,) + shape).unbind(0))
+        fresh, acc_t, acc_g = [], [], []
+        for t, g, r in zip(ts, views, rg):
+            if r:
+                if t.grad is None:
+                    t.grad = g
+                else:
+                    acc_t.append(t.grad)
+                    acc_g.append(g)
+        if dup:
+            # a tensor used several times (a shared default transform) sums its slots:
+            # one add at a time, since a fused add must not see one tensor twice
+            for a, g in zip(acc_t, acc_g):
+                a.add_(g)
+        elif acc_t:
+            torch._foreach_add_(acc_t, acc_g)
+        for z in zeros:
+            if z.grad is None:
+                z.grad = torch.zeros_like(z)
 
     def acquire(self, dev_index):
         if self.idle:
@@ -480,19 +602,22 @@ def _templates():
     return [t for ts in _TEMPLATES.values() for t in ts]
 
 
-def _device_scene_for(args, dev, check=True, grad=False):
+def _device_scene_for(args, dev, check=True, grad=False, direct=False):
     """(flat, scene, arrays) for a TILED render of `args`: a pooled scene of the same
     topology rebuilt on the device when one exists, else a host build that starts a new
     pool.  `arrays` are the float arrays in the description's layout (differentiable
     with `grad`; None for a host build without `grad`).  With check=False the caller
-    reports the rebuild's findings (scene.status())."""
+    reports the rebuild's findings (scene.status()).  With `grad` and `direct`, a pooled
+    topology whose arguments allow it sets flat.direct to its direct-delivery plan and
+    returns arrays that are not differentiable."""
     dev_index = dev.index if dev.index is not None else -1
     bucket = _TEMPLATES.setdefault(len(args), [])
     for k, tpl in enumerate(bucket):
         if tpl.matches(args):
             if k:
                 bucket.insert(0, bucket.pop(k))
-            vals = tpl.device_params(args, dev, grad)
+            plan = tpl.direct_leaves(args, dev) if grad and direct else None
+            vals = tpl.device_params(args, dev, grad and plan is None)
             scene = tpl.acquire(dev_index)
             scene.update(float(args[-1]), _stream(), **vals)
             if check:
@@ -506,6 +631,7 @@ def _device_scene_for(args, dev, check=True, grad=False):
             f = _Flat()
             f.__dict__.update(tpl.flat.__dict__)
             f.eval_positions = args[6]
+            f.direct = None if plan is None else (tpl, plan)
             return f, _Lease(tpl, scene), vals
     flat = flatten_args(args)
     tpl = _SceneTemplate(args, flat)
@@ -596,10 +722,12 @@ class RenderFunction(torch.autograd.Function):
             elif isinstance(shape, Ellipse):
                 args += [diffvg.ShapeType.ellipse, shape.radius, shape.center]
             elif isinstance(shape, Path):
-                assert shape.points.shape[1] == 2
+                # size()/dim() rather than .shape: the loop runs once per shape of scenes of thousands
+                assert shape.points.size(1) == 2
                 ncp = shape.num_control_points
                 args += [diffvg.ShapeType.path, ncp if ncp.dtype == torch.int32 else ncp.to(torch.int32), shape.points]
-                if len(shape.stroke_width.shape) > 0 and shape.stroke_width.shape[0] > 1:
+                sw = shape.stroke_width
+                if sw.dim() > 0 and sw.size(0) > 1:
                     use_thickness = True
                     args.append(shape.stroke_width)
                 else:
@@ -650,7 +778,7 @@ class RenderFunction(torch.autograd.Function):
         dev = get_device()
         start = time.time()
         if _render_mode == _lib.MODE_TILED:
-            flat, scene, vals = _device_scene_for(args, dev, grad=grad)
+            flat, scene, vals = _device_scene_for(args, dev, grad=grad, direct=_direct_leaf_grads)
         else:
             # the comparator: a host-built scene per call, as the reference builds it
             flat = flatten_args(args)
@@ -685,8 +813,11 @@ class RenderFunction(torch.autograd.Function):
         if vals is None:
             vals = {k: None for k in _KINDS}
         meta = (flat, scene, width, height, num_samples_x, num_samples_y, seed)
+        # with direct delivery the node's only edge to the scene tensors is a zero-size
+        # leaf that makes the output require a gradient
+        anchor = torch.empty(0, requires_grad=True) if getattr(flat, 'direct', None) else None
         return _RenderNode.apply(meta, background_image, radius if isinstance(radius, torch.Tensor) else None,
-                                 *[vals[k] for k in _KINDS])
+                                 anchor, *[vals[k] for k in _KINDS])
 
     @staticmethod
     def render_grad(grad_img, width, height, num_samples_x, num_samples_y, seed, background_image, *args):
@@ -732,11 +863,13 @@ _KINDS = ('shape_params', 'points', 'thickness', 'stroke_width', 'color_params',
 
 class _RenderNode(torch.autograd.Function):
     """The render as one autograd node: inputs are the background image, the filter
-    radius and the scene's float arrays by kind (RenderFunction.apply builds them);
-    the backward returns slices of the scene's gradient block for them."""
+    radius, the anchor of a direct-delivery render (else None) and the scene's float
+    arrays by kind (RenderFunction.apply builds them); the backward returns slices of
+    the scene's gradient block for them, or, for a direct-delivery render, writes the
+    argument tensors' gradients itself."""
 
     @staticmethod
-    def forward(ctx, meta, background_image, filter_radius, *kinds):
+    def forward(ctx, meta, background_image, filter_radius, anchor, *kinds):
         flat, scene, width, height, num_samples_x, num_samples_y, seed = meta
         dev = get_device()
         eval_positions = flat.eval_positions
@@ -817,9 +950,12 @@ class _RenderNode(torch.autograd.Function):
         if ctx.radius_shape is not None and need[2]:
             o = layout['filter_radius']
             d_radius = block[o:o + 1].view(ctx.radius_shape)
+        direct = getattr(flat, 'direct', None)
+        if direct is not None:
+            direct[0].deliver(block, layout, direct[1])
         d_kinds = []
         for j, (name, shape) in enumerate(zip(_KINDS, ctx.kind_shapes)):
-            if shape is None or not need[3 + j]:
+            if shape is None or not need[4 + j]:
                 d_kinds.append(None)
                 continue
             o = layout[name]
@@ -831,4 +967,4 @@ class _RenderNode(torch.autograd.Function):
             scene.status()
             print('Backward pass, time: %.5f s' % (time.time() - start))
         _PENDING.append((scene, finite))
-        return (None, d_background_image, d_radius) + tuple(d_kinds)
+        return (None, d_background_image, d_radius, None) + tuple(d_kinds)
