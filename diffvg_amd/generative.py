@@ -11,12 +11,16 @@ reference's order (torch.randn_like for the point jitter, then random.randint fo
 seed, per item), so with equal RNG states the images are the reference's images.
 `force_cpu` is accepted and ignored: rendering stays on the GPU.
 """
+import ctypes
 import random
 
+import numpy as np
 import torch
 
-from .batch_render import render_batch
-from .render_pytorch import RenderFunction
+from . import _lib
+from .batch import Batch
+from .batch_render import _BatchCheck, render_batch
+from .render_pytorch import _PENDING, RenderFunction, _check_gpu, _stream, check_rebuild, flatten_args, synchronize
 from .shape import Path, ShapeGroup
 
 __all__ = ['render', 'opacityStroke2diffvg', 'stroke2diffvg', 'line_render', 'bezier_render']
@@ -140,13 +144,118 @@ def stroke2diffvg(strokes, canvas_size=128):
     return torch.stack(out, 0).permute(0, 3, 1, 2)[:, :3].contiguous()
 
 
+class _StrokePool:
+    """Device scenes of one stroke-batch topology (n open paths of the same segment kinds,
+    one constant stroke colour each, no fill, box filter of radius 0.5), created from the
+    first batch's values and rebuilt on the device from every later batch's values.  A
+    forward pass leases one set of scenes (one per batch item) until its backward has run
+    or its graph is freed, so two forward passes in flight never share a scene."""
+
+    MAX_IDLE = 4
+
+    def __init__(self, desc):
+        self.desc = desc
+        self.color_idx = torch.from_numpy(
+            np.asarray(desc['stroke_offset'], np.int64)[:, None] + np.arange(4, dtype=np.int64))
+        self.idle = []
+        self._dev_idx = {}
+
+    def color_index(self, dev):
+        t = self._dev_idx.get(dev)
+        if t is None:
+            t = self._dev_idx[dev] = self.color_idx.to(dev)
+        return t
+
+    def acquire(self, bs, dev_index):
+        if self.idle:
+            return self.idle.pop()
+        return [_lib.DeviceScene(self.desc, dev_index) for _ in range(bs)]
+
+    def release(self, scenes):
+        if len(self.idle) < self.MAX_IDLE:
+            self.idle.append(scenes)
+
+
+class _SetLease:
+    def __init__(self, pool, scenes):
+        self.pool, self.scenes = pool, scenes
+
+    def __del__(self):
+        try:
+            self.pool.release(self.scenes)
+        except Exception:
+            pass
+
+
+_POOLS = {}
+
+
+class _StrokeBatch(torch.autograd.Function):
+    """A batch of stroke scenes of one topology rendered from batched parameter arrays:
+    points [bs, 2 * num_points], stroke widths [bs, n], stroke colours [bs, n, 4].  Each
+    item's pooled scene is rebuilt on the device from its rows (dvg_scene_update), all
+    items render in one forward batch and one backward batch, and the backward returns
+    the gradients of the batched arrays as slices of the scenes' gradient blocks: the
+    autograd graph holds one node for the whole batch."""
+
+    @staticmethod
+    def forward(ctx, meta, points, widths, colors):
+        pool, seeds, size, samples = meta
+        dev = points.device
+        st = _stream()
+        bs = widths.shape[0]
+        lease = _SetLease(pool, pool.acquire(bs, dev.index if dev.index is not None else -1))
+        col = torch.zeros(bs, pool.desc['num_color_params'], device=dev)
+        col[:, pool.color_index(dev)] = colors.detach()
+        pts = points.detach().contiguous()
+        sw = widths.detach().contiguous()
+        for k, sc in enumerate(lease.scenes):
+            sc.update(pool.desc['filter_radius'], st, points=pts[k], stroke_width=sw[k], color_params=col[k])
+        for sc in lease.scenes:
+            # what Scene construction rejects (non-finite points) is reported now
+            check_rebuild(sc)
+        images = torch.zeros(bs, size, size, 4, device=dev)
+        fwd = Batch([(sc, dict(width=size, height=size, nsx=samples, nsy=samples, seed=seeds[k],
+                               render_image=images[k])) for k, sc in enumerate(lease.scenes)], zero_grads=False)
+        fwd.render(st)
+        ctx.meta, ctx.lease, ctx.fwd = meta, lease, fwd
+        return images
+
+    @staticmethod
+    def backward(ctx, grad_images):
+        pool, seeds, size, samples = ctx.meta
+        scenes = ctx.lease.scenes
+        ctx.fwd = None
+        dev = grad_images.device
+        st = _stream()
+        g = grad_images.to(torch.float32).contiguous()
+        bwd = Batch([(sc, dict(width=size, height=size, nsx=samples, nsy=samples, seed=seeds[k],
+                               d_render_image=g[k])) for k, sc in enumerate(scenes)], zero_grads=True)
+        bwd.render(st)
+        layout, total = scenes[0].grad_layout()
+        block = torch.empty(len(scenes), total, device=dev)
+        for k, sc in enumerate(scenes):
+            _lib.check(sc._lib.dvg_scene_copy_grads(sc.handle, ctypes.c_void_p(block[k].data_ptr()), st))
+        o = layout['points']
+        d_points = block[:, o:o + 2 * pool.desc['num_points_total']]
+        o = layout['stroke_width']
+        d_widths = block[:, o:o + pool.desc['num_shapes']]
+        o = layout['color_params']
+        d_colors = block[:, o:o + pool.desc['num_color_params']][:, pool.color_index(dev)]
+        # the batch's findings (and a non-finite grad_img) are read at the next render call
+        # or synchronize(); the lease stays with them until then
+        _PENDING.append((_BatchCheck(bwd, ctx.lease), g.isfinite().all()))
+        ctx.lease = None
+        return None, d_points, d_widths, d_colors
+
+
 def _segments_render(all_points, all_widths, all_alphas, canvas_size, colors, ncp_of):
     dev = all_points.device
     all_points = 0.5 * (all_points + 1.0) * canvas_size
     all_points = all_points + 1e-4 * torch.randn_like(all_points)
     bs, n_strokes = all_points.shape[0], all_points.shape[1]
     n_out = 3 if colors is not None else 1
-    scenes, items, seeds = [], [], []
+    scenes, seeds = [], []
     for k in range(bs):
         shapes, groups = [], []
         for p in range(n_strokes):
@@ -160,20 +269,38 @@ def _segments_render(all_points, all_widths, all_alphas, canvas_size, colors, nc
             groups.append(ShapeGroup(shape_ids=torch.tensor([len(shapes) - 1]), fill_color=None,
                                      stroke_color=color))
         scenes.append((canvas_size, canvas_size, shapes, groups))
-        items.append((shapes, groups))
         seeds.append(random.randint(0, 1000000))
-    rasters = _render_seeded(items, seeds, canvas_size, canvas_size, 2, dev)
-    output = torch.zeros(bs, n_out, canvas_size, canvas_size, device=dev)
-    for k, raster in enumerate(rasters):
-        raster = raster.permute(2, 0, 1).view(4, canvas_size, canvas_size)
-        alpha = raster[3:4]
-        if colors is not None:
-            image = raster[:3]
-            alpha = alpha.repeat(3, 1, 1)
-        else:
-            image = raster[:1]
-        output[k] = image * alpha
-    return output, scenes
+    if dev.type == 'cuda':
+        rasters = _render_strokes_batched(all_points, all_widths, all_alphas, colors, scenes, seeds, canvas_size)
+    else:
+        rasters = _render_seeded([(sh, gr) for _, _, sh, gr in scenes], seeds, canvas_size, canvas_size, 2, dev)
+        rasters = torch.stack(rasters, 0)
+    raster = rasters.permute(0, 3, 1, 2)
+    alpha = raster[:, 3:4]
+    if colors is not None:
+        output = raster[:, :3] * alpha.repeat(1, 3, 1, 1)
+    else:
+        output = raster[:, :1] * alpha
+    return output.contiguous(), scenes
+
+
+def _render_strokes_batched(all_points, all_widths, all_alphas, colors, scenes, seeds, size):
+    """The batch's images [bs, size, size, 4] through _StrokeBatch: the scenes share one
+    topology, so their parameters go to the device as three batched arrays."""
+    _check_gpu(read_findings=False)
+    synchronize()
+    bs, n = all_widths.shape
+    npts = all_points.shape[2]
+    w, h, shapes, groups = scenes[0]
+    key = (bs, n, npts, size, all_points.device.index, tuple(shapes[0].num_control_points.tolist()))
+    pool = _POOLS.get(key)
+    if pool is None:
+        flat = flatten_args(RenderFunction.serialize_scene(w, h, shapes, groups))
+        pool = _POOLS[key] = _StrokePool(flat.desc)
+    pts = all_points.reshape(bs, n * npts * 2).to(torch.float32)
+    rgb = colors if colors is not None else torch.ones(bs, n, 3, device=all_points.device)
+    col = torch.cat([rgb, all_alphas.unsqueeze(-1)], -1).to(torch.float32)
+    return _StrokeBatch.apply((pool, list(seeds), size, 2), pts, all_widths.to(torch.float32), col)
 
 
 def line_render(all_points, all_widths, all_alphas, force_cpu=True, canvas_size=32, colors=None):

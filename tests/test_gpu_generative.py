@@ -247,3 +247,47 @@ def test_generative_front_end_matches_oracle(kind):
     want = [want_pts, want_w, want_a] + ([] if colors is None else [want_c])
     for name, a, b in zip(('points', 'widths', 'alphas', 'colors'), got, want):
         assert _rel(a.cpu(), torch.from_numpy(b)) <= 1e-3, name
+
+
+def test_generative_forwards_in_flight_keep_their_scenes():
+    """Two bezier_render passes of one topology before either backward: each holds its own
+    set of pooled scenes, so each backward returns its own pass's gradients (equal to
+    those of the pass run alone); after both, the sets go back to the pool."""
+    print('generative in flight', flush=True)
+    bs, n = 3, 4
+    g = torch.Generator(device='cuda').manual_seed(9)
+    inputs = [(torch.rand(bs, n, 7, 2, device='cuda', generator=g) * 1.6 - 0.8,
+               torch.rand(bs, n, device='cuda', generator=g) + 0.5,
+               torch.rand(bs, n, device='cuda', generator=g) * 0.8 + 0.2) for _ in range(2)]
+
+    def forward(i):
+        pts, w, a = (t.clone().requires_grad_(True) for t in inputs[i])
+        torch.manual_seed(i)
+        random.seed(i)
+        out, _ = generative.bezier_render(pts, w, a, canvas_size=24)
+        return out, (pts, w, a)
+
+    alone = []
+    for i in range(2):
+        out, leaves = forward(i)
+        out.sum().backward()
+        alone.append((out.detach(), [t.grad.clone() for t in leaves]))
+    outs = [forward(i) for i in range(2)]
+    for i in (1, 0):
+        out, leaves = outs[i]
+        out.sum().backward()
+        assert torch.equal(out.detach(), alone[i][0])
+        for a, b in zip((t.grad for t in leaves), alone[i][1]):
+            assert _rel(a, b) <= 1e-5
+    del outs, out
+    pd.synchronize()
+    pools = [p for k, p in generative._POOLS.items() if k[:4] == (bs, n, 7, 24)]
+    assert len(pools) == 1 and len(pools[0].idle) == 2
+
+
+def test_generative_non_finite_points_raise():
+    pts = torch.zeros(2, 3, 2, 2, device='cuda')
+    pts[1, 2, 0, 0] = float('nan')
+    with pytest.raises(AssertionError):
+        generative.line_render(pts, torch.ones(2, 3, device='cuda'), torch.ones(2, 3, device='cuda'))
+    pd.synchronize()
