@@ -369,6 +369,24 @@ def leg_batched(steps, warmup, autotune=False, e2e_steps=4, plain_steps=1):
         torch.cuda.synchronize()
         t_batched = (time.perf_counter() - t0) / e2e_steps
         progress('batch: render_batch end to end %.1f ms' % (t_batched * 1e3))
+    t_direct = None
+    if e2e_steps:
+        # the same steps with the gradients written straight into the parameters' .grad
+        pydiffvg.set_direct_leaf_gradients(True)
+        try:
+            for _ in range(2):
+                batched_step()
+            pydiffvg.synchronize()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(e2e_steps):
+                batched_step()
+            pydiffvg.synchronize()
+            torch.cuda.synchronize()
+            t_direct = (time.perf_counter() - t0) / e2e_steps
+        finally:
+            pydiffvg.set_direct_leaf_gradients(False)
+        progress('batch: render_batch end to end, direct leaf gradients %.1f ms' % (t_direct * 1e3))
 
     # ---- plain: one RenderFunction call per scene (PLAIN schedule)
     pydiffvg.set_render_mode('plain')
@@ -396,7 +414,7 @@ def leg_batched(steps, warmup, autotune=False, e2e_steps=4, plain_steps=1):
         progress('batch: plain %.1f ms' % (t_plain * 1e3))
     pydiffvg.set_render_mode('tiled')
     return dict(samples=samples, scenes=n, paths=int(sum(len(it['shapes']) for it in items)), t_graph=t_graph,
-                t_batched=t_batched, t_plain=t_plain, kernels=kern, tuned=tuned, steps=steps,
+                t_batched=t_batched, t_direct=t_direct, t_plain=t_plain, kernels=kern, tuned=tuned, steps=steps,
                 e2e_steps=e2e_steps, plain_steps=plain_steps)
 
 
@@ -498,11 +516,13 @@ def leg_serving(n_req, hbm_budget_gib=64, max_batch=32, sweep=True, p99_bar_ms=5
                 hbm_budget_gib=hbm_budget_gib, max_batch=max_batch)
 
 
-def leg_single(cfg, steps, warmup, mode='tiled', prefilter_steps=0):
+def leg_single(cfg, steps, warmup, mode='tiled', prefilter_steps=0, direct_steps=0):
     """One 512x512 scene (2: 300 closed paths, Gaussian filter; 3: 2,048 strokes, box
     filter): a step is serialize_scene + RenderFunction.apply + L2 loss + backward with
     the gradients on the device.  Per-kernel times come from the library's HIP events
-    over the same number of steps run again after the timed ones."""
+    over the same number of steps run again after the timed ones.  `direct_steps` more
+    steps are timed with the gradients written straight into the parameters' .grad
+    (set_direct_leaf_gradients)."""
     import numpy as np
     import torch
     import diffvg_amd as pydiffvg
@@ -566,9 +586,96 @@ def leg_single(cfg, steps, warmup, mode='tiled', prefilter_steps=0):
         torch.cuda.synchronize()
         tp = (time.perf_counter() - tp0) / prefilter_steps
         pre = {'value': round(w * h * nsx * nsy / tp / 1e6, 4), 'ms_per_step': round(tp * 1e3, 4)}
+    direct = None
+    if direct_steps and mode == 'tiled':
+        pydiffvg.set_direct_leaf_gradients(True)
+        try:
+            for _ in range(2):
+                step()
+            pydiffvg.synchronize()
+            torch.cuda.synchronize()
+            td0 = time.perf_counter()
+            for _ in range(direct_steps):
+                step()
+            pydiffvg.synchronize()
+            torch.cuda.synchronize()
+            td = (time.perf_counter() - td0) / direct_steps
+        finally:
+            pydiffvg.set_direct_leaf_gradients(False)
+        direct = {'value': round(w * h * nsx * nsy / td / 1e6, 4), 'ms_per_step': round(td * 1e3, 4),
+                  'what': 'the same step with the gradients written straight into the parameters\' .grad '
+                          '(set_direct_leaf_gradients)'}
+        progress('single %s direct leaf gradients: %.2f ms per step' % (c['name'], td * 1e3))
     pydiffvg.set_render_mode('tiled')
     return dict(name=c['name'], w=w, h=h, nsx=nsx, nsy=nsy, paths=len(c['shapes']), samples=w * h * nsx * nsy,
-                sec=sec, kernels=kern, prefilter=pre, steps=steps, warmup=warmup, mode=mode)
+                sec=sec, kernels=kern, prefilter=pre, steps=steps, warmup=warmup, mode=mode, direct=direct)
+
+
+def leg_generative(steps=20, warmup=5, bs=32, n=16, size=32, loop_steps=5):
+    """The generative-model front end at the GAN app's defaults (16 two-segment cubic
+    strokes with colours, 32x32, 2x2 samples; batch 32): a step is bezier_render + L2 loss
+    + backward into the stroke tensors.  Beside it the reference's structure: one
+    RenderFunction call per batch item (rendering.py:262-290), on the same GPU."""
+    import random
+    import torch
+    import diffvg_amd as pydiffvg
+    from diffvg_amd import generative
+
+    dev = torch.device('cuda:0')
+    pydiffvg.set_use_gpu(True)
+    pydiffvg.set_device(dev)
+    g = torch.Generator(device=dev).manual_seed(3)
+    pts = (torch.rand(bs, n, 7, 2, device=dev, generator=g) * 1.6 - 0.8).requires_grad_(True)
+    widths = (torch.rand(bs, n, device=dev, generator=g) * 1.5 + 0.5).requires_grad_(True)
+    alphas = (torch.rand(bs, n, device=dev, generator=g) * 0.8 + 0.2).requires_grad_(True)
+    colors = torch.rand(bs, n, 3, device=dev, generator=g).requires_grad_(True)
+    target = torch.rand(bs, 3, size, size, device=dev, generator=g)
+    random.seed(0)
+
+    def batched():
+        out, _ = generative.bezier_render(pts, widths, alphas, canvas_size=size, colors=colors)
+        ((out - target) ** 2).sum().backward()
+
+    def loop():
+        cs = size
+        ap = 0.5 * (pts + 1.0) * cs
+        ap = ap + 1e-4 * torch.randn_like(ap)
+        outs = []
+        for k in range(bs):
+            shapes, groups = [], []
+            for p in range(n):
+                shapes.append(pydiffvg.Path(num_control_points=torch.zeros(2, dtype=torch.int32) + 2,
+                                            points=ap[k, p].contiguous(), stroke_width=widths[k, p],
+                                            is_closed=False))
+                groups.append(pydiffvg.ShapeGroup(shape_ids=torch.tensor([len(shapes) - 1]), fill_color=None,
+                                                  stroke_color=torch.cat([colors[k, p], alphas[k, p].view(1)])))
+            raster = generative.render(cs, cs, shapes, groups, samples=2).permute(2, 0, 1)
+            outs.append(raster[:3] * raster[3:4])
+        ((torch.stack(outs) - target) ** 2).sum().backward()
+
+    def timed(fn, k, w):
+        for _ in range(w):
+            fn()
+        pydiffvg.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        pydiffvg.synchronize()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    tb = timed(batched, steps, warmup)
+    tl = timed(loop, loop_steps, 2)
+    samples = bs * size * size * 4
+    progress('generative strokes: batched %.2f ms, per-item loop %.2f ms per step' % (tb * 1e3, tl * 1e3))
+    return {'workload': 'bezier_strokes_b%d_n%d_%d' % (bs, n, size), 'batch': bs, 'strokes': n,
+            'canvas': [size, size], 'spp': [2, 2],
+            'value': round(samples / tb / 1e6, 4), 'unit': 'Msamples/s', 'ms_per_step': round(tb * 1e3, 4),
+            'per_item_loop': {'ms_per_step': round(tl * 1e3, 4), 'value': round(samples / tl / 1e6, 4),
+                              'what': 'one RenderFunction fwd+bwd per batch item (the reference structure)'},
+            'speedup_vs_loop': round(tl / tb, 2),
+            'what': 'bezier_render (one autograd node, one forward and one backward batch) + L2 loss + backward'}
 
 
 def _roofline(cpu, kern, per_kernel, workload):
@@ -623,6 +730,11 @@ def _batched_fields(b):
                   'what': 'one RenderFunction fwd+bwd per scene on the PLAIN schedule (reference launch structure)'},
         'speedup_graph_vs_plain': round(b['t_plain'] / b['t_graph'], 2),
         'speedup_batched_vs_plain': round(b['t_plain'] / b['t_batched'], 2),
+        'batched_end_to_end_direct_leaf_grads': None if b.get('t_direct') is None else {
+            'ms_per_step': round(b['t_direct'] * 1e3, 3), 'value': round(b['samples'] / b['t_direct'] / 1e6, 4),
+            'speedup_vs_plain': round(b['t_plain'] / b['t_direct'], 2),
+            'what': 'the same step with set_direct_leaf_gradients(True): the backward writes each parameter '
+                    'tensor\'s .grad itself'},
         'kernels_ms_eager': {k: round(v['avg_ms'], 4) for k, v in b['kernels'].items()},
         'kernel_ms_per_step': round(sum(v['total_ms'] for v in b['kernels'].values()) / b['steps'], 4),
         'autotune': b['tuned'],
@@ -654,10 +766,14 @@ def _single_fields(t, p):
            'spp': [t['nsx'], t['nsy']]}
     if t['prefilter']:
         out['prefilter_on'] = t['prefilter']
+    if t.get('direct'):
+        out['direct_leaf_grads'] = t['direct']
     if p is not None:
         out['plain'] = {'value': round(p['samples'] / p['sec'] / 1e6, 4), 'ms_per_step': round(p['sec'] * 1e3, 4),
                         'kernel_ms_per_step': round(sum(v['total_ms'] for v in p['kernels'].values()) / p['steps'], 4)}
         out['speedup_vs_plain'] = round(p['sec'] / t['sec'], 2)
+        if t.get('direct'):
+            out['direct_leaf_grads']['speedup_vs_plain'] = round(p['sec'] * 1e3 / t['direct']['ms_per_step'], 2)
     return out
 
 
@@ -669,7 +785,7 @@ def run_headline(a):
     s = leg_serving(a.requests, a.hbm_budget, a.max_batch)
     singles = {}
     for cfg, (st, wu, pst) in ((2, (30, 10, 8)), (3, (12, 4, 0))):
-        t = leg_single(cfg, st, wu, 'tiled', prefilter_steps=pst)
+        t = leg_single(cfg, st, wu, 'tiled', prefilter_steps=pst, direct_steps=st)
         p = leg_single(cfg, 3, 1, 'plain')
         singles[t['name']] = _single_fields(t, p)
     out = {
@@ -685,6 +801,7 @@ def run_headline(a):
     out.update(_batched_fields(b))
     out['serving_stream'] = _serving_fields(s)
     out['single_scene'] = singles
+    out['generative_strokes'] = leg_generative()
     out['roofline'] = _roofline(cpu, b['kernels'], BATCH_KERNELS, 'uneven_scenes_256')
     out['cpu_baseline'] = _cpu_baseline(cpu)
     print(json.dumps(out))
@@ -707,6 +824,14 @@ def run_config(a):
         out.update(_batched_fields(b))
         out['roofline'] = _roofline(cpu, b['kernels'], BATCH_KERNELS, 'uneven_scenes_256')
         out['cpu_baseline'] = _cpu_baseline(cpu)
+    elif a.config == 'gen':
+        gst = a.steps if a.steps_given else 20
+        gw = a.warmup if a.warmup_given else 5
+        r = leg_generative(gst, gw)
+        out = dict(base, value=r['value'], steps=gst, warmup=gw, ms_per_step=r['ms_per_step'],
+                   config={'workload': r['workload'], 'batch': r['batch'], 'strokes': r['strokes'],
+                           'canvas': r['canvas'], 'spp': r['spp']})
+        out['generative_strokes'] = r
     elif a.config == 5:
         s = leg_serving(a.requests, a.hbm_budget, a.max_batch, sweep=not a.no_sweep)
         out = dict(base, value=round(s['samples'] / s['span_b'] / 1e6, 4), steps=s['n_req'], warmup=8,
@@ -715,7 +840,8 @@ def run_config(a):
         out['serving_stream'] = _serving_fields(s)
     else:
         cpu = None if a.no_cpu_baseline else run_cpu_baseline(a.config)
-        t = leg_single(a.config, a.steps, a.warmup, a.mode, prefilter_steps=8 if a.config == 2 else 0)
+        t = leg_single(a.config, a.steps, a.warmup, a.mode, prefilter_steps=8 if a.config == 2 else 0,
+                       direct_steps=a.steps)
         f = _single_fields(t, None)
         out = dict(base, value=f['value'], steps=a.steps, warmup=a.warmup, ms_per_step=f['ms_per_step'],
                    config={'workload': t['name'], 'canvas': [t['w'], t['h']], 'spp': [t['nsx'], t['nsy']],
@@ -763,9 +889,10 @@ def main():
                     help='timed steps of the line\'s workload (default 20 graph replays of the 256-scene batch; '
                          '50 steps of a single scene)')
     ap.add_argument('--warmup', type=int, default=None, help='untimed steps first (default 5; 20 for a single scene)')
-    ap.add_argument('--config', default='headline', choices=['headline', '2', '3', '4', '5'],
+    ap.add_argument('--config', default='headline', choices=['headline', '2', '3', '4', '5', 'gen'],
                     help='headline (default): the 256-scene batch with the serving stream and the single scenes '
-                         'as extra keys; 2/3: one 512x512 scene; 4: the batch alone; 5: the serving stream alone')
+                         'as extra keys; 2/3: one 512x512 scene; 4: the batch alone; 5: the serving stream alone; '
+                         'gen: the generative stroke batch')
     ap.add_argument('--mode', default='tiled', choices=['tiled', 'plain'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-baseline-only', action='store_true')
@@ -781,6 +908,7 @@ def main():
     ap.add_argument('--max-batch', type=int, default=32, help='requests per batch (serving)')
     a = ap.parse_args()
     single = a.config in ('2', '3')
+    a.steps_given, a.warmup_given = a.steps is not None, a.warmup is not None
     if a.steps is None:
         a.steps = 50 if single else 20
     if a.warmup is None:
@@ -798,7 +926,7 @@ def main():
     if a.config == 'headline':
         run_headline(a)
     else:
-        a.config = int(a.config)
+        a.config = a.config if a.config == 'gen' else int(a.config)
         run_config(a)
 
 

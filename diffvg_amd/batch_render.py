@@ -15,6 +15,7 @@ on the host after its first step.
 import torch
 
 from . import _lib
+from . import render_pytorch as _rp
 from .batch import Batch
 from .device import get_device
 from .render_pytorch import (_KINDS, OutputType, _PENDING, _Lease, _check_gpu, _device_scene_for, _stream,
@@ -30,7 +31,9 @@ class BatchRenderFunction(torch.autograd.Function):
 
     As RenderFunction.apply, the autograd node is taken over each scene's float arrays
     by kind and its filter radius, gathered from the argument tensors by differentiable
-    concatenation, not over every argument tensor."""
+    concatenation, not over every argument tensor; with set_direct_leaf_gradients(True)
+    a scene whose argument tensors allow it has its gradients written into their .grad
+    by the backward instead (the node then holds a zero-size anchor leaf)."""
 
     @classmethod
     def apply(cls, spec, *args):
@@ -43,7 +46,8 @@ class BatchRenderFunction(torch.autograd.Function):
             for (w, h, nsx, nsy, seed, n) in spec:
                 a = args[k:k + n]
                 k += n
-                flat, scene, vals = _device_scene_for(a, dev, check=False, grad=grad)
+                flat, scene, vals = _device_scene_for(a, dev, check=False, grad=grad,
+                                                      direct=_rp._direct_leaf_grads)
                 scenes.append(scene)
                 flats.append(flat)
                 if flat.output_type != OutputType.color or flat.eval_positions.shape[0] != 0:
@@ -55,6 +59,8 @@ class BatchRenderFunction(torch.autograd.Function):
                 inputs += [None] * len(_KINDS) if vals is None else [vals[kd] for kd in _KINDS]
             if k != len(args):
                 raise ValueError('spec does not account for every argument')
+            if any(getattr(f, 'direct', None) for f in flats):
+                inputs.append(torch.empty(0, requires_grad=True))
             for sc in scenes:
                 check_rebuild(sc)
             # findings of earlier backward passes, read once this call's scene work is on
@@ -116,9 +122,12 @@ class _BatchNode(torch.autograd.Function):
         out = [None] * len(ctx.shapes)
         for i, sc in enumerate(ctx.scenes):
             base = per * i
-            if not any(need[1 + base + j] for j in range(per)):
+            direct = getattr(ctx.flats[i], 'direct', None)
+            if direct is None and not any(need[1 + base + j] for j in range(per)):
                 continue
             block, layout = sc.grads_device(dev, st)
+            if direct is not None:
+                direct[0].deliver(block, layout, direct[1])
             for j, name in enumerate(('filter_radius',) + _KINDS):
                 shape = ctx.shapes[base + j]
                 if shape is None or not need[1 + base + j]:

@@ -16,6 +16,7 @@ Differences from the reference, all deliberate:
 * ``render_grad`` with a background image works (the reference reads an undefined
   variable there, render_pytorch.py:638).
 """
+import itertools
 import operator
 import time
 import warnings
@@ -521,19 +522,23 @@ class _SceneTemplate:
         if self._direct_runs() is False:
             return None
         ts = self._run_get(args)
-        zs = self._zero_get(args)
-        every = ts + zs
         rg = list(map(_requires_grad, ts))
-        if not any(rg):
+        sel = list(itertools.compress(ts, rg))
+        if not sel:
             return None
-        if any(map(_grad_fn, every)) or any(map(_hooks, every)):
+        zs = [z for z in self._zero_get(args) if z.requires_grad]
+        sel += zs
+        # only the tensors that receive a gradient matter: they must be leaves without
+        # hooks, float32 on the render device, and each slot's tensor used once (or the
+        # slots of a shared tensor are summed, below)
+        if any(map(_grad_fn, sel)) or any(map(_hooks, sel)):
             return None
         if dev.type == 'cuda' and dev.index is None:
             dev = torch.device('cuda', torch.cuda.current_device())
-        if set(map(_device, ts)) != {dev} or set(map(_dtype, ts)) != {torch.float32}:
+        if set(map(_device, sel)) != {dev} or set(map(_dtype, sel)) != {torch.float32}:
             return None
-        dup = len(set(map(id, every))) != len(every)
-        return ts, rg, [z for z in zs if z.requires_grad], dup
+        dup = len(set(map(id, sel))) != len(sel)
+        return ts, rg, zs, dup
 
     def deliver(self, block, layout, plan):
         """Writes the gradients of a direct-delivery plan into the tensors' .grad."""
@@ -715,57 +720,63 @@ class RenderFunction(torch.autograd.Function):
         num_shape_groups = len(shape_groups)
         args = [canvas_width, canvas_height, num_shapes, num_shape_groups, output_type, use_prefiltering,
                 eval_positions.to(get_device())]
+        # locals: the loops run once per shape and group of scenes of thousands
+        extend, append = args.extend, args.append
+        i32 = torch.int32
+        t_path, t_circle, t_ellipse = diffvg.ShapeType.path, diffvg.ShapeType.circle, diffvg.ShapeType.ellipse
+        t_rect = diffvg.ShapeType.rect
+        c_const = diffvg.ColorType.constant
+        Tensor = torch.Tensor
         for shape in shapes:
-            use_thickness = False
-            if isinstance(shape, Circle):
-                args += [diffvg.ShapeType.circle, shape.radius, shape.center]
-            elif isinstance(shape, Ellipse):
-                args += [diffvg.ShapeType.ellipse, shape.radius, shape.center]
-            elif isinstance(shape, Path):
-                # size()/dim() rather than .shape: the loop runs once per shape of scenes of thousands
-                assert shape.points.size(1) == 2
+            kind = type(shape)
+            if kind is Path or (kind is not Polygon and isinstance(shape, Path)):
+                points = shape.points
+                assert points.size(1) == 2
                 ncp = shape.num_control_points
-                args += [diffvg.ShapeType.path, ncp if ncp.dtype == torch.int32 else ncp.to(torch.int32), shape.points]
                 sw = shape.stroke_width
                 if sw.dim() > 0 and sw.size(0) > 1:
-                    use_thickness = True
-                    args.append(shape.stroke_width)
+                    # per-point thickness; the stroke width slot then holds zero
+                    extend((t_path, ncp if ncp.dtype == i32 else ncp.to(i32), points, sw, shape.is_closed,
+                            shape.use_distance_approx, torch.tensor(0.0)))
                 else:
-                    args.append(None)
-                args += [shape.is_closed, shape.use_distance_approx]
+                    extend((t_path, ncp if ncp.dtype == i32 else ncp.to(i32), points, None, shape.is_closed,
+                            shape.use_distance_approx, sw))
+            elif isinstance(shape, Circle):
+                extend((t_circle, shape.radius, shape.center, shape.stroke_width))
+            elif isinstance(shape, Ellipse):
+                extend((t_ellipse, shape.radius, shape.center, shape.stroke_width))
             elif isinstance(shape, Polygon):
                 assert shape.points.shape[1] == 2
                 n = shape.points.shape[0] if shape.is_closed else shape.points.shape[0] - 1
-                args += [diffvg.ShapeType.path, torch.zeros(n, dtype=torch.int32), shape.points, None,
-                         shape.is_closed, False]
+                extend((t_path, torch.zeros(n, dtype=i32), shape.points, None, shape.is_closed, False,
+                        shape.stroke_width))
             elif isinstance(shape, Rect):
-                args += [diffvg.ShapeType.rect, shape.p_min, shape.p_max]
+                extend((t_rect, shape.p_min, shape.p_max, shape.stroke_width))
             else:
                 raise TypeError('unsupported shape %r' % (type(shape),))
-            args.append(torch.tensor(0.0) if use_thickness else shape.stroke_width)
         for shape_group in shape_groups:
             ids = shape_group.shape_ids
-            args.append(ids if ids.dtype == torch.int32 else ids.to(torch.int32))
-            for kind, color in (('fill', shape_group.fill_color), ('stroke', shape_group.stroke_color)):
+            append(ids if ids.dtype == i32 else ids.to(i32))
+            fill = shape_group.fill_color
+            for color in (fill, shape_group.stroke_color):
                 if color is None:
-                    args.append(None)
-                elif isinstance(color, torch.Tensor):
-                    args += [diffvg.ColorType.constant, color]
+                    append(None)
+                elif isinstance(color, Tensor):
+                    extend((c_const, color))
                 elif isinstance(color, LinearGradient):
-                    args += [diffvg.ColorType.linear_gradient, color.begin, color.end, color.offsets,
-                             color.stop_colors]
+                    extend((diffvg.ColorType.linear_gradient, color.begin, color.end, color.offsets,
+                            color.stop_colors))
                 elif isinstance(color, RadialGradient):
-                    args += [diffvg.ColorType.radial_gradient, color.center, color.radius, color.offsets,
-                             color.stop_colors]
+                    extend((diffvg.ColorType.radial_gradient, color.center, color.radius, color.offsets,
+                            color.stop_colors))
                 else:
                     raise TypeError('unsupported paint %r' % (type(color),))
-                if kind == 'fill' and color is not None:
-                    for shape_id in shape_group.shape_ids.tolist():
-                        if isinstance(shapes[shape_id], Path) and not shapes[shape_id].is_closed:
-                            warnings.warn('Detected non-closed paths with fill color. This might causes '
-                                          'unexpected results.', Warning)
-            args.append(shape_group.use_even_odd_rule)
-            args.append(shape_group.shape_to_canvas.contiguous())
+            if fill is not None:
+                for shape_id in ids.tolist():
+                    if isinstance(shapes[shape_id], Path) and not shapes[shape_id].is_closed:
+                        warnings.warn('Detected non-closed paths with fill color. This might causes '
+                                      'unexpected results.', Warning)
+            extend((shape_group.use_even_odd_rule, shape_group.shape_to_canvas.contiguous()))
         args.append(filter.type)
         args.append(filter.radius)
         return args

@@ -89,3 +89,35 @@ def test_direct_leaf_gradients_fall_back_for_non_leaves():
     pd.synchronize()
     assert base.grad is not None and torch.isfinite(base.grad).all()
     assert float(base.grad.abs().max()) > 0
+
+
+def test_render_batch_direct_leaf_gradients_match_engine():
+    """render_batch over scenes of several topologies: direct delivery gives every
+    argument tensor the gradient the engine gives it."""
+    from diffvg_amd.batch_render import render_batch
+    print('render_batch direct leaf gradients', flush=True)
+    cases = ['strokes', 'gradients', 'transformed', 'overlap_eo', 'primitives']
+    items = [_to_cuda(scenes.CASES[c]()) for c in cases]
+    targets = [torch.rand(sc['canvas'][1], sc['canvas'][0], 4, device='cuda',
+                          generator=torch.Generator(device='cuda').manual_seed(i)) for i, sc in enumerate(items)]
+    leaves = [t for sc in items for t in _leaves(sc)]
+
+    def run(direct):
+        pd.set_direct_leaf_gradients(direct)
+        for t in leaves:
+            t.requires_grad_(True)
+            t.grad = None
+        lst = [(sc['canvas'][0], sc['canvas'][1], sc['shapes'], sc['groups']) for sc in items]
+        for _ in range(2):  # the first call builds the topologies on the host
+            imgs = render_batch(lst, 2, 2, list(range(len(items))))
+            sum(((im - t) ** 2).sum() for im, t in zip(imgs, targets)).backward()
+        pd.synchronize()
+        return [im.detach() for im in imgs], [t.grad.clone() for t in leaves]
+
+    ie, ge = run(False)
+    idr, gd = run(True)
+    for a, b in zip(ie, idr):
+        assert torch.equal(a, b)
+    for a, b in zip(gd, ge):
+        scale = max(float(b.abs().max()), 1e-6)
+        assert float((a - b).abs().max()) <= 1e-4 * scale
