@@ -874,7 +874,9 @@ def load_traffic(kernel, workload):
         return None
     for line in open(path):
         cells = [c.strip().strip('`') for c in line.strip().strip('|').split('|')]
-        if len(cells) >= 3 and cells[0] == kernel and cells[1] == workload:
+        # rows name the kernel as rocprof does (template arguments included); the first
+        # instantiation listed stands for the bench's kernel name
+        if len(cells) >= 3 and cells[0].split('<')[0] == kernel and cells[1] == workload:
             try:
                 return float(cells[-1])
             except ValueError:

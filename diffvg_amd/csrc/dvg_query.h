@@ -243,18 +243,6 @@ DVG_DEV float quintic_root(const Quintic &q, float lb, float ub) {
     return t;
 }
 
-// The radii of a segment's points: the stroke radius, or the per-point thickness.
-DVG_DEV void seg_radii(const DevScene &S, int s, const int4 &idx, float r, float &r0, float &r1, float &r2,
-                       float &r3) {
-    r0 = r1 = r2 = r3 = r;
-    if (S.has_thick[s] != 0) {
-        r0 = S.thickness[idx.x];
-        r1 = S.thickness[idx.y];
-        r2 = S.thickness[idx.z];
-        r3 = S.thickness[idx.w];
-    }
-}
-
 // Every point of cubic segment `seg` is certainly farther than r from pt (the chords of an
 // 8-step polyline, the chords' deviation bound and a rounding margin, from the segment's
 // prep); false for r = inf.
@@ -272,10 +260,9 @@ DVG_DEV bool cubic_farther_than(const DevScene &S, int seg, V2 p0, V2 pt, float 
     return wf::cubic_certainly_farther_prep(poly, md.x, md.y, p0.x, p0.y, pt.x, pt.y, r);
 }
 
-// One stroke segment test of within_distance (within_distance.h:32-276), up to the
-// cubic's quintic search.  Returns 1 (hit), 0 (miss), -1 (quadratic distance-approximation
-// miss, which ends the whole path query in the reference) or 2 (a cubic the endpoint
-// and distance-bound tests leave open: cubic_within_quintic decides it).
+// One stroke segment test of within_distance (within_distance.h:32-276).
+// Returns 1 (hit), 0 (miss) or -1 (quadratic distance-approximation miss, which ends
+// the whole path query in the reference).
 DVG_DEV int segment_within(const DevScene &S, int s, int seg, V2 pt, float r) {
     V2 p[4];
     int4 idx;
@@ -323,26 +310,32 @@ DVG_DEV int segment_within(const DevScene &S, int s, int seg, V2 pt, float r) {
         return 0;
     }
     V2 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3];
-    float r0, r1, r2, r3;
-    seg_radii(S, s, idx, r, r0, r1, r2, r3);
+    float r0 = r, r1 = r, r2 = r, r3 = r;
+    if (thick) {
+        r0 = S.thickness[idx.x];
+        r1 = S.thickness[idx.y];
+        r2 = S.thickness[idx.z];
+        r3 = S.thickness[idx.w];
+    }
     if (dist_sq(p0, pt) < r0 * r0) return 1;  // eval(0) == p0 exactly
     if (dist_sq(ceval(p0, p1, p2, p3, 1), pt) < r3 * r3) return 1;
     // every curve point certainly farther than the largest radius magnitude (the radius
-    // at t is a convex combination of r0..r3): the quintic search would find nothing
-    if (cubic_farther_than(S, seg, p0, pt, fmaxf(fmaxf(fabsf(r0), fabsf(r1)), fmaxf(fabsf(r2), fabsf(r3)))))
-        return 0;
-    return 2;
-}
-
-// The quintic search of a cubic stroke segment (within_distance.h:184-272): a stationary
-// point of the squared distance within [0, 1] closer than the interpolated radius.
-DVG_DEV bool cubic_within_quintic(const DevScene &S, int s, int seg, V2 pt, float r) {
-    V2 p[4];
-    int4 idx;
-    seg_load(S, seg, p, idx);
-    V2 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3];
-    float r0, r1, r2, r3;
-    seg_radii(S, s, idx, r, r0, r1, r2, r3);
+    // at t is a convex combination of r0..r3): the quintic search below would find nothing
+    {
+        const float4 *q = S.seg_wf + kSegPrep * (size_t)seg + 3;
+        float poly[16];
+        for (int i = 0; i < 4; i++) {
+            const float4 v = q[i];
+            poly[4 * i] = v.x;
+            poly[4 * i + 1] = v.y;
+            poly[4 * i + 2] = v.z;
+            poly[4 * i + 3] = v.w;
+        }
+        const float4 md = q[4];
+        if (wf::cubic_certainly_farther_prep(poly, md.x, md.y, p0.x, p0.y, pt.x, pt.y,
+                                             fmaxf(fmaxf(fabsf(r0), fabsf(r1)), fmaxf(fabsf(r2), fabsf(r3)))))
+            return 0;
+    }
     Quintic q = cubic_quintic(p0, p1, p2, p3, pt);
     float iv[4];
     int niv = quintic_intervals(q, iv);
@@ -361,39 +354,22 @@ DVG_DEV bool cubic_within_quintic(const DevScene &S, int s, int seg, V2 pt, floa
         float t = quintic_root(q, lb, ub);
         float tt = 1 - t;
         float rr = (tt * tt * tt) * r0 + (3 * tt * tt * t) * r1 + (3 * tt * t * t) * r2 + (t * t * t) * r3;
-        if (dist_sq(ceval(p0, p1, p2, p3, t), pt) < rr * rr) return true;
+        if (dist_sq(ceval(p0, p1, p2, p3, t), pt) < rr * rr) return 1;
         if (upper >= 1.f) break;
         lower = upper;
     }
-    return false;
+    return 0;
 }
 
-// within_distance for a path: its segments in the reference's visit order, the first hit
-// (or approximation miss) ending the query.  Each lane scans ahead with the cheap tests
-// to the next segment that needs the quintic search and leaves the scan there; the lanes
-// of a wave that stopped then run their searches together, and the scan resumes.  Every
-// lane still takes its segments in order (the same tests, the same answer), but the
-// quintic code runs once per round for all lanes that need it rather than once for each
-// segment index at which some lane needs it.
 DVG_DEV bool within_distance_path(const DevScene &S, int s, V2 pt, float r) {
     int b = S.seg_begin[s], n = S.nsegs[s];
-    int i = 0;
-    while (true) {
-        int need = -1;
-        for (; i < n; i++) {
-            int seg = S.seg_dfs[b + i];
-            if (n > 1 && !in_box(S.seg_box[seg], pt, S.seg_maxr[seg])) continue;
-            int res = segment_within(S, s, seg, pt, r);
-            if (res == 2) {
-                need = seg;
-                i++;
-                break;
-            }
-            if (res != 0) return res > 0;
-        }
-        if (need < 0) return false;
-        if (cubic_within_quintic(S, s, need, pt, r)) return true;
+    for (int i = 0; i < n; i++) {
+        int seg = S.seg_dfs[b + i];
+        if (n > 1 && !in_box(S.seg_box[seg], pt, S.seg_maxr[seg])) continue;
+        int res = segment_within(S, s, seg, pt, r);
+        if (res != 0) return res > 0;
     }
+    return false;
 }
 
 DVG_DEV bool within_distance_shape(const DevScene &S, int s, V2 pt, float r) {
@@ -540,49 +516,31 @@ DVG_DEV float cubic_closest(const DevScene &S, int seg, V2 pt, V2 &cp, float &t_
 
 // closest_point for a path (compute_distance.h:27-300): the segments in the reference's
 // visit order, the first strict minimum kept.  A cubic whose every point is certainly
-// farther than the current minimum cannot lower it and is skipped (the test is checked
-// against the oracle's closest point on the host).  Each lane scans with the cheap tests
-// to its next cubic that needs the quintic search and leaves the scan there; the lanes
-// of a wave that stopped run their searches together, and the scan resumes: every lane
-// still takes its segments in order, so minima and ties are the reference's.
+// farther than the current minimum cannot lower it and is skipped before its quintic
+// search (the bound is checked against the oracle's closest point on the host); the
+// minima and ties are the reference's.
 DVG_DEV bool closest_point_path(const DevScene &S, int s, V2 pt, float max_radius, PathInfo &info, V2 &result) {
     int b = S.seg_begin[s], n = S.nsegs[s];
     float min_dist = max_radius;
     V2 ret{0, 0};
     bool found = false;
-    int i = 0;
-    while (true) {
-        int need = -1;
-        for (; i < n; i++) {
-            int seg = S.seg_dfs[b + i];
-            if (n > 1 && !in_box(S.seg_box[seg], pt, min_dist)) continue;
-            if (S.seg_kind[seg] == 2) {
-                if (cubic_farther_than(S, seg, ld_pt(S, S.seg_idx[seg].x), pt, min_dist)) continue;
-                need = seg;
-                i++;
-                break;
-            }
-            V2 cp;
-            float tr;
-            float dd = segment_closest_lq(S, s, seg, pt, cp, tr);
-            if (dd < min_dist) {
-                min_dist = dd;
-                ret = cp;
-                info.base_point_id = seg - b;
-                info.point_id = S.seg_pid[seg];
-                info.t_root = tr;
-                found = true;
-            }
-        }
-        if (need < 0) break;
+    for (int i = 0; i < n; i++) {
+        int seg = S.seg_dfs[b + i];
+        if (n > 1 && !in_box(S.seg_box[seg], pt, min_dist)) continue;
         V2 cp;
         float tr;
-        float dd = cubic_closest(S, need, pt, cp, tr);
+        float dd;
+        if (S.seg_kind[seg] == 2) {
+            if (cubic_farther_than(S, seg, ld_pt(S, S.seg_idx[seg].x), pt, min_dist)) continue;
+            dd = cubic_closest(S, seg, pt, cp, tr);
+        } else {
+            dd = segment_closest_lq(S, s, seg, pt, cp, tr);
+        }
         if (dd < min_dist) {
             min_dist = dd;
             ret = cp;
-            info.base_point_id = need - b;
-            info.point_id = S.seg_pid[need];
+            info.base_point_id = seg - b;
+            info.point_id = S.seg_pid[seg];
             info.t_root = tr;
             found = true;
         }

@@ -139,3 +139,26 @@ def test_direct_delivery_not_for_multi_element_stroke_widths():
     args = pd.RenderFunction.serialize_scene(w, h, sc['shapes'], sc['groups'])
     tpl = rp._SceneTemplate(args, rp.flatten_args(args))
     assert tpl.direct_leaves(args, torch.device('cpu')) is None
+
+
+def test_direct_delivery_rechecks_after_an_accepted_call():
+    """The second call with the same tensors reuses the first call's checks except for
+    hooks; other tensors or changed gradient flags are checked in full."""
+    sc, args, tpl, leaves, layout, block, sizes = _setup('strokes')
+    cpu = torch.device('cpu')
+    assert tpl.direct_leaves(args, cpu) is not None
+    assert tpl.direct_leaves(args, cpu) is not None
+    p = sc['shapes'][0].points
+    h = p.register_hook(lambda g: g)
+    assert tpl.direct_leaves(args, cpu) is None
+    h.remove()
+    i = next(k for k, a in enumerate(args) if a is p)
+    args2 = list(args)
+    args2[i] = p * 1.0
+    assert tpl.direct_leaves(args2, cpu) is None
+    q = sc['shapes'][1].points
+    q.requires_grad_(False)
+    plan = tpl.direct_leaves(args, cpu)
+    assert plan is not None
+    tpl.deliver(block.clone(), layout, plan)
+    assert q.grad is None and p.grad is not None
