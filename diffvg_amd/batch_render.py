@@ -23,6 +23,8 @@ from .render_pytorch import (_KINDS, OutputType, _PENDING, _Lease, _check_gpu, _
 
 __all__ = ['BatchRenderFunction', 'render_batch']
 
+_HINTS = []
+
 
 class BatchRenderFunction(torch.autograd.Function):
     """apply(spec, *args): spec is a tuple of (width, height, num_samples_x,
@@ -46,8 +48,9 @@ class BatchRenderFunction(torch.autograd.Function):
             for (w, h, nsx, nsy, seed, n) in spec:
                 a = args[k:k + n]
                 k += n
+                hint = _HINTS[len(scenes)] if len(scenes) < len(_HINTS) else None
                 flat, scene, vals = _device_scene_for(a, dev, check=False, grad=grad,
-                                                      direct=_rp._direct_leaf_grads)
+                                                      direct=_rp._direct_leaf_grads, hint=hint)
                 scenes.append(scene)
                 flats.append(flat)
                 if flat.output_type != OutputType.color or flat.eval_positions.shape[0] != 0:
@@ -59,6 +62,9 @@ class BatchRenderFunction(torch.autograd.Function):
                 inputs += [None] * len(_KINDS) if vals is None else [vals[kd] for kd in _KINDS]
             if k != len(args):
                 raise ValueError('spec does not account for every argument')
+            # the template each position used, tried first by the next batch (training
+            # loops render the same scenes in the same order)
+            _HINTS[:] = [getattr(sc, '_template', None) for sc in scenes]
             if any(getattr(f, 'direct', None) for f in flats):
                 inputs.append(torch.empty(0, requires_grad=True))
             for sc in scenes:

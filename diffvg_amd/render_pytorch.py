@@ -354,12 +354,17 @@ def _picker(pos):
 
 
 _size = torch.Tensor.size
+_unflatten = torch._C._nn.unflatten_dense_tensors
 _version = operator.attrgetter('_version')
 _requires_grad = operator.attrgetter('requires_grad')
 _grad_fn = operator.attrgetter('grad_fn')
 _hooks = operator.attrgetter('_backward_hooks')
 _device = operator.attrgetter('device')
 _dtype = operator.attrgetter('dtype')
+
+
+class _Resized(Exception):
+    """A template's argument tensor was resized in place since its last match."""
 
 
 class _SceneTemplate:
@@ -411,6 +416,12 @@ class _SceneTemplate:
         self._dev_tables = {}
         self._runs = None
         self._direct_last = None
+        # the float argument tensors of the last match and the element count of each
+        # kind's gather: the same tensor objects need no shape comparison, and a gather
+        # whose count differs (a tensor resized in place since) is caught in device_params
+        self._float_seen = None
+        self.kind_numel = {kind: sum(int(t.numel()) for t in self._get_kind[kind](args))
+                           for kind in self._get_kind}
 
     def matches(self, args):
         if len(args) != self.nargs:
@@ -418,8 +429,12 @@ class _SceneTemplate:
         try:
             if self._get_scalars(args) != self.scalar_vals:
                 return False
-            if list(map(_size, self._get_floats(args))) != self.float_shapes:
-                return False
+            floats = self._get_floats(args)
+            seen = self._float_seen
+            if seen is None or not all(map(operator.is_, floats, seen)):
+                if list(map(_size, floats)) != self.float_shapes:
+                    return False
+                self._float_seen = floats
         except (RuntimeError, TypeError, AttributeError):
             return False
         cur = self._get_ints(args)
@@ -475,6 +490,8 @@ class _SceneTemplate:
                             raise
                         # tensors of one kind on several devices: move each first
                         srcs = [t.to(dev) for t in srcs]
+                if out.numel() != self.kind_numel[kind]:
+                    raise _Resized()
                 return out.to(device=dev, dtype=torch.float32).contiguous()
             params = cat('params')
             points = cat('points')
@@ -514,6 +531,23 @@ class _SceneTemplate:
             self._zero_pos = sorted(set(self.float_pos) - set(run_pos))
             self._zero_get = _picker(self._zero_pos)
             self._runs = runs
+            # how deliver cuts the views: a long run with one view + unbind; consecutive
+            # short runs of one kind (paths of differing point counts) with one
+            # unflatten_dense_tensors call over their slots.  (kind, block offset, count of
+            # floats, first and last-plus-one index into the run order, run or None)
+            ops, i = [], 0
+            for r in runs:
+                kind, off, cnt, shape, m = r[0], r[1], r[2], r[3], r[4]
+                last = ops[-1] if ops else None
+                if m >= 8:
+                    ops.append([kind, off, m * cnt, i, i + m, r])
+                elif last and last[5] is None and last[0] == kind and last[1] + last[2] == off:
+                    last[2] += m * cnt
+                    last[4] = i + m
+                else:
+                    ops.append([kind, off, m * cnt, i, i + m, None])
+                i += m
+            self._ops = ops
         return self._runs
 
     def direct_leaves(self, args, dev):
@@ -554,18 +588,24 @@ class _SceneTemplate:
     def deliver(self, block, layout, plan):
         """Writes the gradients of a direct-delivery plan into the tensors' .grad."""
         ts, rg, zeros, dup = plan
-        views = []
-        for kind, off, cnt, shape, m, _pos in self._runs:
+        acc_t, acc_g = [], []
+        for kind, off, n, i0, i1, run in self._ops:
+            sel = rg[i0:i1]
+            if not any(sel):
+                continue
             o = layout[kind] + off
-            views.extend(block[o:o + m * cnt].view((m,) + shape).unbind(0))
-        fresh, acc_t, acc_g = [], [], []
-        for t, g, r in zip(ts, views, rg):
-            if r:
-                if t.grad is None:
-                    t.grad = g
-                else:
-                    acc_t.append(t.grad)
-                    acc_g.append(g)
+            part = ts[i0:i1]
+            if run is not None:
+                views = block[o:o + n].view((run[4],) + run[3]).unbind(0)
+            else:
+                views = _unflatten(block[o:o + n], part)
+            for t, g, r in zip(part, views, sel):
+                if r:
+                    if t.grad is None:
+                        t.grad = g
+                    else:
+                        acc_t.append(t.grad)
+                        acc_g.append(g)
         if dup:
             # a tensor used several times (a shared default transform) sums its slots:
             # one add at a time, since a fused add must not see one tensor twice
@@ -618,22 +658,35 @@ def _templates():
     return [t for ts in _TEMPLATES.values() for t in ts]
 
 
-def _device_scene_for(args, dev, check=True, grad=False, direct=False):
+def _device_scene_for(args, dev, check=True, grad=False, direct=False, hint=None):
     """(flat, scene, arrays) for a TILED render of `args`: a pooled scene of the same
     topology rebuilt on the device when one exists, else a host build that starts a new
     pool.  `arrays` are the float arrays in the description's layout (differentiable
     with `grad`; None for a host build without `grad`).  With check=False the caller
     reports the rebuild's findings (scene.status()).  With `grad` and `direct`, a pooled
     topology whose arguments allow it sets flat.direct to its direct-delivery plan and
-    returns arrays that are not differentiable."""
+    returns arrays that are not differentiable.  `hint`: a template to try first (the
+    one the caller's previous call with these arguments used)."""
     dev_index = dev.index if dev.index is not None else -1
     bucket = _TEMPLATES.setdefault(len(args), [])
-    for k, tpl in enumerate(bucket):
+    order = enumerate(bucket)
+    if hint is not None and hint.nargs == len(args):
+        k0 = next((k for k, t in enumerate(bucket) if t is hint), None)
+        if k0 is not None:
+            order = itertools.chain([(k0, hint)], ((k, t) for k, t in enumerate(bucket) if k != k0))
+    for k, tpl in order:
         if tpl.matches(args):
+            plan = tpl.direct_leaves(args, dev) if grad and direct else None
+            try:
+                vals = tpl.device_params(args, dev, grad and plan is None)
+            except _Resized:
+                # compare the shapes after all; a template they no longer match is passed
+                tpl._float_seen = None
+                if not tpl.matches(args):
+                    continue
+                vals = tpl.device_params(args, dev, grad and plan is None)
             if k:
                 bucket.insert(0, bucket.pop(k))
-            plan = tpl.direct_leaves(args, dev) if grad and direct else None
-            vals = tpl.device_params(args, dev, grad and plan is None)
             scene = tpl.acquire(dev_index)
             scene.update(float(args[-1]), _stream(), **vals)
             if check:
