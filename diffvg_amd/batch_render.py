@@ -18,8 +18,8 @@ from . import _lib
 from . import render_pytorch as _rp
 from .batch import Batch
 from .device import get_device
-from .render_pytorch import (_KINDS, OutputType, _PENDING, _Lease, _check_gpu, _device_scene_for, _stream,
-                             check_rebuild, synchronize)
+from .render_pytorch import (_KINDS, OutputType, _PENDING, _Lease, _Resized, _batch_params, _check_gpu,
+                             _device_scene_for, _find_template, _pooled_scene, _stream, check_rebuild, synchronize)
 
 __all__ = ['BatchRenderFunction', 'render_batch']
 
@@ -43,14 +43,36 @@ class BatchRenderFunction(torch.autograd.Function):
         dev = get_device()
         grad = torch.is_grad_enabled()
         scenes, flats, inputs = [], [], []
-        k = 0
+        direct = _rp._direct_leaf_grads
+        # the argument list of each scene, and the pooled topology it matches (if any)
+        arg_lists, k = [], 0
+        for (w, h, nsx, nsy, seed, n) in spec:
+            arg_lists.append(args[k:k + n])
+            k += n
+        tpls = [_find_template(a, _HINTS[i] if i < len(_HINTS) else None) for i, a in enumerate(arg_lists)]
+        plans = [tp.direct_leaves(a, dev) if (tp is not None and grad and direct) else None
+                 for tp, a in zip(tpls, arg_lists)]
+        pooled = [i for i, tp in enumerate(tpls) if tp is not None]
+        gathered = {}
+        if len(pooled) > 1:
+            # one gather per kind over all pooled scenes instead of one per scene
+            try:
+                need = grad and any(plans[i] is None for i in pooled)
+                for i, vals in zip(pooled, _batch_params([tpls[i] for i in pooled],
+                                                         [arg_lists[i] for i in pooled], dev, need)):
+                    if plans[i] is not None:  # delivered directly: no edge into the graph
+                        vals = {kd: None if v is None else v.detach() for kd, v in vals.items()}
+                    gathered[i] = vals
+            except _Resized:
+                gathered = {}
         try:
-            for (w, h, nsx, nsy, seed, n) in spec:
-                a = args[k:k + n]
-                k += n
-                hint = _HINTS[len(scenes)] if len(scenes) < len(_HINTS) else None
-                flat, scene, vals = _device_scene_for(a, dev, check=False, grad=grad,
-                                                      direct=_rp._direct_leaf_grads, hint=hint)
+            for i, (w, h, nsx, nsy, seed, n) in enumerate(spec):
+                a = arg_lists[i]
+                if i in gathered:
+                    flat, scene, vals = _pooled_scene(tpls[i], a, dev, gathered[i], plans[i], False)
+                else:
+                    flat, scene, vals = _device_scene_for(a, dev, check=False, grad=grad, direct=direct,
+                                                          hint=tpls[i])
                 scenes.append(scene)
                 flats.append(flat)
                 if flat.output_type != OutputType.color or flat.eval_positions.shape[0] != 0:

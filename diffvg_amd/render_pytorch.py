@@ -355,6 +355,7 @@ def _picker(pos):
 
 _size = torch.Tensor.size
 _unflatten = torch._C._nn.unflatten_dense_tensors
+_flatten = torch._C._nn.flatten_dense_tensors
 _version = operator.attrgetter('_version')
 _requires_grad = operator.attrgetter('requires_grad')
 _grad_fn = operator.attrgetter('grad_fn')
@@ -483,7 +484,7 @@ class _SceneTemplate:
                         elif mode == 'cat_flat':
                             out = torch.cat(srcs).reshape(-1)
                         else:
-                            out = torch.cat([t.reshape(-1) for t in srcs])
+                            out = _flatten(srcs)
                         break
                     except RuntimeError:
                         if attempt:
@@ -658,6 +659,58 @@ def _templates():
     return [t for ts in _TEMPLATES.values() for t in ts]
 
 
+def _find_template(args, hint=None, skip=()):
+    """The pooled topology `args` match (moved to the front of its bucket), or None.
+    `hint`: a template to try first (the one the caller's previous call with these
+    arguments used); `skip`: templates not to try."""
+    bucket = _TEMPLATES.get(len(args))
+    if not bucket:
+        return None
+    order = enumerate(bucket)
+    if hint is not None and hint.nargs == len(args):
+        k0 = next((k for k, t in enumerate(bucket) if t is hint), None)
+        if k0 is not None:
+            order = itertools.chain([(k0, hint)], ((k, t) for k, t in enumerate(bucket) if k != k0))
+    for k, tpl in order:
+        if tpl not in skip and tpl.matches(args):
+            if k:
+                bucket.insert(0, bucket.pop(k))
+            return tpl
+    return None
+
+
+def _pooled_scene(tpl, args, dev, vals, plan, check):
+    """(flat, lease, arrays) for a pooled scene of `tpl` rebuilt on the device from `vals`."""
+    scene = tpl.acquire(dev.index if dev.index is not None else -1)
+    scene.update(float(args[-1]), _stream(), **vals)
+    if check:
+        # what Scene construction rejects is reported now, as the reference reports it
+        # (one wait for the rebuild kernels, before the render is enqueued)
+        try:
+            check_rebuild(scene)
+        except Exception:
+            tpl.release(scene)
+            raise
+    f = _Flat()
+    f.__dict__.update(tpl.flat.__dict__)
+    f.eval_positions = args[6]
+    f.direct = None if plan is None else (tpl, plan)
+    return f, _Lease(tpl, scene), vals
+
+
+def _host_built_scene(args, dev, grad):
+    """(flat, lease, arrays) for a topology seen for the first time: built on the host,
+    starting a new pool."""
+    flat = flatten_args(args)
+    tpl = _SceneTemplate(args, flat)
+    if sum(len(ts) for ts in _TEMPLATES.values()) >= _MAX_TEMPLATES:
+        _TEMPLATES.clear()
+    _TEMPLATES.setdefault(len(args), []).insert(0, tpl)
+    scene = _lib.DeviceScene(flat.desc, dev.index if dev.index is not None else -1, _lib.MODE_TILED)
+    vals = tpl.device_params(args, dev, True) if grad else None
+    return flat, _Lease(tpl, scene), vals
+
+
 def _device_scene_for(args, dev, check=True, grad=False, direct=False, hint=None):
     """(flat, scene, arrays) for a TILED render of `args`: a pooled scene of the same
     topology rebuilt on the device when one exists, else a host build that starts a new
@@ -665,52 +718,109 @@ def _device_scene_for(args, dev, check=True, grad=False, direct=False, hint=None
     with `grad`; None for a host build without `grad`).  With check=False the caller
     reports the rebuild's findings (scene.status()).  With `grad` and `direct`, a pooled
     topology whose arguments allow it sets flat.direct to its direct-delivery plan and
-    returns arrays that are not differentiable.  `hint`: a template to try first (the
-    one the caller's previous call with these arguments used)."""
-    dev_index = dev.index if dev.index is not None else -1
-    bucket = _TEMPLATES.setdefault(len(args), [])
-    order = enumerate(bucket)
-    if hint is not None and hint.nargs == len(args):
-        k0 = next((k for k, t in enumerate(bucket) if t is hint), None)
-        if k0 is not None:
-            order = itertools.chain([(k0, hint)], ((k, t) for k, t in enumerate(bucket) if k != k0))
-    for k, tpl in order:
-        if tpl.matches(args):
-            plan = tpl.direct_leaves(args, dev) if grad and direct else None
+    returns arrays that are not differentiable.  `hint`: a template to try first."""
+    skip = []
+    while True:
+        tpl = _find_template(args, hint, skip)
+        if tpl is None:
+            return _host_built_scene(args, dev, grad)
+        plan = tpl.direct_leaves(args, dev) if grad and direct else None
+        try:
+            vals = tpl.device_params(args, dev, grad and plan is None)
+        except _Resized:
+            # compare the shapes after all; a template they no longer match is passed
+            tpl._float_seen = None
+            if not tpl.matches(args):
+                skip.append(tpl)
+                continue
+            vals = tpl.device_params(args, dev, grad and plan is None)
+        return _pooled_scene(tpl, args, dev, vals, plan, check)
+
+
+_GATHER_KINDS = (('params', 'shape_params'), ('points', 'points'), ('color', 'color_params'),
+                 ('xform', 'shape_to_canvas'))
+_BATCH_TABLES = {}
+
+
+def _batch_tables(tpls, dev):
+    """Index tables of a batch of templates (cached per sequence of templates): element
+    offsets of each scene in every batch-wide gather, and the stroke-width and thickness
+    index maps of the whole batch."""
+    key = (tuple(map(id, tpls)), dev)
+    t = _BATCH_TABLES.get(key)
+    if t is not None and all(a is b for a, b in zip(t['tpls'], tpls)):
+        return t
+    offs = {}
+    for kind in ('params', 'points', 'thick', 'sw', 'color', 'xform'):
+        offs[kind] = np.concatenate([[0], np.cumsum([tp.kind_numel[kind] for tp in tpls])]).astype(np.int64)
+    # stroke widths: per scene, shape s takes its stroke-width argument's first element
+    sw_first, sw_zero, ns_off = [], [], [0]
+    pt_off = [0]
+    for i, tp in enumerate(tpls):
+        sw_first.append(tp.flat.sw_first + offs['sw'][i])
+        sw_zero.append(tp.flat.sw_zero)
+        ns_off.append(ns_off[-1] + len(tp.flat.sw_first))
+        pt_off.append(pt_off[-1] + tp.flat.desc['num_points_total'])
+    thick_idx = [np.arange(o, o + n) + pt_off[i] for i, tp in enumerate(tpls) for o, n in tp.flat.thick_ranges]
+    t = dict(tpls=list(tpls), offs=offs, ns_off=ns_off, pt_off=pt_off,
+             sw_first=torch.from_numpy(np.concatenate(sw_first) if sw_first else np.zeros(0, np.int64)).to(dev),
+             sw_zero=torch.from_numpy(np.concatenate(sw_zero).astype(np.bool_) if sw_zero else np.zeros(0, bool)).to(dev),
+             thick_idx=torch.from_numpy(np.concatenate(thick_idx)).to(dev) if thick_idx else None)
+    if len(_BATCH_TABLES) > 16:
+        _BATCH_TABLES.clear()
+    _BATCH_TABLES[key] = t
+    return t
+
+
+def _batch_params(tpls, arg_lists, dev, grad):
+    """Device arrays of many pooled scenes gathered with one op per kind over all of
+    them, then cut per scene (views): the per-scene form of device_params, for batches.
+    Raises _Resized when a tensor was resized in place since its template matched."""
+    T = _batch_tables(tpls, dev)
+    offs = T['offs']
+    with torch.set_grad_enabled(grad):
+        def gather(kind):
+            srcs, modes = [], set()
+            for tp, a in zip(tpls, arg_lists):
+                mode = tp.cat_mode[kind]
+                if mode is not None:
+                    srcs.extend(tp._get_kind[kind](a))
+                    modes.add(mode)
+            if not srcs:
+                return None
             try:
-                vals = tpl.device_params(args, dev, grad and plan is None)
-            except _Resized:
-                # compare the shapes after all; a template they no longer match is passed
-                tpl._float_seen = None
-                if not tpl.matches(args):
-                    continue
-                vals = tpl.device_params(args, dev, grad and plan is None)
-            if k:
-                bucket.insert(0, bucket.pop(k))
-            scene = tpl.acquire(dev_index)
-            scene.update(float(args[-1]), _stream(), **vals)
-            if check:
-                # what Scene construction rejects is reported now, as the reference reports
-                # it (one wait for the rebuild kernels, before the render is enqueued)
-                try:
-                    check_rebuild(scene)
-                except Exception:
-                    tpl.release(scene)
-                    raise
-            f = _Flat()
-            f.__dict__.update(tpl.flat.__dict__)
-            f.eval_positions = args[6]
-            f.direct = None if plan is None else (tpl, plan)
-            return f, _Lease(tpl, scene), vals
-    flat = flatten_args(args)
-    tpl = _SceneTemplate(args, flat)
-    if sum(len(ts) for ts in _TEMPLATES.values()) >= _MAX_TEMPLATES:
-        _TEMPLATES.clear()
-        bucket = _TEMPLATES.setdefault(len(args), [])
-    bucket.insert(0, tpl)
-    scene = _lib.DeviceScene(flat.desc, dev_index, _lib.MODE_TILED)
-    vals = tpl.device_params(args, dev, True) if grad else None
-    return flat, _Lease(tpl, scene), vals
+                if modes == {'cat'}:
+                    out = torch.cat(srcs)
+                elif modes == {'stack'}:
+                    out = torch.stack(srcs)
+                elif modes == {'cat_flat'}:
+                    out = torch.cat(srcs).reshape(-1)
+                else:
+                    out = _flatten(srcs)
+            except RuntimeError:
+                out = _flatten([t.to(dev) for t in srcs])
+            if out.numel() != int(offs[kind][-1]):
+                raise _Resized()
+            return out.to(device=dev, dtype=torch.float32).contiguous()
+        big = {name: gather(kind) for kind, name in _GATHER_KINDS}
+        sw = gather('sw')
+        if sw is not None:
+            sw = torch.where(T['sw_zero'], torch.zeros((), device=dev), sw[T['sw_first']])
+        thick = None
+        if T['thick_idx'] is not None:
+            thick = torch.zeros(T['pt_off'][-1], dtype=torch.float32, device=dev).index_put(
+                (T['thick_idx'],), gather('thick'))
+    out = []
+    for i, tp in enumerate(tpls):
+        vals = {}
+        for kind, name in _GATHER_KINDS:
+            o0, o1 = int(offs[kind][i]), int(offs[kind][i + 1])
+            vals[name] = big[name][o0:o1] if tp.cat_mode[kind] is not None else None
+        vals['stroke_width'] = sw[T['ns_off'][i]:T['ns_off'][i + 1]] if tp.cat_mode['sw'] is not None else None
+        vals['thickness'] = (thick[T['pt_off'][i]:T['pt_off'][i + 1]]
+                             if tp.flat.thick_ranges else None)
+        out.append(vals)
+    return out
 
 
 def check_rebuild(scene):

@@ -47,3 +47,46 @@ def test_in_place_resize_is_caught_by_the_gather():
         pass  # ... the gather's element count does
     tpl._float_seen = None
     assert not tpl.matches(b)
+
+
+def test_batch_gather_equals_per_scene_gathers():
+    """render_batch gathers the float arrays of all pooled scenes with one op per kind and
+    cuts them per scene; each scene's arrays must equal its own gather, values and
+    gradients (scenes with circles, rects, gradients, per-point thickness, transforms,
+    the same topology twice)."""
+    names = ['primitives', 'gradients', 'thick_stroke', 'transformed', 'strokes', 'mixed_segments', 'strokes']
+    cases = [scenes.CASES[n]() for n in names]
+    for sc in cases:
+        for s in sc['shapes']:
+            for a in ('points', 'thickness', 'radius', 'center', 'p_min', 'p_max', 'stroke_width'):
+                t = getattr(s, a, None)
+                if isinstance(t, torch.Tensor) and t.is_floating_point():
+                    t.requires_grad_(True)
+    arg_lists = [_args(sc) for sc in cases]
+    tpls = []
+    for a in arg_lists:
+        tp = next((t for t in tpls if t.matches(a)), None) or rp._SceneTemplate(a, rp.flatten_args(a))
+        tpls.append(tp)
+    cpu = torch.device('cpu')
+    batch = rp._batch_params(tpls, arg_lists, cpu, True)
+    weights = {}
+    loss_b, loss_s = 0, 0
+    for tp, a, vb in zip(tpls, arg_lists, batch):
+        vs = tp.device_params(a, cpu, True)
+        for k in rp._KINDS:
+            assert (vs[k] is None) == (vb[k] is None), k
+            if vs[k] is None:
+                continue
+            assert torch.equal(vs[k], vb[k]), k
+            w = weights.setdefault((id(tp), k, len(weights)), torch.randn(vs[k].shape))
+            loss_s = loss_s + (vs[k] * w).sum()
+            loss_b = loss_b + (vb[k] * w).sum()
+    leaves = [t for sc in cases for s in sc['shapes'] for t in (getattr(s, a, None) for a in
+              ('points', 'thickness', 'radius', 'center', 'p_min', 'p_max', 'stroke_width'))
+              if isinstance(t, torch.Tensor) and t.requires_grad]
+    gb = torch.autograd.grad(loss_b, leaves, allow_unused=True)
+    gs = torch.autograd.grad(loss_s, leaves, allow_unused=True)
+    for a, b in zip(gb, gs):
+        assert (a is None) == (b is None)
+        if a is not None:
+            assert torch.allclose(a, b, rtol=1e-6, atol=1e-6)
