@@ -19,6 +19,7 @@ Differences from the reference, all deliberate:
 import itertools
 import operator
 import time
+import weakref
 import warnings
 from enum import IntEnum
 
@@ -355,6 +356,16 @@ def _picker(pos):
 
 _size = torch.Tensor.size
 _unflatten = torch._C._nn.unflatten_dense_tensors
+_deref = weakref.ref.__call__
+
+
+def _weak(tensors):
+    return tuple(map(weakref.ref, tensors))
+
+
+def _same(tensors, refs):
+    """`tensors` are the objects `refs` (weak references) point to, in order."""
+    return len(tensors) == len(refs) and all(map(operator.is_, map(_deref, refs), tensors))
 _flatten = torch._C._nn.flatten_dense_tensors
 _version = operator.attrgetter('_version')
 _requires_grad = operator.attrgetter('requires_grad')
@@ -411,8 +422,10 @@ class _SceneTemplate:
         self.int_bytes = flat.int_bytes
         # the integer topology tensors last seen with these contents, and their versions:
         # the same tensors, unmodified, need no comparison of contents
-        self._int_seen = self._get_ints(args)
-        self._int_versions = list(map(_version, self._int_seen))
+        # identity caches hold weak references: a template keeps no argument tensor alive
+        ints = self._get_ints(args)
+        self._int_seen = _weak(ints)
+        self._int_versions = list(map(_version, ints))
         self.idle = []
         self._dev_tables = {}
         self._runs = None
@@ -432,20 +445,20 @@ class _SceneTemplate:
                 return False
             floats = self._get_floats(args)
             seen = self._float_seen
-            if seen is None or not all(map(operator.is_, floats, seen)):
+            if seen is None or not _same(floats, seen):
                 if list(map(_size, floats)) != self.float_shapes:
                     return False
-                self._float_seen = floats
+                self._float_seen = _weak(floats)
         except (RuntimeError, TypeError, AttributeError):
             return False
         cur = self._get_ints(args)
-        if all(map(operator.is_, cur, self._int_seen)) and list(map(_version, cur)) == self._int_versions:
+        if _same(cur, self._int_seen) and list(map(_version, cur)) == self._int_versions:
             return True
         ncp = _cat_int([args[i] for i in self.flat.pos['ncp']])
         gids = _cat_int([args[i] for i in self.flat.pos['ids']])
         if ncp.tobytes() + b'|' + gids.tobytes() != self.int_bytes:
             return False
-        self._int_seen = cur
+        self._int_seen = _weak(cur)
         self._int_versions = list(map(_version, cur))
         return True
 
@@ -560,14 +573,14 @@ class _SceneTemplate:
         ts = self._run_get(args)
         rg = list(map(_requires_grad, ts))
         last = self._direct_last
-        if last is not None and last[1] == rg and all(map(operator.is_, ts, last[0])):
+        if last is not None and last[1] == rg and _same(ts, last[0]):
             # the tensors of the last accepted call, with the same gradient flags: they are
             # still leaves on the same device (a tensor that requires no gradient and
             # becomes a non-leaf changes its flag); only hooks may have been added since
-            sel, zs, dup = last[2], last[3], last[4]
+            sel = list(itertools.compress(ts, rg)) + [z for z in self._zero_get(args) if z.requires_grad]
             if any(map(_hooks, sel)):
                 return None
-            return ts, rg, zs, dup
+            return ts, rg, sel[len(sel) - last[2]:] if last[2] else [], last[3]
         sel = list(itertools.compress(ts, rg))
         if not sel:
             return None
@@ -583,7 +596,7 @@ class _SceneTemplate:
         if set(map(_device, sel)) != {dev} or set(map(_dtype, sel)) != {torch.float32}:
             return None
         dup = len(set(map(id, sel))) != len(sel)
-        self._direct_last = (ts, rg, sel, zs, dup)
+        self._direct_last = (_weak(ts), rg, len(zs), dup)
         return ts, rg, zs, dup
 
     def deliver(self, block, layout, plan):
