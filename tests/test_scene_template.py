@@ -90,3 +90,19 @@ def test_batch_gather_equals_per_scene_gathers():
         assert (a is None) == (b is None)
         if a is not None:
             assert torch.allclose(a, b, rtol=1e-6, atol=1e-6)
+
+
+def test_template_keeps_no_argument_tensor_alive():
+    import gc
+    import weakref
+    sc = scenes.CASES['strokes']()
+    for s in sc['shapes']:
+        s.points.requires_grad_(True)
+    a = _args(sc)
+    tpl = rp._SceneTemplate(a, rp.flatten_args(a))
+    b = _args(sc)
+    assert tpl.matches(b) and tpl.direct_leaves(b, torch.device('cpu')) is not None
+    ref = weakref.ref(sc['shapes'][0].points)
+    del sc, a, b
+    gc.collect()
+    assert ref() is None
