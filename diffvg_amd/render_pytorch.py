@@ -717,7 +717,7 @@ def _host_built_scene(args, dev, grad):
     flat = flatten_args(args)
     tpl = _SceneTemplate(args, flat)
     if sum(len(ts) for ts in _TEMPLATES.values()) >= _MAX_TEMPLATES:
-        _TEMPLATES.clear()
+        clear_scene_cache()
     _TEMPLATES.setdefault(len(args), []).insert(0, tpl)
     scene = _lib.DeviceScene(flat.desc, dev.index if dev.index is not None else -1, _lib.MODE_TILED)
     vals = tpl.device_params(args, dev, True) if grad else None
@@ -848,8 +848,12 @@ def check_rebuild(scene):
 
 
 def clear_scene_cache():
-    """Drops the pooled device scenes (they are rebuilt on demand)."""
+    """Drops the pooled device scenes (they are rebuilt on demand), and every table that
+    refers to their templates."""
     _TEMPLATES.clear()
+    _BATCH_TABLES.clear()
+    from . import batch_render
+    batch_render._HINTS.clear()
 
 
 # Render passes whose device-side findings (fragment overflow, a non-finite grad_img, a

@@ -106,3 +106,20 @@ def test_template_keeps_no_argument_tensor_alive():
     del sc, a, b
     gc.collect()
     assert ref() is None
+
+
+def test_clearing_the_scene_cache_drops_every_template_reference():
+    import gc
+    import weakref
+    from diffvg_amd import batch_render
+    sc = scenes.CASES['strokes']()
+    a = _args(sc)
+    tpl = rp._SceneTemplate(a, rp.flatten_args(a))
+    rp._TEMPLATES.setdefault(len(a), []).insert(0, tpl)
+    rp._batch_tables([tpl, tpl], torch.device('cpu'))
+    batch_render._HINTS[:] = [tpl]
+    ref = weakref.ref(tpl)
+    del tpl
+    rp.clear_scene_cache()
+    gc.collect()
+    assert ref() is None
