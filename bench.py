@@ -611,6 +611,60 @@ def leg_single(cfg, steps, warmup, mode='tiled', prefilter_steps=0, direct_steps
                 sec=sec, kernels=kern, prefilter=pre, steps=steps, warmup=warmup, mode=mode, direct=direct)
 
 
+def leg_scale(steps=10, plain_steps=2):
+    """Scenes of unusual scale (12,000 groups at 256x256; 3,138 groups over a 16x16
+    canvas; paths of 1,500 and 600 cubic segments at 128x128): a step is serialize_scene +
+    RenderFunction.apply + L2 loss + backward, on the tiled schedule and on PLAIN."""
+    import torch
+    import diffvg_amd as pydiffvg
+    from diffvg_amd import workloads
+
+    dev = torch.device('cuda:0')
+    pydiffvg.set_use_gpu(True)
+    pydiffvg.set_device(dev)
+    L, read_profile = _profile_lib()
+    out = {}
+    for make in workloads.SCALE:
+        c = make(device=dev)
+        w, h = c['canvas']
+        make_params(c, dev)
+        target = torch.rand(h, w, 4, device=dev, generator=torch.Generator(device=dev).manual_seed(1))
+
+        def step():
+            args = pydiffvg.RenderFunction.serialize_scene(w, h, c['shapes'], c['groups'], c['filter'])
+            img = pydiffvg.RenderFunction.apply(w, h, 2, 2, 0, None, *args)
+            ((img - target) ** 2).sum().backward()
+
+        res = {'paths': len(c['shapes']), 'canvas': [w, h]}
+        for mode, k in (('tiled', steps), ('plain', plain_steps)):
+            pydiffvg.set_render_mode(mode)
+            for _ in range(2):
+                step()
+            pydiffvg.synchronize()
+            torch.cuda.synchronize()
+            read_profile()
+            L.dvg_profile_enable(1)
+            t0 = time.perf_counter()
+            for _ in range(k):
+                step()
+            pydiffvg.synchronize()
+            torch.cuda.synchronize()
+            sec = (time.perf_counter() - t0) / k
+            L.dvg_profile_enable(0)
+            kern = read_profile()
+            res[mode] = {'ms_per_step': round(sec * 1e3, 3),
+                         'kernel_ms_per_step': round(sum(v['total_ms'] for v in kern.values()) / k, 3)}
+        pydiffvg.set_render_mode('tiled')
+        res['speedup_vs_plain'] = round(res['plain']['ms_per_step'] / res['tiled']['ms_per_step'], 2)
+        res['kernel_speedup_vs_plain'] = round(res['plain']['kernel_ms_per_step'] /
+                                               max(res['tiled']['kernel_ms_per_step'], 1e-9), 2)
+        progress('scale %s: tiled %.2f ms (kernels %.2f), plain %.2f ms (kernels %.2f)' % (
+            c['name'], res['tiled']['ms_per_step'], res['tiled']['kernel_ms_per_step'],
+            res['plain']['ms_per_step'], res['plain']['kernel_ms_per_step']))
+        out[c['name']] = res
+    return out
+
+
 def leg_generative(steps=20, warmup=5, bs=32, n=16, size=32, loop_steps=5):
     """The generative-model front end at the GAN app's defaults (16 two-segment cubic
     strokes with colours, 32x32, 2x2 samples; batch 32): a step is bezier_render + L2 loss
@@ -802,6 +856,7 @@ def run_headline(a):
     out['serving_stream'] = _serving_fields(s)
     out['single_scene'] = singles
     out['generative_strokes'] = leg_generative()
+    out['scale_scenes'] = leg_scale()
     out['roofline'] = _roofline(cpu, b['kernels'], BATCH_KERNELS, 'uneven_scenes_256')
     out['cpu_baseline'] = _cpu_baseline(cpu)
     print(json.dumps(out))
@@ -824,6 +879,12 @@ def run_config(a):
         out.update(_batched_fields(b))
         out['roofline'] = _roofline(cpu, b['kernels'], BATCH_KERNELS, 'uneven_scenes_256')
         out['cpu_baseline'] = _cpu_baseline(cpu)
+    elif a.config == 'scale':
+        r = leg_scale()
+        first = next(iter(r.values()))
+        out = dict(base, value=None, steps=10, warmup=2, ms_per_step=first['tiled']['ms_per_step'],
+                   config={'workload': 'scale_scenes', 'scenes': list(r)})
+        out['scale_scenes'] = r
     elif a.config == 'gen':
         gst = a.steps if a.steps_given else 20
         gw = a.warmup if a.warmup_given else 5
@@ -891,10 +952,10 @@ def main():
                     help='timed steps of the line\'s workload (default 20 graph replays of the 256-scene batch; '
                          '50 steps of a single scene)')
     ap.add_argument('--warmup', type=int, default=None, help='untimed steps first (default 5; 20 for a single scene)')
-    ap.add_argument('--config', default='headline', choices=['headline', '2', '3', '4', '5', 'gen'],
+    ap.add_argument('--config', default='headline', choices=['headline', '2', '3', '4', '5', 'gen', 'scale'],
                     help='headline (default): the 256-scene batch with the serving stream and the single scenes '
                          'as extra keys; 2/3: one 512x512 scene; 4: the batch alone; 5: the serving stream alone; '
-                         'gen: the generative stroke batch')
+                         'gen: the generative stroke batch; scale: scenes of unusual scale')
     ap.add_argument('--mode', default='tiled', choices=['tiled', 'plain'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-baseline-only', action='store_true')
@@ -928,7 +989,7 @@ def main():
     if a.config == 'headline':
         run_headline(a)
     else:
-        a.config = a.config if a.config == 'gen' else int(a.config)
+        a.config = a.config if a.config in ('gen', 'scale') else int(a.config)
         run_config(a)
 
 

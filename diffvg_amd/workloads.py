@@ -16,6 +16,10 @@ numpy.random.default_rng(workload number); parameters are float32 torch tensors 
            cubics per path, resolution in {28, 64, 128, 256}, 10% linear/radial fills
   5 serving_request    one request of the serving stream: path count log-uniform in [1, 4096],
            resolution in {128, 256, 512, 1024}, colour or SDF, prefilter on or off
+
+Scenes of unusual scale (SCALE): many_groups (12,000 small groups, 256x256),
+crowded_tile (3,138 groups over a 16x16 canvas), long_paths (closed and open paths of
+1,500 and 600 cubic segments, 128x128).
 """
 import math
 
@@ -25,7 +29,7 @@ import torch
 from . import diffvg
 from .color import LinearGradient, RadialGradient
 from .pixel_filter import PixelFilter
-from .shape import Circle, Path, ShapeGroup
+from .shape import Circle, Path, Rect, ShapeGroup
 
 
 def _t(x, device):
@@ -133,5 +137,83 @@ def serving_request(rng, device='cpu', res=None, n_paths=None, sdf=None, prefilt
                 filter=PixelFilter(type=diffvg.FilterType.box, radius=torch.tensor(0.5)),
                 nsx=1 if sdf else 2, nsy=1 if sdf else 2, sdf=sdf, prefilter=prefilter)
 
+
+def many_groups(device='cpu', size=256, n=12000, seed=5):
+    """10^4 small groups: closed quadratic triangles, half of them stroked, spread over
+    the canvas (a tile's candidate list holds tens of groups, the scene ten thousand)."""
+    rng = np.random.default_rng(seed)
+    shapes, groups = [], []
+    for i in range(n):
+        c = rng.uniform(2, size - 2, 2)
+        r = rng.uniform(0.8, 3.0)
+        ang = rng.uniform(0, 2 * math.pi) + np.array([0, 2.1, 4.2])
+        pts = []
+        for k in range(3):
+            a0, a1 = ang[k], ang[(k + 1) % 3]
+            p0 = c + r * np.array([math.cos(a0), math.sin(a0)])
+            pm = c + 0.7 * r * np.array([math.cos(0.5 * (a0 + a1)), math.sin(0.5 * (a0 + a1))])
+            pts += [p0, pm]
+        stroked = i % 2 == 0
+        shapes.append(Path(num_control_points=torch.ones(3, dtype=torch.int32), points=_t(pts, device),
+                           is_closed=True, stroke_width=_t(rng.uniform(0.2, 0.8), device)))
+        groups.append(ShapeGroup(shape_ids=torch.tensor([i]), fill_color=_t(np.r_[rng.uniform(0, 1, 3), 0.7], device),
+                                 stroke_color=_t(rng.uniform(0, 1, 4), device) if stroked else None))
+    return dict(name='many_groups_12000_256', canvas=(size, size), shapes=shapes, groups=groups,
+                filter=PixelFilter(type=diffvg.FilterType.box, radius=torch.tensor(0.5)), nsx=2, nsy=2)
+
+
+def crowded_tile(device='cpu', size=16, grid=56, seed=6):
+    """More groups overlapping one tile than one candidate pass of the tiled kernels holds
+    (grid^2 = 3,136 tiny triangles in a 16x16 canvas) under two larger translucent paths."""
+    rng = np.random.default_rng(seed)
+    shapes, groups = [], []
+    step = size / grid
+    for iy in range(grid):
+        for ix in range(grid):
+            c = np.array([(ix + 0.5) * step, (iy + 0.5) * step]) + rng.uniform(-0.1, 0.1, 2) * step
+            r = 0.45 * step
+            pts = [c + [r, 0], c + [-0.5 * r, 0.87 * r], c + [-0.5 * r, -0.87 * r]]
+            shapes.append(Path(num_control_points=torch.zeros(3, dtype=torch.int32), points=_t(pts, device),
+                               is_closed=True, stroke_width=_t(0.1, device)))
+            groups.append(ShapeGroup(shape_ids=torch.tensor([len(shapes) - 1]),
+                                     fill_color=_t(np.r_[rng.uniform(0, 1, 3), 0.6], device)))
+    for k in range(2):
+        c = np.array([size * (0.35 + 0.3 * k), size * 0.5])
+        pts = c + rng.normal(0, size * 0.25, (9, 2))
+        shapes.append(Path(num_control_points=torch.full((3,), 2, dtype=torch.int32), points=_t(pts, device),
+                           is_closed=True, stroke_width=_t(0.6, device)))
+        groups.append(ShapeGroup(shape_ids=torch.tensor([len(shapes) - 1]),
+                                 fill_color=_t(np.r_[rng.uniform(0, 1, 3), 0.5], device),
+                                 stroke_color=_t([0.1, 0.2, 0.9, 0.8], device)))
+    return dict(name='crowded_tile_3138_16', canvas=(size, size), shapes=shapes, groups=groups,
+                filter=PixelFilter(type=diffvg.FilterType.box, radius=torch.tensor(0.5)), nsx=2, nsy=2)
+
+
+def long_paths(device='cpu', size=128, seed=7, n_closed=1500, n_open=600):
+    """A closed path of 1,500 cubic segments (a wobbling ring) and an open stroke of 600
+    cubic segments (a spiral), over a translucent rectangle."""
+    rng = np.random.default_rng(seed)
+    c = np.array([size / 2, size / 2])
+    t = np.linspace(0, 2 * math.pi, 3 * n_closed, endpoint=False)
+    rad = size * 0.36 + size * 0.05 * np.sin(37 * t) + rng.normal(0, 0.4, t.shape)
+    ring = c + np.stack([rad * np.cos(t), rad * np.sin(t)], 1)
+    u = np.linspace(0, 6 * math.pi, 3 * n_open + 1)
+    sr = size * 0.04 + size * 0.018 * u
+    spiral = c + np.stack([sr * np.cos(u), sr * np.sin(u)], 1)
+    shapes = [Rect(p_min=_t([size * 0.1, size * 0.2], device), p_max=_t([size * 0.8, size * 0.7], device)),
+              Path(num_control_points=torch.full((n_closed,), 2, dtype=torch.int32), points=_t(ring, device),
+                   is_closed=True, stroke_width=_t(1.2, device)),
+              Path(num_control_points=torch.full((n_open,), 2, dtype=torch.int32), points=_t(spiral, device),
+                   is_closed=False, stroke_width=_t(1.5, device))]
+    groups = [ShapeGroup(shape_ids=torch.tensor([0]), fill_color=_t([0.2, 0.5, 0.9, 0.6], device)),
+              ShapeGroup(shape_ids=torch.tensor([1]), fill_color=_t([0.9, 0.4, 0.1, 0.7], device),
+                         stroke_color=_t([0.1, 0.1, 0.1, 0.9], device), use_even_odd_rule=False),
+              ShapeGroup(shape_ids=torch.tensor([2]), fill_color=None, stroke_color=_t([0.2, 0.8, 0.3, 0.8], device))]
+    return dict(name='long_paths_1500_600_128', canvas=(size, size), shapes=shapes, groups=groups,
+                filter=PixelFilter(type=diffvg.FilterType.box, radius=torch.tensor(0.5)), nsx=2, nsy=2)
+
+
+# scenes of unusual scale (the GPU scale tests and the bench's scale leg)
+SCALE = (many_groups, crowded_tile, long_paths)
 
 WORKLOADS = {1: single_circle, 2: closed_paths_300, 3: strokes_2048}

@@ -10,6 +10,7 @@ import numpy as np
 import torch
 
 import diffvg_amd as pd
+from diffvg_amd import workloads
 
 
 def T(x):
@@ -165,73 +166,21 @@ def flat_case(case, filt=None):
     return flatten(w, h, sc['shapes'], sc['groups'], ftype, frad)
 
 
-def many_groups(size=256, n=12000, seed=5):
-    """10^4 small groups: closed quadratic triangles, half of them stroked, spread over
-    the canvas (a tile's candidate list holds tens of groups, the scene ten thousand)."""
-    rng = np.random.default_rng(seed)
-    shapes, groups = [], []
-    for i in range(n):
-        c = rng.uniform(2, size - 2, 2)
-        r = rng.uniform(0.8, 3.0)
-        ang = rng.uniform(0, 2 * math.pi) + np.array([0, 2.1, 4.2])
-        pts = []
-        for k in range(3):
-            a0, a1 = ang[k], ang[(k + 1) % 3]
-            p0 = c + r * np.array([math.cos(a0), math.sin(a0)])
-            pm = c + 0.7 * r * np.array([math.cos(0.5 * (a0 + a1)), math.sin(0.5 * (a0 + a1))])
-            pts += [p0, pm]
-        stroked = i % 2 == 0
-        shapes.append(pd.Path(num_control_points=torch.ones(3, dtype=torch.int32), points=T(pts), is_closed=True,
-                              stroke_width=T(rng.uniform(0.2, 0.8))))
-        groups.append(pd.ShapeGroup(shape_ids=torch.tensor([i]), fill_color=T(np.r_[rng.uniform(0, 1, 3), 0.7]),
-                                    stroke_color=T(rng.uniform(0, 1, 4)) if stroked else None))
-    return dict(canvas=(size, size), shapes=shapes, groups=groups, filter=('box', 0.5))
+
+def _from_workload(c):
+    return dict(canvas=c['canvas'], shapes=c['shapes'], groups=c['groups'], filter=('box', 0.5))
 
 
-def crowded_tile(size=16, grid=56, seed=6):
-    """More groups overlapping one tile than one candidate pass holds (grid^2 = 3,136 tiny
-    triangles in a 16x16 canvas) under two larger translucent paths."""
-    rng = np.random.default_rng(seed)
-    shapes, groups = [], []
-    step = size / grid
-    for iy in range(grid):
-        for ix in range(grid):
-            c = np.array([(ix + 0.5) * step, (iy + 0.5) * step]) + rng.uniform(-0.1, 0.1, 2) * step
-            r = 0.45 * step
-            pts = [c + [r, 0], c + [-0.5 * r, 0.87 * r], c + [-0.5 * r, -0.87 * r]]
-            shapes.append(pd.Path(num_control_points=torch.zeros(3, dtype=torch.int32), points=T(pts),
-                                  is_closed=True, stroke_width=T(0.1)))
-            groups.append(pd.ShapeGroup(shape_ids=torch.tensor([len(shapes) - 1]),
-                                        fill_color=T(np.r_[rng.uniform(0, 1, 3), 0.6])))
-    for k in range(2):
-        c = np.array([size * (0.35 + 0.3 * k), size * 0.5])
-        pts = c + rng.normal(0, size * 0.25, (9, 2))
-        shapes.append(pd.Path(num_control_points=torch.full((3,), 2, dtype=torch.int32), points=T(pts),
-                              is_closed=True, stroke_width=T(0.6)))
-        groups.append(pd.ShapeGroup(shape_ids=torch.tensor([len(shapes) - 1]),
-                                    fill_color=T(np.r_[rng.uniform(0, 1, 3), 0.5]),
-                                    stroke_color=T([0.1, 0.2, 0.9, 0.8])))
-    return dict(canvas=(size, size), shapes=shapes, groups=groups, filter=('box', 0.5))
+def many_groups():
+    """10^4 small groups (diffvg_amd.workloads.many_groups)."""
+    return _from_workload(workloads.many_groups())
 
 
-def long_paths(size=128, seed=7, n_closed=1500, n_open=600):
-    """A closed path of 1,500 cubic segments (a wobbling ring) and an open stroke of 600
-    cubic segments (a spiral), over a translucent rectangle."""
-    rng = np.random.default_rng(seed)
-    c = np.array([size / 2, size / 2])
-    t = np.linspace(0, 2 * math.pi, 3 * n_closed, endpoint=False)
-    rad = size * 0.36 + size * 0.05 * np.sin(37 * t) + rng.normal(0, 0.4, t.shape)
-    ring = c + np.stack([rad * np.cos(t), rad * np.sin(t)], 1)
-    u = np.linspace(0, 6 * math.pi, 3 * n_open + 1)
-    sr = size * 0.04 + size * 0.018 * u
-    spiral = c + np.stack([sr * np.cos(u), sr * np.sin(u)], 1)
-    shapes = [pd.Rect(p_min=T([size * 0.1, size * 0.2]), p_max=T([size * 0.8, size * 0.7])),
-              pd.Path(num_control_points=torch.full((n_closed,), 2, dtype=torch.int32), points=T(ring),
-                      is_closed=True, stroke_width=T(1.2)),
-              pd.Path(num_control_points=torch.full((n_open,), 2, dtype=torch.int32), points=T(spiral),
-                      is_closed=False, stroke_width=T(1.5))]
-    groups = [pd.ShapeGroup(shape_ids=torch.tensor([0]), fill_color=T([0.2, 0.5, 0.9, 0.6])),
-              pd.ShapeGroup(shape_ids=torch.tensor([1]), fill_color=T([0.9, 0.4, 0.1, 0.7]),
-                            stroke_color=T([0.1, 0.1, 0.1, 0.9]), use_even_odd_rule=False),
-              pd.ShapeGroup(shape_ids=torch.tensor([2]), fill_color=None, stroke_color=T([0.2, 0.8, 0.3, 0.8]))]
-    return dict(canvas=(size, size), shapes=shapes, groups=groups, filter=('box', 0.5))
+def crowded_tile():
+    """More groups over one tile than one candidate pass holds (workloads.crowded_tile)."""
+    return _from_workload(workloads.crowded_tile())
+
+
+def long_paths():
+    """Paths of 1,500 and 600 cubic segments (workloads.long_paths)."""
+    return _from_workload(workloads.long_paths())
