@@ -63,7 +63,13 @@ struct RefitTables {
     float *thick_leaf0 = nullptr;      // [ns] radius of leaf 0 of a thickness path's segment BVH
     int *order = nullptr;              // [nseg] sort scratch
     int *gorder = nullptr;             // [n_ids] sort scratch
+    const int *long_ids = nullptr;     // [n_long] paths of more than kRefitSegCap segments
+    int n_long = 0;
 };
+
+// the device rebuild gives a path of up to kRefitSegCap segments one wave, a longer one
+// a workgroup
+constexpr int kRefitSegCap = 512;
 
 constexpr int kErrInts = 16;  // ints of SceneHost::err
 

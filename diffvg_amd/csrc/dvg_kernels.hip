@@ -2683,10 +2683,19 @@ DVG_DEV float4 to_f4(const bld::Box &b) { return make_float4(b.minx, b.miny, b.m
 // each extreme, as the sequential fold keeps it), the length sum and the CDF are
 // sequential (one lane, from LDS), and the leaf order is the sorted order of the
 // segments' y-centres -- by rank when the keys are distinct, else by the same introsort
-// on one lane.  Paths with more than kRefitSegCap segments run on one lane from global
-// memory.
+// on one lane.  Paths with more than kRefitSegCap segments get a workgroup each
+// (k_refit_long).
 constexpr int kRefitWaves = kBlock / 64;
-constexpr int kRefitSegCap = 512;
+// a leaf's sort key beside its index: the introsort then reads keys without an
+// indirection, and yields the permutation it yields for indices with a key comparator
+struct KeyIdx {
+    float k;
+    int i;
+};
+struct KeyLess {
+    DVG_DEV bool operator()(const KeyIdx &a, const KeyIdx &b) const { return a.k < b.k; }
+};
+constexpr int kSortStack = 96;  // >= stl::sort_stack_ints(n) for n <= 4096
 
 // (v, i) <- the first (lowest index) of the extreme values in the wave, as a sequential
 // fold `cur = p < cur ? p : cur` (MIN) or `cur = p > cur ? p : cur` keeps it
@@ -2758,6 +2767,14 @@ DVG_DEV void refit_path_serial(const RefitArgs &A, int s) {
         stack);
     int *seg_dfs = writable(S.seg_dfs);
     for (int i = 0; i < nb; i++) seg_dfs[c0 + i] = c0 + order[A.R.seg_perm[c0 + i]];
+    for (int k = 0; k < bld::seg_blocks(nb); k++) {
+        bld::Box bb = bld::box_empty();
+        float br = -INFINITY;
+        for (int i = k * bld::kSegBlk; i < min(nb, (k + 1) * bld::kSegBlk); i++)
+            bld::seg_block_add(bb, br, to_box(seg_box[seg_dfs[c0 + i]]), seg_maxr[seg_dfs[c0 + i]]);
+        writable(S.seg_blk)[c0 + k] = to_f4(bb);
+        writable(S.seg_blk_r)[c0 + k] = br;
+    }
     A.R.thick_leaf0[s] = seg_maxr[c0 + order[0]];
     if (!finite) atomicOr(A.uerr, kUpdNonFinitePoints);
     writable(S.shape_box)[s] = to_f4(box);
@@ -2767,10 +2784,12 @@ DVG_DEV void refit_path_serial(const RefitArgs &A, int s) {
 
 __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
     const DevScene &S = A.S;
-    __shared__ float s_key[kRefitWaves][kRefitSegCap];
+    __shared__ KeyIdx s_kv[kRefitWaves][kRefitSegCap];
+    __shared__ int s_stack[kRefitWaves][kSortStack];
     __shared__ float s_len[kRefitWaves][kRefitSegCap];
     __shared__ int s_ord[kRefitWaves][kRefitSegCap];
     __shared__ float s_maxr[kRefitWaves][kRefitSegCap];
+    __shared__ float4 s_box[kRefitWaves][kRefitSegCap];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int s = blockIdx.x * kRefitWaves + wv;
     if (s >= S.ns) return;  // whole waves; nothing below synchronises the workgroup
@@ -2786,11 +2805,10 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
         return;
     }
     const int c0 = S.seg_begin[s], nb = S.nsegs[s], p0 = S.pt_off[s], np = S.npt[s];
-    if (nb > kRefitSegCap) {
-        if (lane == 0) refit_path_serial(A, s);
-        return;
-    }
-    float *key = s_key[wv], *lens = s_len[wv], *maxr = s_maxr[wv];
+    if (nb > kRefitSegCap) return;  // k_refit_long
+    KeyIdx *kv = s_kv[wv];
+    float *lens = s_len[wv], *maxr = s_maxr[wv];
+    float4 *boxes = s_box[wv];
     int *ord = s_ord[wv];
     const bool thick = S.has_thick[s] != 0;
     const float r = S.stroke_width[s];
@@ -2836,10 +2854,10 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
         }
         seg_box[seg] = to_f4(b);
         seg_maxr[seg] = rr;
+        boxes[i] = to_f4(b);
         maxr[i] = rr;
         lens[i] = bld::segment_length(k, p);
-        key[i] = bld::box_ycentre(b);
-        ord[i] = i;
+        kv[i] = KeyIdx{bld::box_ycentre(b), i};
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -2852,10 +2870,10 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
         const int i = lane + 64 * t;
         rank[t] = 0;
         if (i >= nb) continue;
-        const float ki = key[i];
+        const float ki = kv[i].k;
         plain = plain && ki == ki;
         for (int j = 0; j < nb; j++) {
-            const float kj = key[j];
+            const float kj = kv[j].k;
             rank[t] += kj < ki ? 1 : 0;
             plain = plain && (j == i || kj < ki || ki < kj);
         }
@@ -2869,14 +2887,25 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
             if (i < nb) ord[rank[t]] = i;
         }
     } else if (lane == 0) {
-        int stack[192];
-        stl::sort(ord, nb, [&](int a, int b) { return key[a] < key[b]; }, stack);
+        stl::sort(kv, nb, KeyLess{}, s_stack[wv]);
+        for (int i = 0; i < nb; i++) ord[i] = kv[i].i;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int *seg_dfs = writable(S.seg_dfs);
     for (int i = lane; i < nb; i += 64) seg_dfs[c0 + i] = c0 + ord[A.R.seg_perm[c0 + i]];
+    // segment blocks over the visit order, one lane per block (nb <= kRefitSegCap)
+    for (int k = lane; k < bld::seg_blocks(nb); k += 64) {
+        bld::Box bb = bld::box_empty();
+        float br = -INFINITY;
+        for (int i = k * bld::kSegBlk; i < min(nb, (k + 1) * bld::kSegBlk); i++) {
+            const int leaf = ord[A.R.seg_perm[c0 + i]];
+            bld::seg_block_add(bb, br, to_box(boxes[leaf]), maxr[leaf]);
+        }
+        writable(S.seg_blk)[c0 + k] = to_f4(bb);
+        writable(S.seg_blk_r)[c0 + k] = br;
+    }
     if (lane == 0) {
         // the sequential length sum, PMF and CDF of the host build
         float length = 0.f;
@@ -2893,6 +2922,169 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
         A.R.thick_leaf0[s] = maxr[ord[0]];
         if (!all_finite) atomicOr(A.uerr, kUpdNonFinitePoints);
         writable(S.shape_box)[s] = make_float4(x0, y0, x1, y1);
+        writable(S.shape_length)[s] = len;
+        if (!isfinite(len)) atomicOr(A.uerr, kUpdNonFiniteLength);
+    }
+}
+
+// Paths of more than kRefitSegCap segments, one workgroup each, with the per-wave
+// rebuild's arithmetic: first-extreme folds for the shape box (per wave, then across
+// the waves in order), leaf order by rank when the y-centre keys are distinct (else the
+// introsort on one thread), the sequential length sum and CDF on one thread from LDS.
+// Paths of more than kRefitLongCap segments run on one thread from global memory.
+constexpr int kRefitLongThreads = 1024;
+constexpr int kRefitLongCap = 4096;
+
+template <bool MIN>
+DVG_DEV void first_extreme_combine(float &v, int &i, float ov, int oi) {
+    bool better = MIN ? ov < v : ov > v;
+    bool worse = MIN ? v < ov : v > ov;
+    if (better || (!worse && oi < i)) {
+        v = ov;
+        i = oi;
+    }
+}
+
+__global__ __launch_bounds__(kRefitLongThreads) void k_refit_long(RefitArgs A) {
+    const DevScene &S = A.S;
+    const int s = A.R.long_ids[blockIdx.x];
+    const int c0 = S.seg_begin[s], nb = S.nsegs[s], p0 = S.pt_off[s], np = S.npt[s];
+    const int tid = threadIdx.x, wv = tid >> 6;
+    if (nb > kRefitLongCap) {  // uniform over the workgroup: no barrier is reached
+        if (tid == 0) refit_path_serial(A, s);
+        return;
+    }
+    constexpr int kW = kRefitLongThreads / 64;
+    __shared__ KeyIdx kv[kRefitLongCap];
+    __shared__ float lens[kRefitLongCap];
+    __shared__ int ord[kRefitLongCap];
+    __shared__ int s_stack[kSortStack];
+    __shared__ float s_ext[4][kW];
+    __shared__ int s_exti[4][kW];
+    __shared__ int s_nonfinite, s_tie;
+    if (tid == 0) {
+        s_nonfinite = 0;
+        s_tie = 0;
+    }
+    const bool thick = S.has_thick[s] != 0;
+    const float r = S.stroke_width[s];
+    // shape box and finiteness over the points
+    float x0 = __builtin_inff(), y0 = x0, x1 = -x0, y1 = -x0;
+    int ix0 = INT_MAX, iy0 = INT_MAX, ix1 = INT_MAX, iy1 = INT_MAX;
+    bool finite = true;
+    for (int i = tid; i < np; i += kRefitLongThreads) {
+        float2 q = S.points[p0 + i];
+        finite = finite && isfinite(q.x) && isfinite(q.y) && (!thick || isfinite(S.thickness[p0 + i]));
+        if (ix0 == INT_MAX) {
+            x0 = x1 = q.x;
+            y0 = y1 = q.y;
+            ix0 = iy0 = ix1 = iy1 = i;
+        } else {
+            if (q.x < x0) { x0 = q.x; ix0 = i; }
+            if (q.y < y0) { y0 = q.y; iy0 = i; }
+            if (q.x > x1) { x1 = q.x; ix1 = i; }
+            if (q.y > y1) { y1 = q.y; iy1 = i; }
+        }
+    }
+    wave_first_extreme<true>(x0, ix0);
+    wave_first_extreme<true>(y0, iy0);
+    wave_first_extreme<false>(x1, ix1);
+    wave_first_extreme<false>(y1, iy1);
+    __syncthreads();  // s_nonfinite and s_tie are set
+    if ((tid & 63) == 0) {
+        s_ext[0][wv] = x0, s_exti[0][wv] = ix0;
+        s_ext[1][wv] = y0, s_exti[1][wv] = iy0;
+        s_ext[2][wv] = x1, s_exti[2][wv] = ix1;
+        s_ext[3][wv] = y1, s_exti[3][wv] = iy1;
+    }
+    if (!finite) s_nonfinite = 1;
+    // per segment: box, radius, length; y-centre keys for the leaf sort
+    float4 *seg_box = writable(S.seg_box);
+    float *seg_maxr = writable(S.seg_maxr), *seg_pmf = writable(S.seg_pmf), *seg_cdf = writable(S.seg_cdf);
+    for (int i = tid; i < nb; i += kRefitLongThreads) {
+        const int seg = c0 + i, k = S.seg_kind[seg];
+        const int4 ix = S.seg_idx[seg];
+        const int id[4] = {ix.x, ix.y, ix.z, ix.w};
+        bld::P2 p[4];
+        bld::Box b = bld::box_empty();
+        float rr = -1;
+        for (int j = 0; j <= k + 1; j++) {
+            float2 q = S.points[id[j]];
+            p[j] = bld::P2{q.x, q.y};
+            b = bld::merge(b, p[j]);
+            float rj = thick ? S.thickness[id[j]] : r;
+            rr = j == 0 ? rj : bld::mx(rr, rj);
+        }
+        seg_box[seg] = to_f4(b);
+        seg_maxr[seg] = rr;
+        lens[i] = bld::segment_length(k, p);
+        kv[i] = KeyIdx{bld::box_ycentre(b), i};
+    }
+    __syncthreads();
+    // leaf order by rank; a tie or a NaN key hands the order to the introsort
+    for (int i = tid; i < nb; i += kRefitLongThreads) {
+        const float ki = kv[i].k;
+        bool plain = ki == ki;
+        int rank = 0;
+        for (int j = 0; j < nb; j++) {
+            const float kj = kv[j].k;
+            rank += kj < ki ? 1 : 0;
+            plain = plain && (j == i || kj < ki || ki < kj);
+        }
+        if (plain) {
+            ord[rank] = i;
+        } else {
+            s_tie = 1;
+        }
+    }
+    __syncthreads();
+    if (s_tie) {  // read after the barrier: uniform over the workgroup
+        if (tid == 0) {
+            stl::sort(kv, nb, KeyLess{}, s_stack);
+            for (int i = 0; i < nb; i++) ord[i] = kv[i].i;
+        }
+        __syncthreads();
+    }
+    int *seg_dfs = writable(S.seg_dfs);
+    for (int i = tid; i < nb; i += kRefitLongThreads) seg_dfs[c0 + i] = c0 + ord[A.R.seg_perm[c0 + i]];
+    for (int k = tid; k < bld::seg_blocks(nb); k += kRefitLongThreads) {
+        bld::Box bb = bld::box_empty();
+        float br = -INFINITY;
+        for (int i = k * bld::kSegBlk; i < min(nb, (k + 1) * bld::kSegBlk); i++) {
+            const int leaf = c0 + ord[A.R.seg_perm[c0 + i]];
+            bld::seg_block_add(bb, br, to_box(seg_box[leaf]), seg_maxr[leaf]);
+        }
+        writable(S.seg_blk)[c0 + k] = to_f4(bb);
+        writable(S.seg_blk_r)[c0 + k] = br;
+    }
+    if (tid == 0) {
+        // the sequential length sum, PMF and CDF of the host build
+        float length = 0.f;
+        for (int i = 0; i < nb; i++) length += lens[i];
+        const float len = 0.f + length;
+        const float inv_length = 1.f / len;
+        float prev = 0.f;
+        for (int i = 0; i < nb; i++) {
+            float dd = lens[i] * inv_length;
+            seg_pmf[c0 + i] = dd;
+            prev = i == 0 ? dd : dd + prev;
+            seg_cdf[c0 + i] = prev;
+        }
+        A.R.thick_leaf0[s] = seg_maxr[c0 + ord[0]];
+        float e[4];
+        int ei[4];
+        for (int c = 0; c < 4; c++) {
+            e[c] = s_ext[c][0];
+            ei[c] = s_exti[c][0];
+        }
+        for (int w = 1; w < kW; w++) {
+            first_extreme_combine<true>(e[0], ei[0], s_ext[0][w], s_exti[0][w]);
+            first_extreme_combine<true>(e[1], ei[1], s_ext[1][w], s_exti[1][w]);
+            first_extreme_combine<false>(e[2], ei[2], s_ext[2][w], s_exti[2][w]);
+            first_extreme_combine<false>(e[3], ei[3], s_ext[3][w], s_exti[3][w]);
+        }
+        if (s_nonfinite) atomicOr(A.uerr, kUpdNonFinitePoints);
+        writable(S.shape_box)[s] = make_float4(e[0], e[1], e[2], e[3]);
         writable(S.shape_length)[s] = len;
         if (!isfinite(len)) atomicOr(A.uerr, kUpdNonFiniteLength);
     }
@@ -3030,6 +3222,7 @@ int update_scene(SceneHost &sc, const dvg_scene_params &p, float filter_radius, 
         DVG_PROF(st, "k_refit");
         k_refit_clear<<<1, 64, 0, st>>>(A.uerr);
         k_refit_shapes<<<(D.ns + kRefitWaves - 1) / kRefitWaves, kBlock, 0, st>>>(A);
+        if (A.R.n_long > 0) k_refit_long<<<A.R.n_long, kRefitLongThreads, 0, st>>>(A);
         if (D.ng > 0) k_refit_groups<<<(D.ng + kBlock - 1) / kBlock, kBlock, 0, st>>>(A);
         k_refit_cdf<<<1, kBlock, 0, st>>>(A);
         if (D.nseg > 0) k_seg_prep<<<(D.nseg + kBlock - 1) / kBlock, kBlock, 0, st>>>(D);

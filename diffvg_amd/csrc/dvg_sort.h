@@ -11,7 +11,8 @@
 //     2 floor(log2 n) partitions, after which a range is heap-sorted;
 //   * a final insertion pass sorts the first 16 elements and inserts the rest one by one.
 // Only comparison outcomes steer it, so sorting element indices with a comparator on
-// their keys yields the permutation std::sort yields for the elements themselves.
+// their keys -- or (key, index) pairs with a comparator on the key -- yields the
+// permutation std::sort yields for the elements themselves.
 // tests/csrc/sort_check.cpp compares it with std::sort on inputs full of ties.
 #pragma once
 
@@ -35,8 +36,8 @@ DVG_SORT_HD int floor_log2(int n) {
     return k;
 }
 
-template <class Less>
-DVG_SORT_HD void sift_up(int *a, int hole, int top, int v, Less &less) {
+template <class T, class Less>
+DVG_SORT_HD void sift_up(T *a, int hole, int top, T v, Less &less) {
     int parent = (hole - 1) / 2;
     while (hole > top && less(a[parent], v)) {
         a[hole] = a[parent];
@@ -47,8 +48,8 @@ DVG_SORT_HD void sift_up(int *a, int hole, int top, int v, Less &less) {
 }
 
 // move the larger child up until the hole reaches the bottom, then sift v up from there
-template <class Less>
-DVG_SORT_HD void adjust_heap(int *a, int hole, int len, int v, Less &less) {
+template <class T, class Less>
+DVG_SORT_HD void adjust_heap(T *a, int hole, int len, T v, Less &less) {
     const int top = hole;
     int child = hole;
     while (child < (len - 1) / 2) {
@@ -65,29 +66,30 @@ DVG_SORT_HD void adjust_heap(int *a, int hole, int len, int v, Less &less) {
     sift_up(a, hole, top, v, less);
 }
 
-template <class Less>
-DVG_SORT_HD void heap_sort(int *a, int n, Less &less) {
+template <class T, class Less>
+DVG_SORT_HD void heap_sort(T *a, int n, Less &less) {
     if (n < 2) return;
     for (int parent = (n - 2) / 2;; parent--) {
         adjust_heap(a, parent, n, a[parent], less);
         if (parent == 0) break;
     }
     for (int last = n - 1; last > 0; last--) {
-        int v = a[last];
+        T v = a[last];
         a[last] = a[0];
         adjust_heap(a, 0, last, v, less);
     }
 }
 
-DVG_SORT_HD void swap_at(int *a, int i, int j) {
-    int t = a[i];
+template <class T>
+DVG_SORT_HD void swap_at(T *a, int i, int j) {
+    T t = a[i];
     a[i] = a[j];
     a[j] = t;
 }
 
 // median of a[i], a[j], a[k] swapped into a[r]
-template <class Less>
-DVG_SORT_HD void median_to(int *a, int r, int i, int j, int k, Less &less) {
+template <class T, class Less>
+DVG_SORT_HD void median_to(T *a, int r, int i, int j, int k, Less &less) {
     int m;
     if (less(a[i], a[j])) {
         if (less(a[j], a[k]))
@@ -107,8 +109,8 @@ DVG_SORT_HD void median_to(int *a, int r, int i, int j, int k, Less &less) {
 }
 
 // Hoare partition of [lo, hi) around the pivot at a[p] (p < lo); returns the cut
-template <class Less>
-DVG_SORT_HD int partition(int *a, int lo, int hi, int p, Less &less) {
+template <class T, class Less>
+DVG_SORT_HD int partition(T *a, int lo, int hi, int p, Less &less) {
     for (;;) {
         while (less(a[lo], a[p])) lo++;
         hi--;
@@ -119,9 +121,9 @@ DVG_SORT_HD int partition(int *a, int lo, int hi, int p, Less &less) {
     }
 }
 
-template <class Less>
-DVG_SORT_HD void linear_insert(int *a, int i, Less &less) {
-    int v = a[i];
+template <class T, class Less>
+DVG_SORT_HD void linear_insert(T *a, int i, Less &less) {
+    T v = a[i];
     int j = i;
     while (j > 0 && less(v, a[j - 1])) {
         a[j] = a[j - 1];
@@ -130,11 +132,11 @@ DVG_SORT_HD void linear_insert(int *a, int i, Less &less) {
     a[j] = v;
 }
 
-template <class Less>
-DVG_SORT_HD void insertion_sort(int *a, int lo, int hi, Less &less) {
+template <class T, class Less>
+DVG_SORT_HD void insertion_sort(T *a, int lo, int hi, Less &less) {
     for (int i = lo + 1; i < hi; i++) {
         if (less(a[i], a[lo])) {
-            int v = a[i];
+            T v = a[i];
             for (int j = i; j > lo; j--) a[j] = a[j - 1];
             a[lo] = v;
         } else {
@@ -145,8 +147,8 @@ DVG_SORT_HD void insertion_sort(int *a, int lo, int hi, Less &less) {
 
 // Sorts a[0..n) with `less(x, y)` on element values; `stack` holds sort_stack_ints(n)
 // ints: the pending right parts of the partition loop, at most one per depth level.
-template <class Less>
-DVG_SORT_HD void sort(int *a, int n, Less less, int *stack) {
+template <class T, class Less>
+DVG_SORT_HD void sort(T *a, int n, Less less, int *stack) {
     if (n < 2) return;
     int top = 0;
     int lo = 0, hi = n, depth = 2 * floor_log2(n);

@@ -1,7 +1,8 @@
 // Host check of diffvg_amd/csrc/dvg_sort.h against the C++ library's std::sort: both
 // sort index arrays with a comparator on keys drawn from small alphabets (many ties),
 // sorted, reversed and organ-pipe inputs, and keys shaped to exhaust the partition depth
-// budget; every permutation must be identical.
+// budget; every permutation must be identical, also when dvg_sort.h sorts (key, index)
+// pairs with a comparator on the key.
 #include <algorithm>
 #include <cstdint>
 #include <numeric>
@@ -48,7 +49,16 @@ void sort_fuzz(int64_t trials, uint64_t seed, int64_t *out) {
         std::sort(a.begin(), a.end(), less);
         stack.assign(dvg::stl::sort_stack_ints(n), 0);
         dvg::stl::sort(b.data(), n, less, stack.data());
-        if (a != b) {
+        struct KeyIdx {
+            float k;
+            int i;
+        };
+        std::vector<KeyIdx> kv(n);
+        for (int i = 0; i < n; i++) kv[i] = KeyIdx{key[i], i};
+        dvg::stl::sort(kv.data(), n, [](const KeyIdx &x, const KeyIdx &y) { return x.k < y.k; }, stack.data());
+        bool pairs_same = true;
+        for (int i = 0; i < n; i++) pairs_same = pairs_same && kv[i].i == a[i];
+        if (a != b || !pairs_same) {
             if (first < 0) first = n;
             bad++;
         }

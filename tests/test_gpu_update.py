@@ -47,7 +47,10 @@ def ties(size=64):
 def long_paths(size=96):
     """Paths longer than one wave: a closed 200-cubic loop (distinct sort keys over
     several lanes per segment), a 150-segment staircase whose risers share y-centres
-    (ties), and a 600-segment zigzag (past the rebuild's per-wave segment limit)."""
+    (ties), a 600-segment zigzag (past the rebuild's per-wave segment limit: one
+    workgroup, distinct keys), a 700-segment staircase (one workgroup, ties) and a
+    4,200-segment polyline whose segments all share one y-centre (past the workgroup's
+    limit: one thread)."""
     rng = np.random.default_rng(7)
     t = np.linspace(0, 2 * np.pi, 601)[:-1]
     rad = 30 + 6 * np.sin(5 * t) + rng.uniform(-1, 1, t.shape)
@@ -64,11 +67,23 @@ def long_paths(size=96):
     zig = np.stack([xs, 10 + 4 * (np.arange(601) % 2) + 0.01 * np.arange(601)], 1)
     p_zig = pd.Path(num_control_points=torch.zeros(600, dtype=torch.int32), points=scenes.T(zig), is_closed=False,
                     stroke_width=scenes.T(0.7))
-    shapes = [p_loop, p_stair, p_zig]
+    stair2 = [[2.0, 70.0]]
+    for k in range(350):
+        x, y = stair2[-1]
+        stair2 += [[x + 0.25, y], [x + 0.25, y - 0.5 * ((k % 7) == 0)]]
+    p_stair2 = pd.Path(num_control_points=torch.zeros(700, dtype=torch.int32), points=scenes.T(np.array(stair2)),
+                       is_closed=False, stroke_width=scenes.T(0.6))
+    xs = np.linspace(3, 93, 4201)
+    flat = np.stack([xs, 50 + 3.0 * (np.arange(4201) % 2)], 1)
+    p_flat = pd.Path(num_control_points=torch.zeros(4200, dtype=torch.int32), points=scenes.T(flat),
+                     is_closed=False, stroke_width=scenes.T(0.5))
+    shapes = [p_loop, p_stair, p_zig, p_stair2, p_flat]
     groups = [pd.ShapeGroup(shape_ids=torch.tensor([0]), fill_color=scenes.T([0.2, 0.4, 0.9, 0.7]),
                             stroke_color=scenes.T([0, 0, 0, 1])),
               pd.ShapeGroup(shape_ids=torch.tensor([1]), fill_color=None, stroke_color=scenes.T([1, 0, 0, 1])),
-              pd.ShapeGroup(shape_ids=torch.tensor([2]), fill_color=None, stroke_color=scenes.T([0, 0.6, 0, 1]))]
+              pd.ShapeGroup(shape_ids=torch.tensor([2]), fill_color=None, stroke_color=scenes.T([0, 0.6, 0, 1])),
+              pd.ShapeGroup(shape_ids=torch.tensor([3]), fill_color=None, stroke_color=scenes.T([0.5, 0, 0.5, 1])),
+              pd.ShapeGroup(shape_ids=torch.tensor([4]), fill_color=None, stroke_color=scenes.T([0, 0, 0.8, 0.6]))]
     return dict(canvas=(size, size), shapes=shapes, groups=groups, filter=('box', 0.5))
 
 
