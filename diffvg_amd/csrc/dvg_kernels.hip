@@ -2695,7 +2695,19 @@ struct KeyIdx {
 struct KeyLess {
     DVG_DEV bool operator()(const KeyIdx &a, const KeyIdx &b) const { return a.k < b.k; }
 };
-constexpr int kSortStack = 96;  // >= stl::sort_stack_ints(n) for n <= 4096
+constexpr int kSortStack = 96;  // >= 3 (2 floor_log2(n) + 2) for n <= 4096
+
+// where the introsort's final insertion pass puts element i of a[0..n): the stable
+// rank, for keys without NaN
+DVG_DEV int stable_rank(const KeyIdx *a, int n, int i) {
+    const float ki = a[i].k;
+    int pos = 0;
+    for (int j = 0; j < n; j++) {
+        const float kj = a[j].k;
+        pos += (kj < ki || (j < i && !(ki < kj))) ? 1 : 0;
+    }
+    return pos;
+}
 
 // (v, i) <- the first (lowest index) of the extreme values in the wave, as a sequential
 // fold `cur = p < cur ? p : cur` (MIN) or `cur = p > cur ? p : cur` keeps it
@@ -2863,7 +2875,7 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // leaf order by rank when the keys are distinct numbers
-    bool plain = true;
+    bool plain = true, number = true;
     int rank[kRefitSegCap / 64];
 #pragma unroll
     for (int t = 0; t < kRefitSegCap / 64; t++) {
@@ -2871,6 +2883,7 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
         rank[t] = 0;
         if (i >= nb) continue;
         const float ki = kv[i].k;
+        number = number && ki == ki;
         plain = plain && ki == ki;
         for (int j = 0; j < nb; j++) {
             const float kj = kv[j].k;
@@ -2878,7 +2891,7 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
             plain = plain && (j == i || kj < ki || ki < kj);
         }
     }
-    const bool distinct = __all(plain);
+    const bool distinct = __all(plain), numbers = __all(number);
     __builtin_amdgcn_wave_barrier();
     if (distinct) {
 #pragma unroll
@@ -2886,7 +2899,19 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
             const int i = lane + 64 * t;
             if (i < nb) ord[rank[t]] = i;
         }
+    } else if (numbers) {
+        // ties: the introsort's partition phase on one lane, its final insertion pass
+        // as the stable rank on all lanes
+        if (lane == 0) {
+            KeyLess less;
+            stl::partition_phase(kv, 0, nb, 2 * stl::floor_log2(nb), less, s_stack[wv]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int i = lane; i < nb; i += 64) ord[stable_rank(kv, nb, i)] = kv[i].i;
     } else if (lane == 0) {
+        // a NaN key (the update reports non-finite points): any permutation serves
         stl::sort(kv, nb, KeyLess{}, s_stack[wv]);
         for (int i = 0; i < nb; i++) ord[i] = kv[i].i;
     }
@@ -2935,6 +2960,74 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
 constexpr int kRefitLongThreads = 1024;
 constexpr int kRefitLongCap = 4096;
 
+constexpr int kRefitBigRange = 192;  // ranges the workgroup partitions together
+constexpr int kRefitMaxBig = kRefitLongCap / (kRefitBigRange + 1) + 1;
+constexpr int kRefitMaxSmall = kRefitLongCap / (stl::kInsertionLimit + 1) + 1;
+
+// exclusive prefix sum of one int per thread over a kRefitLongThreads workgroup;
+// `total` gets the sum.  Every thread of the workgroup must call it.
+DVG_DEV int block_excl_scan(int v, int &total, int *s_scan) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_scan[wv] = x;
+    __syncthreads();
+    int base = 0, tot = 0;
+    for (int w = 0; w < kRefitLongThreads / 64; w++) {
+        const int t = s_scan[w];
+        base += w < wv ? t : 0;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return base + x - v;
+}
+
+// stl::partition_by_stoppers(a, b, e, p) by the workgroup: each thread takes up to
+// four consecutive positions, numbers its left stoppers by a prefix count and its right
+// stoppers by a suffix count, and the first M pairs with L_k < R_k swap.  Returns the
+// cut in every thread; a[] is final when it returns.  Lpos holds e - b + 1 ints, Rpos
+// e - b + 2.
+DVG_DEV int block_partition(KeyIdx *a, int b, int e, int p, int *Lpos, int *Rpos, int *s_scan) {
+    const int tid = threadIdx.x;
+    const int per = (e - b + kRefitLongThreads - 1) / kRefitLongThreads;
+    const int i0 = min(e, b + tid * per), i1 = min(e, i0 + per);
+    const float P = a[p].k;
+    int nl = 0, nr = 0;
+    for (int i = i0; i < i1; i++) {
+        nl += !(a[i].k < P) ? 1 : 0;
+        nr += !(P < a[i].k) ? 1 : 0;
+    }
+    int NL, NR;
+    int kl = block_excl_scan(nl, NL, s_scan);
+    const int pr = block_excl_scan(nr, NR, s_scan);
+    int kr = NR - pr;  // right stoppers at positions >= i0: the index of the first one here
+    for (int i = i0; i < i1; i++) {
+        if (!(a[i].k < P)) Lpos[++kl] = i;
+        if (!(P < a[i].k)) Rpos[kr--] = i;
+    }
+    if (tid == 0) Rpos[NR + 1] = p;
+    __syncthreads();
+    int cnt = 0;
+    for (int k = tid + 1; k <= min(NL, NR + 1); k += kRefitLongThreads) cnt += Lpos[k] < Rpos[k] ? 1 : 0;
+    int M;
+    (void)block_excl_scan(cnt, M, s_scan);
+    // the swapped pairs are disjoint: every swapped L lies below every swapped R
+    for (int k = tid + 1; k <= M; k += kRefitLongThreads) {
+        const KeyIdx x = a[Lpos[k]], y = a[Rpos[k]];
+        a[Lpos[k]] = y;
+        a[Rpos[k]] = x;
+    }
+    int cut = M + 1 <= NL ? Lpos[M + 1] : e;
+    if (M >= 1 && Rpos[M] < cut) cut = Rpos[M];
+    __syncthreads();
+    return cut;
+}
+
 template <bool MIN>
 DVG_DEV void first_extreme_combine(float &v, int &i, float ov, int oi) {
     bool better = MIN ? ov < v : ov > v;
@@ -2957,8 +3050,12 @@ __global__ __launch_bounds__(kRefitLongThreads) void k_refit_long(RefitArgs A) {
     constexpr int kW = kRefitLongThreads / 64;
     __shared__ KeyIdx kv[kRefitLongCap];
     __shared__ float lens[kRefitLongCap];
-    __shared__ int ord[kRefitLongCap];
+    __shared__ int ord[kRefitLongCap];  // also the left stoppers of the tie sort
+    __shared__ int rpos[kRefitLongCap + 2];
     __shared__ int s_stack[kSortStack];
+    __shared__ int s_scan[kRefitLongThreads / 64];
+    __shared__ int4 s_big[2][kRefitMaxBig], s_small[kRefitMaxSmall];
+    __shared__ int s_nbig[2], s_nsmall;
     __shared__ float s_ext[4][kW];
     __shared__ int s_exti[4][kW];
     __shared__ int s_nonfinite, s_tie;
@@ -3034,15 +3131,58 @@ __global__ __launch_bounds__(kRefitLongThreads) void k_refit_long(RefitArgs A) {
         if (plain) {
             ord[rank] = i;
         } else {
-            s_tie = 1;
+            atomicOr(&s_tie, ki == ki ? 1 : 2);
         }
     }
     __syncthreads();
-    if (s_tie) {  // read after the barrier: uniform over the workgroup
+    const int tie = s_tie;  // read after the barrier: uniform over the workgroup
+    if (tie & 2) {
+        // a NaN key (the update reports non-finite points): any permutation serves
         if (tid == 0) {
             stl::sort(kv, nb, KeyLess{}, s_stack);
             for (int i = 0; i < nb; i++) ord[i] = kv[i].i;
         }
+        __syncthreads();
+    } else if (tie) {
+        // ties: the introsort's schedule, with the ranges longer than kRefitBigRange
+        // partitioned by the workgroup one level at a time, the others by one thread
+        // each, and the final insertion pass as the stable rank
+        KeyLess less;
+        auto classify = [&](int lo, int hi, int depth, int into) {
+            if (hi - lo <= stl::kInsertionLimit) return;
+            if (hi - lo <= kRefitBigRange || depth == 0)
+                s_small[s_nsmall++] = make_int4(lo, hi, depth, 0);
+            else
+                s_big[into][s_nbig[into]++] = make_int4(lo, hi, depth, 0);
+        };
+        if (tid == 0) {
+            s_nsmall = 0;
+            s_nbig[0] = s_nbig[1] = 0;
+            classify(0, nb, 2 * stl::floor_log2(nb), 0);
+        }
+        __syncthreads();
+        for (int cur = 0; s_nbig[cur] > 0; cur ^= 1) {
+            const int nrng = s_nbig[cur];
+            if (tid == 0) s_nbig[cur ^ 1] = 0;
+            for (int r = 0; r < nrng; r++) {
+                const int4 R = s_big[cur][r];
+                if (tid == 0) stl::median_to(kv, R.x, R.x + 1, R.x + (R.y - R.x) / 2, R.y - 1, less);
+                __syncthreads();
+                const int cut = block_partition(kv, R.x + 1, R.y, R.x, ord, rpos, s_scan);
+                if (tid == 0) {
+                    classify(R.x, cut, R.z - 1, cur ^ 1);
+                    classify(cut, R.y, R.z - 1, cur ^ 1);
+                }
+            }
+            __syncthreads();
+        }
+        for (int r = tid; r < s_nsmall; r += kRefitLongThreads) {
+            const int4 R = s_small[r];
+            int stack[kSortStack];
+            stl::partition_phase(kv, R.x, R.y, R.z, less, stack);
+        }
+        __syncthreads();
+        for (int i = tid; i < nb; i += kRefitLongThreads) ord[stable_rank(kv, nb, i)] = kv[i].i;
         __syncthreads();
     }
     int *seg_dfs = writable(S.seg_dfs);

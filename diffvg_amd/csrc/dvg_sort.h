@@ -145,13 +145,15 @@ DVG_SORT_HD void insertion_sort(T *a, int lo, int hi, Less &less) {
     }
 }
 
-// Sorts a[0..n) with `less(x, y)` on element values; `stack` holds sort_stack_ints(n)
-// ints: the pending right parts of the partition loop, at most one per depth level.
+// The partition phase of sort() on [lo, hi) with `depth` partitions left: ranges
+// longer than kInsertionLimit are partitioned (heap-sorted once the depth budget is
+// spent) until every range left has at most kInsertionLimit elements.  Ranges are
+// disjoint and each is processed from its own contents and depth, so they may be
+// processed in any order (or at once) with the same result.  `stack` holds
+// sort_stack_ints(hi - lo) ints: the pending right parts, at most one per depth level.
 template <class T, class Less>
-DVG_SORT_HD void sort(T *a, int n, Less less, int *stack) {
-    if (n < 2) return;
+DVG_SORT_HD void partition_phase(T *a, int lo, int hi, int depth, Less &less, int *stack) {
     int top = 0;
-    int lo = 0, hi = n, depth = 2 * floor_log2(n);
     for (;;) {
         while (hi - lo > kInsertionLimit) {
             if (depth == 0) {
@@ -172,12 +174,54 @@ DVG_SORT_HD void sort(T *a, int n, Less less, int *stack) {
         hi = stack[--top];
         lo = stack[--top];
     }
+}
+
+// The final pass of sort(): insertion sort of the whole array.  Insertion sort is
+// stable, so for a strict weak order `less` it leaves element i of the array at
+// #{j : less(a[j], a[i])} + #{j < i : a[j] equivalent to a[i]}.
+template <class T, class Less>
+DVG_SORT_HD void final_insertion(T *a, int n, Less &less) {
     if (n > kInsertionLimit) {
         insertion_sort(a, 0, kInsertionLimit, less);
         for (int i = kInsertionLimit; i < n; i++) linear_insert(a, i, less);
     } else {
         insertion_sort(a, 0, n, less);
     }
+}
+
+// Sorts a[0..n) with `less(x, y)` on element values; `stack` holds sort_stack_ints(n)
+// ints.
+template <class T, class Less>
+DVG_SORT_HD void sort(T *a, int n, Less less, int *stack) {
+    if (n < 2) return;
+    partition_phase(a, 0, n, 2 * floor_log2(n), less, stack);
+    final_insertion(a, n, less);
+}
+
+// partition(a, lo, hi, p) restated through its stoppers, the form a workgroup computes
+// in parallel.  Left stoppers are the positions i of [lo, hi) with !less(a[i], a[p]) in
+// increasing order L1 < L2 < ...; right stoppers the positions with !less(a[p], a[i]) in
+// decreasing order R1 > R2 > ..., followed by p itself (which stops the right scan).
+// The k-th round of the Hoare loop stops at L_k and R_k -- the positions between the
+// previous stops are unmodified -- and swaps them while L_k < R_k; as L rises and R
+// falls, the rounds that swap are k = 1..M with M the number of k for which L_k < R_k.
+// The scan that ends the loop stops at L_{M+1}, or earlier at R_M, which holds a left
+// stopper after its swap: the cut is min(L_{M+1}, R_M).  Lpos and Rpos hold hi - lo + 2
+// ints.  (A left stopper always exists after median_to; with none the cut is hi.)
+template <class T, class Less>
+DVG_SORT_HD int partition_by_stoppers(T *a, int lo, int hi, int p, Less &less, int *Lpos, int *Rpos) {
+    int nl = 0, nr = 0;
+    for (int i = lo; i < hi; i++)
+        if (!less(a[i], a[p])) Lpos[++nl] = i;
+    for (int i = hi - 1; i >= lo; i--)
+        if (!less(a[p], a[i])) Rpos[++nr] = i;
+    Rpos[++nr] = p;
+    int m = 0;
+    for (int k = 1; k <= nl && k <= nr; k++) m += Lpos[k] < Rpos[k] ? 1 : 0;
+    for (int k = 1; k <= m; k++) swap_at(a, Lpos[k], Rpos[k]);
+    int cut = m + 1 <= nl ? Lpos[m + 1] : hi;
+    if (m >= 1 && Rpos[m] < cut) cut = Rpos[m];
+    return cut;
 }
 
 // ints the `stack` argument of sort() needs for n elements

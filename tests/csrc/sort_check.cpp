@@ -2,7 +2,10 @@
 // sort index arrays with a comparator on keys drawn from small alphabets (many ties),
 // sorted, reversed and organ-pipe inputs, and keys shaped to exhaust the partition depth
 // budget; every permutation must be identical, also when dvg_sort.h sorts (key, index)
-// pairs with a comparator on the key.
+// pairs with a comparator on the key, and when the sort runs in the form the device
+// rebuild runs it: large ranges partitioned through their stoppers level by level, the
+// remaining ranges by the sequential partition phase, the final insertion pass as a
+// stable rank.
 #include <algorithm>
 #include <cstdint>
 #include <numeric>
@@ -10,6 +13,57 @@
 #include <vector>
 
 #include "../../diffvg_amd/csrc/dvg_sort.h"
+
+namespace {
+struct KeyIdx {
+    float k;
+    int i;
+};
+struct KeyLess {
+    bool operator()(const KeyIdx &x, const KeyIdx &y) const { return x.k < y.k; }
+};
+
+// the device rebuild's schedule of dvg::stl::sort, run sequentially: ranges longer than
+// `big` (with depth left) are partitioned by partition_by_stoppers one level at a time,
+// the others by partition_phase, then every element goes to its stable rank
+std::vector<int> two_phase_order(std::vector<KeyIdx> a, int big) {
+    using namespace dvg::stl;
+    const int n = (int)a.size();
+    KeyLess less;
+    struct R {
+        int lo, hi, depth;
+    };
+    std::vector<R> cur, next, small;
+    auto classify = [&](R r, std::vector<R> &into) {
+        if (r.hi - r.lo <= kInsertionLimit) return;
+        if (r.hi - r.lo <= big || r.depth == 0)
+            small.push_back(r);
+        else
+            into.push_back(r);
+    };
+    if (n >= 2) classify(R{0, n, 2 * floor_log2(n)}, cur);
+    std::vector<int> Lpos(n + 2), Rpos(n + 2), stack(sort_stack_ints(n));
+    while (!cur.empty()) {
+        next.clear();
+        for (const R &r : cur) {
+            const int depth = r.depth - 1;
+            median_to(a.data(), r.lo, r.lo + 1, r.lo + (r.hi - r.lo) / 2, r.hi - 1, less);
+            int cut = partition_by_stoppers(a.data(), r.lo + 1, r.hi, r.lo, less, Lpos.data(), Rpos.data());
+            classify(R{r.lo, cut, depth}, next);
+            classify(R{cut, r.hi, depth}, next);
+        }
+        cur.swap(next);
+    }
+    for (const R &r : small) partition_phase(a.data(), r.lo, r.hi, r.depth, less, stack.data());
+    std::vector<int> out(n);
+    for (int i = 0; i < n; i++) {
+        int pos = 0;
+        for (int j = 0; j < n; j++) pos += (less(a[j], a[i]) || (j < i && !less(a[i], a[j]))) ? 1 : 0;
+        out[pos] = a[i].i;
+    }
+    return out;
+}
+}  // namespace
 
 extern "C" {
 // out[0] = arrays compared, out[1] = arrays whose permutations differ, out[2] = first
@@ -49,16 +103,15 @@ void sort_fuzz(int64_t trials, uint64_t seed, int64_t *out) {
         std::sort(a.begin(), a.end(), less);
         stack.assign(dvg::stl::sort_stack_ints(n), 0);
         dvg::stl::sort(b.data(), n, less, stack.data());
-        struct KeyIdx {
-            float k;
-            int i;
-        };
         std::vector<KeyIdx> kv(n);
         for (int i = 0; i < n; i++) kv[i] = KeyIdx{key[i], i};
-        dvg::stl::sort(kv.data(), n, [](const KeyIdx &x, const KeyIdx &y) { return x.k < y.k; }, stack.data());
+        std::vector<KeyIdx> kv0 = kv;
+        dvg::stl::sort(kv.data(), n, KeyLess{}, stack.data());
         bool pairs_same = true;
         for (int i = 0; i < n; i++) pairs_same = pairs_same && kv[i].i == a[i];
-        if (a != b || !pairs_same) {
+        const int big = t % 3 == 0 ? 17 : t % 3 == 1 ? 40 : 192;
+        bool two_phase_same = n > 3000 || two_phase_order(kv0, big) == a;
+        if (a != b || !pairs_same || !two_phase_same) {
             if (first < 0) first = n;
             bad++;
         }
@@ -94,6 +147,8 @@ void sort_adversary(int n, int64_t *out) {
     auto less = [&](int x, int y) { return val[x] < val[y]; };
     std::sort(a.begin(), a.end(), less);
     dvg::stl::sort(b.data(), n, less, stack.data());
-    out[0] = a == b ? 0 : 1;
+    std::vector<KeyIdx> kv(n);
+    for (int i = 0; i < n; i++) kv[i] = KeyIdx{(float)val[i], i};
+    out[0] = (a == b && (n > 3000 || two_phase_order(kv, 40) == a)) ? 0 : 1;
 }
 }

@@ -48,9 +48,10 @@ def long_paths(size=96):
     """Paths longer than one wave: a closed 200-cubic loop (distinct sort keys over
     several lanes per segment), a 150-segment staircase whose risers share y-centres
     (ties), a 600-segment zigzag (past the rebuild's per-wave segment limit: one
-    workgroup, distinct keys), a 700-segment staircase (one workgroup, ties) and a
-    4,200-segment polyline whose segments all share one y-centre (past the workgroup's
-    limit: one thread)."""
+    workgroup, distinct keys), a 700-segment staircase (one workgroup, ties), a
+    2,500-segment polyline whose segments all share one y-centre (one workgroup, every
+    element a stopper of every partition) and a 4,200-segment one like it (past the
+    workgroup's limit: one thread)."""
     rng = np.random.default_rng(7)
     t = np.linspace(0, 2 * np.pi, 601)[:-1]
     rad = 30 + 6 * np.sin(5 * t) + rng.uniform(-1, 1, t.shape)
@@ -77,13 +78,18 @@ def long_paths(size=96):
     flat = np.stack([xs, 50 + 3.0 * (np.arange(4201) % 2)], 1)
     p_flat = pd.Path(num_control_points=torch.zeros(4200, dtype=torch.int32), points=scenes.T(flat),
                      is_closed=False, stroke_width=scenes.T(0.5))
-    shapes = [p_loop, p_stair, p_zig, p_stair2, p_flat]
+    xs = np.linspace(3, 93, 2501)
+    flat2 = np.stack([xs, 20 + 3.0 * (np.arange(2501) % 2)], 1)
+    p_flat2 = pd.Path(num_control_points=torch.zeros(2500, dtype=torch.int32), points=scenes.T(flat2),
+                      is_closed=False, stroke_width=scenes.T(0.5))
+    shapes = [p_loop, p_stair, p_zig, p_stair2, p_flat, p_flat2]
     groups = [pd.ShapeGroup(shape_ids=torch.tensor([0]), fill_color=scenes.T([0.2, 0.4, 0.9, 0.7]),
                             stroke_color=scenes.T([0, 0, 0, 1])),
               pd.ShapeGroup(shape_ids=torch.tensor([1]), fill_color=None, stroke_color=scenes.T([1, 0, 0, 1])),
               pd.ShapeGroup(shape_ids=torch.tensor([2]), fill_color=None, stroke_color=scenes.T([0, 0.6, 0, 1])),
               pd.ShapeGroup(shape_ids=torch.tensor([3]), fill_color=None, stroke_color=scenes.T([0.5, 0, 0.5, 1])),
-              pd.ShapeGroup(shape_ids=torch.tensor([4]), fill_color=None, stroke_color=scenes.T([0, 0, 0.8, 0.6]))]
+              pd.ShapeGroup(shape_ids=torch.tensor([4]), fill_color=None, stroke_color=scenes.T([0, 0, 0.8, 0.6])),
+              pd.ShapeGroup(shape_ids=torch.tensor([5]), fill_color=None, stroke_color=scenes.T([0.8, 0.6, 0, 0.6]))]
     return dict(canvas=(size, size), shapes=shapes, groups=groups, filter=('box', 0.5))
 
 
