@@ -614,7 +614,9 @@ def leg_single(cfg, steps, warmup, mode='tiled', prefilter_steps=0, direct_steps
 def leg_scale(steps=10, plain_steps=2):
     """Scenes of unusual scale (12,000 groups at 256x256; 3,138 groups over a 16x16
     canvas; paths of 1,500 and 600 cubic segments at 128x128): a step is serialize_scene +
-    RenderFunction.apply + L2 loss + backward, on the tiled schedule and on PLAIN."""
+    RenderFunction.apply + L2 loss + backward (gradients reset as an optimizer's
+    zero_grad does), on the tiled schedule, on it with direct leaf gradients, and on
+    PLAIN."""
     import torch
     import diffvg_amd as pydiffvg
     from diffvg_amd import workloads
@@ -627,17 +629,21 @@ def leg_scale(steps=10, plain_steps=2):
     for make in workloads.SCALE:
         c = make(device=dev)
         w, h = c['canvas']
-        make_params(c, dev)
+        params = make_params(c, dev)
         target = torch.rand(h, w, 4, device=dev, generator=torch.Generator(device=dev).manual_seed(1))
 
         def step():
+            for p in params:
+                p.grad = None
             args = pydiffvg.RenderFunction.serialize_scene(w, h, c['shapes'], c['groups'], c['filter'])
             img = pydiffvg.RenderFunction.apply(w, h, 2, 2, 0, None, *args)
             ((img - target) ** 2).sum().backward()
 
         res = {'paths': len(c['shapes']), 'canvas': [w, h]}
-        for mode, k in (('tiled', steps), ('plain', plain_steps)):
+        for label, mode, k in (('tiled', 'tiled', steps), ('tiled_direct_leaf_grads', 'tiled', steps),
+                               ('plain', 'plain', plain_steps)):
             pydiffvg.set_render_mode(mode)
+            pydiffvg.set_direct_leaf_gradients(label == 'tiled_direct_leaf_grads')
             for _ in range(2):
                 step()
             pydiffvg.synchronize()
@@ -652,15 +658,17 @@ def leg_scale(steps=10, plain_steps=2):
             sec = (time.perf_counter() - t0) / k
             L.dvg_profile_enable(0)
             kern = read_profile()
-            res[mode] = {'ms_per_step': round(sec * 1e3, 3),
-                         'kernel_ms_per_step': round(sum(v['total_ms'] for v in kern.values()) / k, 3)}
+            res[label] = {'ms_per_step': round(sec * 1e3, 3),
+                          'kernel_ms_per_step': round(sum(v['total_ms'] for v in kern.values()) / k, 3)}
         pydiffvg.set_render_mode('tiled')
+        pydiffvg.set_direct_leaf_gradients(False)
         res['speedup_vs_plain'] = round(res['plain']['ms_per_step'] / res['tiled']['ms_per_step'], 2)
         res['kernel_speedup_vs_plain'] = round(res['plain']['kernel_ms_per_step'] /
                                                max(res['tiled']['kernel_ms_per_step'], 1e-9), 2)
-        progress('scale %s: tiled %.2f ms (kernels %.2f), plain %.2f ms (kernels %.2f)' % (
+        progress('scale %s: tiled %.2f ms (kernels %.2f), direct leaf grads %.2f ms, plain %.2f ms (kernels %.2f)' % (
             c['name'], res['tiled']['ms_per_step'], res['tiled']['kernel_ms_per_step'],
-            res['plain']['ms_per_step'], res['plain']['kernel_ms_per_step']))
+            res['tiled_direct_leaf_grads']['ms_per_step'], res['plain']['ms_per_step'],
+            res['plain']['kernel_ms_per_step']))
         out[c['name']] = res
     return out
 

@@ -375,6 +375,24 @@ _device = operator.attrgetter('device')
 _dtype = operator.attrgetter('dtype')
 
 
+_I32 = {}
+
+
+def _as_i32(t):
+    """`t` as int32.  The conversion of another integer type is kept while `t` lives
+    unmodified, so a scene serialized again passes the same tensor objects (and the
+    template's identity check holds)."""
+    if t.dtype == torch.int32:
+        return t
+    key = id(t)
+    e = _I32.get(key)
+    if e is not None and e[0]() is t and e[1] == t._version:
+        return e[2]
+    out = t.to(torch.int32)
+    _I32[key] = (weakref.ref(t, lambda _, k=key: _I32.pop(k, None)), t._version, out)
+    return out
+
+
 class _Resized(Exception):
     """A template's argument tensor was resized in place since its last match."""
 
@@ -927,10 +945,10 @@ class RenderFunction(torch.autograd.Function):
                 sw = shape.stroke_width
                 if sw.dim() > 0 and sw.size(0) > 1:
                     # per-point thickness; the stroke width slot then holds zero
-                    extend((t_path, ncp if ncp.dtype == i32 else ncp.to(i32), points, sw, shape.is_closed,
+                    extend((t_path, ncp if ncp.dtype == i32 else _as_i32(ncp), points, sw, shape.is_closed,
                             shape.use_distance_approx, torch.tensor(0.0)))
                 else:
-                    extend((t_path, ncp if ncp.dtype == i32 else ncp.to(i32), points, None, shape.is_closed,
+                    extend((t_path, ncp if ncp.dtype == i32 else _as_i32(ncp), points, None, shape.is_closed,
                             shape.use_distance_approx, sw))
             elif isinstance(shape, Circle):
                 extend((t_circle, shape.radius, shape.center, shape.stroke_width))
@@ -947,7 +965,7 @@ class RenderFunction(torch.autograd.Function):
                 raise TypeError('unsupported shape %r' % (type(shape),))
         for shape_group in shape_groups:
             ids = shape_group.shape_ids
-            append(ids if ids.dtype == i32 else ids.to(i32))
+            append(ids if ids.dtype == i32 else _as_i32(ids))
             fill = shape_group.fill_color
             for color in (fill, shape_group.stroke_color):
                 if color is None:

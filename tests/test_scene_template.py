@@ -123,3 +123,30 @@ def test_clearing_the_scene_cache_drops_every_template_reference():
     rp.clear_scene_cache()
     gc.collect()
     assert ref() is None
+
+
+def test_int64_shape_ids_serialize_to_the_same_int32_tensor_until_modified():
+    import gc
+    ids = torch.tensor([0, 2])
+    ncp = torch.tensor([2, 2, 1])
+    sc = dict(canvas=(8, 8),
+              shapes=[pd.Path(num_control_points=ncp, points=torch.rand(8, 2) * 8, is_closed=True),
+                      pd.Circle(radius=torch.tensor(1.0), center=torch.tensor([4.0, 4.0])),
+                      pd.Circle(radius=torch.tensor(2.0), center=torch.tensor([3.0, 3.0]))],
+              groups=[pd.ShapeGroup(shape_ids=ids, fill_color=torch.tensor([1.0, 0, 0, 1]))])
+    a, b = _args(sc), _args(sc)
+    pos_ids = next(i for i, x in enumerate(a) if isinstance(x, torch.Tensor) and x.tolist() == [0, 2])
+    assert a[pos_ids] is b[pos_ids] and a[pos_ids].dtype == torch.int32
+    pos_ncp = next(i for i, x in enumerate(a) if isinstance(x, torch.Tensor) and x.tolist() == [2, 2, 1])
+    assert a[pos_ncp] is b[pos_ncp] and a[pos_ncp].dtype == torch.int32
+    # an in-place change is seen
+    ids[1] = 1
+    c = _args(sc)
+    assert c[pos_ids].tolist() == [0, 1] and c[pos_ids] is not a[pos_ids]
+    # the conversion of a tensor that died is dropped
+    n = len(rp._I32)
+    sc['groups'][0].shape_ids = torch.tensor([1, 2])
+    _args(sc)
+    del ids
+    gc.collect()
+    assert len(rp._I32) == n
