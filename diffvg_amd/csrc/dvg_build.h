@@ -137,22 +137,6 @@ DVG_BLD bool similarity(const float *m) {
            ((m[0] == m[4] && m[1] == -m[3]) || (m[0] == -m[4] && m[1] == m[3]));
 }
 
-// Segment blocks: the leaves of a path's segment BVH in visit order, kSegBlk at a time,
-// each block summarised by the union of its leaves' boxes and the largest of their
-// radii.  A query point outside a block's box (grown by the block radius, or by the
-// current closest distance) is outside every leaf box of the block, so a walk that tests
-// the block first visits exactly the leaves the flat walk visits, in the same order.
-// The fold keeps the accumulator when a leaf value is NaN, as `mn(leaf, acc)` does.
-constexpr int kSegBlk = 16;
-DVG_BLD int seg_blocks(int nsegs) { return (nsegs + kSegBlk - 1) / kSegBlk; }
-DVG_BLD void seg_block_add(Box &acc, float &acc_r, const Box &b, float r) {
-    acc.minx = mn(b.minx, acc.minx);
-    acc.miny = mn(b.miny, acc.miny);
-    acc.maxx = mx(b.maxx, acc.maxx);
-    acc.maxy = mx(b.maxy, acc.maxy);
-    acc_r = mx(r, acc_r);
-}
-
 // group flags for the SDF bound: which member supplies the anchor point is topology
 enum { kSdfFindable = 1, kSdfApproxQuad = 2 };
 

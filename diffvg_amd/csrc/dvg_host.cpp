@@ -230,8 +230,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     const int nseg = d.num_ctrl_total;
     std::vector<int4> seg_idx(nseg);
     std::vector<int> seg_kind(nseg, 0), seg_pid(nseg, 0), seg_dfs(nseg, 0), seg_perm(nseg, 0);
-    std::vector<float4> seg_box(nseg), seg_blk(nseg, make_float4(0.f, 0.f, 0.f, 0.f));
-    std::vector<float> seg_blk_r(nseg, 0.f);
+    std::vector<float4> seg_box(nseg);
     std::vector<float> seg_maxr(nseg, 0.f), seg_cdf(nseg, 0.f), seg_pmf(nseg, 0.f);
     std::vector<float2> points(d.num_points_total);
     std::vector<float> thickness(d.num_points_total, 0.f);
@@ -338,14 +337,6 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
             for (int i = 0; i < nb; i++) {
                 seg_dfs[c0 + i] = c0 + leaves[i];
                 seg_perm[c0 + i] = vp[i];
-            }
-            for (int k = 0; k < bld::seg_blocks(nb); k++) {
-                HBox bb = box_empty();
-                float br = -INFINITY;
-                for (int i = k * bld::kSegBlk; i < std::min(nb, (k + 1) * bld::kSegBlk); i++)
-                    bld::seg_block_add(bb, br, boxes[leaves[i]], thick[leaves[i]]);
-                seg_blk[c0 + k] = make_float4(bb.minx, bb.miny, bb.maxx, bb.maxy);
-                seg_blk_r[c0 + k] = br;
             }
             // the top-level radius of a thickness path reads leaf 0 of this BVH (scene.cpp:651-656)
             thick_leaf0[s] = pnodes[0].max_radius;
@@ -505,7 +496,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
                o_params = P.add(params), o_sb = P.add(seg_begin), o_nsegs = P.add(nsegs), o_ptoff = P.add(pt_off),
                o_npt = P.add(npt), o_thick_flag = P.add(has_thick), o_closed = P.add(closed), o_approx = P.add(approx),
                o_sbox = P.add(shape_box), o_sidx = P.add(seg_idx), o_skind = P.add(seg_kind), o_spid = P.add(seg_pid),
-               o_sbx = P.add(seg_box), o_smr = P.add(seg_maxr), o_swf = P.add(std::vector<float4>(kSegPrep * (size_t)nseg)), o_sdfs = P.add(seg_dfs), o_sblk = P.add(seg_blk), o_sblr = P.add(seg_blk_r), o_scdf = P.add(seg_cdf),
+               o_sbx = P.add(seg_box), o_smr = P.add(seg_maxr), o_swf = P.add(std::vector<float4>(kSegPrep * (size_t)nseg)), o_sdfs = P.add(seg_dfs), o_scdf = P.add(seg_cdf),
                o_spmf = P.add(seg_pmf), o_pts = P.add(points), o_th = P.add(thickness), o_goff = P.add(g_off),
                o_gn = P.add(g_n), o_gids = P.add(g_ids), o_ft = P.add(fill_type), o_fo = P.add(fill_off),
                o_fn = P.add(fill_ns), o_st = P.add(stroke_type), o_so = P.add(stroke_off), o_sn = P.add(stroke_ns),
@@ -554,8 +545,6 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
         D.seg_maxr = (const float *)(B + o_smr);
         D.seg_wf = (const float4 *)(B + o_swf);
         D.seg_dfs = (const int *)(B + o_sdfs);
-        D.seg_blk = (const float4 *)(B + o_sblk);
-        D.seg_blk_r = (const float *)(B + o_sblr);
         D.seg_cdf = (const float *)(B + o_scdf);
         D.seg_pmf = (const float *)(B + o_spmf);
         D.points = (const float2 *)(B + o_pts);

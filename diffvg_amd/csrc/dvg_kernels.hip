@@ -2779,14 +2779,6 @@ DVG_DEV void refit_path_serial(const RefitArgs &A, int s) {
         stack);
     int *seg_dfs = writable(S.seg_dfs);
     for (int i = 0; i < nb; i++) seg_dfs[c0 + i] = c0 + order[A.R.seg_perm[c0 + i]];
-    for (int k = 0; k < bld::seg_blocks(nb); k++) {
-        bld::Box bb = bld::box_empty();
-        float br = -INFINITY;
-        for (int i = k * bld::kSegBlk; i < min(nb, (k + 1) * bld::kSegBlk); i++)
-            bld::seg_block_add(bb, br, to_box(seg_box[seg_dfs[c0 + i]]), seg_maxr[seg_dfs[c0 + i]]);
-        writable(S.seg_blk)[c0 + k] = to_f4(bb);
-        writable(S.seg_blk_r)[c0 + k] = br;
-    }
     A.R.thick_leaf0[s] = seg_maxr[c0 + order[0]];
     if (!finite) atomicOr(A.uerr, kUpdNonFinitePoints);
     writable(S.shape_box)[s] = to_f4(box);
@@ -2801,7 +2793,6 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
     __shared__ float s_len[kRefitWaves][kRefitSegCap];
     __shared__ int s_ord[kRefitWaves][kRefitSegCap];
     __shared__ float s_maxr[kRefitWaves][kRefitSegCap];
-    __shared__ float4 s_box[kRefitWaves][kRefitSegCap];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int s = blockIdx.x * kRefitWaves + wv;
     if (s >= S.ns) return;  // whole waves; nothing below synchronises the workgroup
@@ -2820,7 +2811,6 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
     if (nb > kRefitSegCap) return;  // k_refit_long
     KeyIdx *kv = s_kv[wv];
     float *lens = s_len[wv], *maxr = s_maxr[wv];
-    float4 *boxes = s_box[wv];
     int *ord = s_ord[wv];
     const bool thick = S.has_thick[s] != 0;
     const float r = S.stroke_width[s];
@@ -2866,7 +2856,6 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
         }
         seg_box[seg] = to_f4(b);
         seg_maxr[seg] = rr;
-        boxes[i] = to_f4(b);
         maxr[i] = rr;
         lens[i] = bld::segment_length(k, p);
         kv[i] = KeyIdx{bld::box_ycentre(b), i};
@@ -2920,17 +2909,6 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int *seg_dfs = writable(S.seg_dfs);
     for (int i = lane; i < nb; i += 64) seg_dfs[c0 + i] = c0 + ord[A.R.seg_perm[c0 + i]];
-    // segment blocks over the visit order, one lane per block (nb <= kRefitSegCap)
-    for (int k = lane; k < bld::seg_blocks(nb); k += 64) {
-        bld::Box bb = bld::box_empty();
-        float br = -INFINITY;
-        for (int i = k * bld::kSegBlk; i < min(nb, (k + 1) * bld::kSegBlk); i++) {
-            const int leaf = ord[A.R.seg_perm[c0 + i]];
-            bld::seg_block_add(bb, br, to_box(boxes[leaf]), maxr[leaf]);
-        }
-        writable(S.seg_blk)[c0 + k] = to_f4(bb);
-        writable(S.seg_blk_r)[c0 + k] = br;
-    }
     if (lane == 0) {
         // the sequential length sum, PMF and CDF of the host build
         float length = 0.f;
@@ -3187,16 +3165,6 @@ __global__ __launch_bounds__(kRefitLongThreads) void k_refit_long(RefitArgs A) {
     }
     int *seg_dfs = writable(S.seg_dfs);
     for (int i = tid; i < nb; i += kRefitLongThreads) seg_dfs[c0 + i] = c0 + ord[A.R.seg_perm[c0 + i]];
-    for (int k = tid; k < bld::seg_blocks(nb); k += kRefitLongThreads) {
-        bld::Box bb = bld::box_empty();
-        float br = -INFINITY;
-        for (int i = k * bld::kSegBlk; i < min(nb, (k + 1) * bld::kSegBlk); i++) {
-            const int leaf = c0 + ord[A.R.seg_perm[c0 + i]];
-            bld::seg_block_add(bb, br, to_box(seg_box[leaf]), seg_maxr[leaf]);
-        }
-        writable(S.seg_blk)[c0 + k] = to_f4(bb);
-        writable(S.seg_blk_r)[c0 + k] = br;
-    }
     if (tid == 0) {
         // the sequential length sum, PMF and CDF of the host build
         float length = 0.f;

@@ -10,12 +10,9 @@
 // apply the reference's reachability test to each leaf.  Ancestor boxes contain their
 // children and carry the maximum radius, so a leaf is reached exactly when its own
 // inflated box passes the test; with the leaves in visit order, order-dependent
-// results (distance ties, the use_distance_approx early return) match too.  Paths of
-// more than 2 * bld::kSegBlk segments add one level: blocks of consecutive visit
-// positions, each with the union box and largest radius of its leaves.
+// results (distance ties, the use_distance_approx early return) match too.
 #pragma once
 
-#include "dvg_build.h"
 #include "dvg_math.h"
 #include "dvg_winding_fast.h"
 #include "dvg_scene.h"
@@ -114,29 +111,16 @@ DVG_DEV int segment_winding(int k, const V2 *p, V2 pt, const float4 *prep) {
     return w;
 }
 
-// Long paths walk their segments by blocks of bld::kSegBlk visit positions: a block
-// whose box fails the leaf test skips all its leaves, and the leaves of the others are
-// tested as in the flat walk, in the same order.
-constexpr int kSegBlkWalk = 2 * bld::kSegBlk;  // paths with more segments walk by blocks
-
 DVG_DEV int winding_path(const DevScene &S, int s, V2 pt) {
     int b = S.seg_begin[s], n = S.nsegs[s];
     int wn = 0;
-    auto leaf = [&](int seg) {
-        if (n > 1 && !ray_box(S.seg_box[seg], pt)) return;
+    for (int i = 0; i < n; i++) {
+        int seg = b + i;  // the winding sum is order independent
+        if (n > 1 && !ray_box(S.seg_box[seg], pt)) continue;
         V2 p[4];
         int4 idx;
         int k = seg_load(S, seg, p, idx);
         wn += segment_winding(k, p, pt, S.seg_wf + kSegPrep * (size_t)seg);
-    };
-    if (n > kSegBlkWalk) {
-        // the winding sum is order independent: the blocks' visit order serves
-        for (int k = 0; k * bld::kSegBlk < n; k++) {
-            if (!ray_box(S.seg_blk[b + k], pt)) continue;
-            for (int i = k * bld::kSegBlk; i < min(n, (k + 1) * bld::kSegBlk); i++) leaf(S.seg_dfs[b + i]);
-        }
-    } else {
-        for (int i = 0; i < n; i++) leaf(b + i);
     }
     return wn;
 }
@@ -379,15 +363,11 @@ DVG_DEV int segment_within(const DevScene &S, int s, int seg, V2 pt, float r) {
 
 DVG_DEV bool within_distance_path(const DevScene &S, int s, V2 pt, float r) {
     int b = S.seg_begin[s], n = S.nsegs[s];
-    const int step = n > kSegBlkWalk ? bld::kSegBlk : n;
-    for (int i0 = 0, k = 0; i0 < n; i0 += step, k++) {
-        if (step < n && !in_box(S.seg_blk[b + k], pt, S.seg_blk_r[b + k])) continue;
-        for (int i = i0; i < min(n, i0 + step); i++) {
-            int seg = S.seg_dfs[b + i];
-            if (n > 1 && !in_box(S.seg_box[seg], pt, S.seg_maxr[seg])) continue;
-            int res = segment_within(S, s, seg, pt, r);
-            if (res != 0) return res > 0;
-        }
+    for (int i = 0; i < n; i++) {
+        int seg = S.seg_dfs[b + i];
+        if (n > 1 && !in_box(S.seg_box[seg], pt, S.seg_maxr[seg])) continue;
+        int res = segment_within(S, s, seg, pt, r);
+        if (res != 0) return res > 0;
     }
     return false;
 }
@@ -544,31 +524,25 @@ DVG_DEV bool closest_point_path(const DevScene &S, int s, V2 pt, float max_radiu
     float min_dist = max_radius;
     V2 ret{0, 0};
     bool found = false;
-    const int step = n > kSegBlkWalk ? bld::kSegBlk : n;
-    for (int i0 = 0, k = 0; i0 < n; i0 += step, k++) {
-        // min_dist only falls inside a block, so the block's test at its start admits
-        // every leaf the leaf test admits later in it
-        if (step < n && !in_box(S.seg_blk[b + k], pt, min_dist)) continue;
-        for (int i = i0; i < min(n, i0 + step); i++) {
-            int seg = S.seg_dfs[b + i];
-            if (n > 1 && !in_box(S.seg_box[seg], pt, min_dist)) continue;
-            V2 cp;
-            float tr;
-            float dd;
-            if (S.seg_kind[seg] == 2) {
-                if (cubic_farther_than(S, seg, ld_pt(S, S.seg_idx[seg].x), pt, min_dist)) continue;
-                dd = cubic_closest(S, seg, pt, cp, tr);
-            } else {
-                dd = segment_closest_lq(S, s, seg, pt, cp, tr);
-            }
-            if (dd < min_dist) {
-                min_dist = dd;
-                ret = cp;
-                info.base_point_id = seg - b;
-                info.point_id = S.seg_pid[seg];
-                info.t_root = tr;
-                found = true;
-            }
+    for (int i = 0; i < n; i++) {
+        int seg = S.seg_dfs[b + i];
+        if (n > 1 && !in_box(S.seg_box[seg], pt, min_dist)) continue;
+        V2 cp;
+        float tr;
+        float dd;
+        if (S.seg_kind[seg] == 2) {
+            if (cubic_farther_than(S, seg, ld_pt(S, S.seg_idx[seg].x), pt, min_dist)) continue;
+            dd = cubic_closest(S, seg, pt, cp, tr);
+        } else {
+            dd = segment_closest_lq(S, s, seg, pt, cp, tr);
+        }
+        if (dd < min_dist) {
+            min_dist = dd;
+            ret = cp;
+            info.base_point_id = seg - b;
+            info.point_id = S.seg_pid[seg];
+            info.t_root = tr;
+            found = true;
         }
     }
     result = ret;

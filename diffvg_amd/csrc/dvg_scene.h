@@ -61,8 +61,6 @@ struct DevScene {
     const float *seg_maxr;
     const float4 *seg_wf;      // [kSegPrep * nseg] fast-test prep of each cubic segment (from its points)
     const int *seg_dfs;        // per path (indexed from seg_begin): segment ids in visit order
-    const float4 *seg_blk;     // per path (indexed from seg_begin): box of each block of kSegBlk
-    const float *seg_blk_r;    //   visit positions, and the largest segment radius in it
     const float *seg_cdf;
     const float *seg_pmf;
     const float2 *points;
