@@ -839,14 +839,25 @@ def _single_fields(t, p):
     return out
 
 
+def _between_legs():
+    """Drops the pooled scenes an earlier leg left behind (scenes of another workload) and
+    collects the garbage, so that each leg's host side is timed as when it runs alone."""
+    import gc
+    from diffvg_amd import render_pytorch
+    render_pytorch.clear_scene_cache()
+    gc.collect()
+
+
 def run_headline(a):
     """The default line: the 256-scene batch (value), the serving stream and both 512x512
     single scenes."""
     cpu = None if a.no_cpu_baseline else run_cpu_baseline(4)
     b = leg_batched(a.steps, a.warmup, autotune=not a.no_autotune)
+    _between_legs()
     s = leg_serving(a.requests, a.hbm_budget, a.max_batch)
     singles = {}
     for cfg, (st, wu, pst) in ((2, (30, 10, 8)), (3, (12, 4, 0))):
+        _between_legs()
         t = leg_single(cfg, st, wu, 'tiled', prefilter_steps=pst, direct_steps=st)
         p = leg_single(cfg, 3, 1, 'plain')
         singles[t['name']] = _single_fields(t, p)
@@ -863,7 +874,9 @@ def run_headline(a):
     out.update(_batched_fields(b))
     out['serving_stream'] = _serving_fields(s)
     out['single_scene'] = singles
+    _between_legs()
     out['generative_strokes'] = leg_generative()
+    _between_legs()
     out['scale_scenes'] = leg_scale()
     out['roofline'] = _roofline(cpu, b['kernels'], BATCH_KERNELS, 'uneven_scenes_256')
     out['cpu_baseline'] = _cpu_baseline(cpu)
