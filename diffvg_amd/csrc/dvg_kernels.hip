@@ -2702,6 +2702,7 @@ constexpr int kSortStack = 96;  // >= 3 (2 floor_log2(n) + 2) for n <= 4096
 DVG_DEV int stable_rank(const KeyIdx *a, int n, int i) {
     const float ki = a[i].k;
     int pos = 0;
+#pragma unroll 8
     for (int j = 0; j < n; j++) {
         const float kj = a[j].k;
         pos += (kj < ki || (j < i && !(ki < kj))) ? 1 : 0;
@@ -2763,6 +2764,7 @@ DVG_DEV void refit_path_serial(const RefitArgs &A, int s) {
     }
     const float len = 0.f + length;
     const float inv_length = 1.f / len;
+#pragma unroll 8
     for (int i = 0; i < nb; i++) {
         float dd = seg_pmf[c0 + i] * inv_length;
         seg_pmf[c0 + i] = dd;
@@ -2874,6 +2876,7 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
         const float ki = kv[i].k;
         number = number && ki == ki;
         plain = plain && ki == ki;
+#pragma unroll 8
         for (int j = 0; j < nb; j++) {
             const float kj = kv[j].k;
             rank[t] += kj < ki ? 1 : 0;
@@ -2912,10 +2915,12 @@ __global__ __launch_bounds__(kBlock) void k_refit_shapes(RefitArgs A) {
     if (lane == 0) {
         // the sequential length sum, PMF and CDF of the host build
         float length = 0.f;
+#pragma unroll 8
         for (int i = 0; i < nb; i++) length += lens[i];
         const float len = 0.f + length;
         const float inv_length = 1.f / len;
         float prev = 0.f;
+#pragma unroll 8
         for (int i = 0; i < nb; i++) {
             float dd = lens[i] * inv_length;
             seg_pmf[c0 + i] = dd;
@@ -3101,6 +3106,7 @@ __global__ __launch_bounds__(kRefitLongThreads) void k_refit_long(RefitArgs A) {
         const float ki = kv[i].k;
         bool plain = ki == ki;
         int rank = 0;
+#pragma unroll 8
         for (int j = 0; j < nb; j++) {
             const float kj = kv[j].k;
             rank += kj < ki ? 1 : 0;
@@ -3168,10 +3174,12 @@ __global__ __launch_bounds__(kRefitLongThreads) void k_refit_long(RefitArgs A) {
     if (tid == 0) {
         // the sequential length sum, PMF and CDF of the host build
         float length = 0.f;
+#pragma unroll 8
         for (int i = 0; i < nb; i++) length += lens[i];
         const float len = 0.f + length;
         const float inv_length = 1.f / len;
         float prev = 0.f;
+#pragma unroll 8
         for (int i = 0; i < nb; i++) {
             float dd = lens[i] * inv_length;
             seg_pmf[c0 + i] = dd;
