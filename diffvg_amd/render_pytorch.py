@@ -564,21 +564,32 @@ class _SceneTemplate:
             self._zero_get = _picker(self._zero_pos)
             self._runs = runs
             # how deliver cuts the views: a long run with one view + unbind; consecutive
-            # short runs of one kind (paths of differing point counts) with one
-            # unflatten_dense_tensors call over their slots.  (kind, block offset, count of
-            # floats, first and last-plus-one index into the run order, run or None)
-            ops, i = [], 0
+            # short runs of one kind (paths of differing point counts) with one split
+            # along the first dimension when their shapes differ only there (one unbind
+            # for scalars), else with one unflatten_dense_tensors call over their slots.
+            # (kind, block offset, count of floats, first and last-plus-one index into
+            # the run order, run or None, (trailing shape, first-dimension sizes, None for
+            # scalars) or None)
+            ops, shapes, i = [], [], 0
             for r in runs:
                 kind, off, cnt, shape, m = r[0], r[1], r[2], r[3], r[4]
                 last = ops[-1] if ops else None
                 if m >= 8:
-                    ops.append([kind, off, m * cnt, i, i + m, r])
+                    ops.append([kind, off, m * cnt, i, i + m, r, None])
+                    shapes.append(None)
                 elif last and last[5] is None and last[0] == kind and last[1] + last[2] == off:
                     last[2] += m * cnt
                     last[4] = i + m
+                    shapes[-1] += [shape] * m
                 else:
-                    ops.append([kind, off, m * cnt, i, i + m, None])
+                    ops.append([kind, off, m * cnt, i, i + m, None, None])
+                    shapes.append([shape] * m)
                 i += m
+            for op, shp in zip(ops, shapes):
+                if shp and all(len(t) == 0 for t in shp):
+                    op[6] = ((), None)  # scalars: one unbind
+                elif shp and all(len(t) >= 1 and t[1:] == shp[0][1:] for t in shp):
+                    op[6] = (tuple(shp[0][1:]), [t[0] for t in shp])
             self._ops = ops
         return self._runs
 
@@ -621,7 +632,7 @@ class _SceneTemplate:
         """Writes the gradients of a direct-delivery plan into the tensors' .grad."""
         ts, rg, zeros, dup = plan
         acc_t, acc_g = [], []
-        for kind, off, n, i0, i1, run in self._ops:
+        for kind, off, n, i0, i1, run, split in self._ops:
             sel = rg[i0:i1]
             if not any(sel):
                 continue
@@ -629,6 +640,9 @@ class _SceneTemplate:
             part = ts[i0:i1]
             if run is not None:
                 views = block[o:o + n].view((run[4],) + run[3]).unbind(0)
+            elif split is not None:
+                views = (block[o:o + n].unbind(0) if split[1] is None else
+                         block[o:o + n].view((-1,) + split[0]).split(split[1]))
             else:
                 views = _unflatten(block[o:o + n], part)
             for t, g, r in zip(part, views, sel):
