@@ -40,6 +40,29 @@ def build(verbose=False):
     return LIB
 
 
+DELIVER_SRC = os.path.join(CSRC, 'dvg_deliver.cpp')
+DELIVER_LIB = os.path.join(HERE, '_dvg_deliver.so')
+
+
+def build_deliver(verbose=False):
+    """Builds the host-side gradient delivery helper (a torch C++ extension, no device
+    code) into _dvg_deliver.so next to this file."""
+    import shutil
+    import tempfile
+    from torch.utils import cpp_extension
+    tmp = tempfile.mkdtemp(prefix='dvg_deliver_')
+    try:
+        mod = cpp_extension.load(name='_dvg_deliver', sources=[DELIVER_SRC], build_directory=tmp,
+                                 extra_cflags=['-O2'], verbose=verbose)
+        shutil.copyfile(mod.__file__, DELIVER_LIB + '.tmp')
+        os.replace(DELIVER_LIB + '.tmp', DELIVER_LIB)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return DELIVER_LIB
+
+
 if __name__ == '__main__':
     build(verbose='-v' in sys.argv)
+    build_deliver(verbose='-v' in sys.argv)
     print(LIB)
+    print(DELIVER_LIB)

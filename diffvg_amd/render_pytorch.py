@@ -17,6 +17,7 @@ Differences from the reference, all deliberate:
   variable there, render_pytorch.py:638).
 """
 import itertools
+import os
 import operator
 import time
 import weakref
@@ -375,6 +376,26 @@ _device = operator.attrgetter('device')
 _dtype = operator.attrgetter('dtype')
 
 
+_NATIVE = []
+
+
+def _native():
+    """The host-side delivery helper (_dvg_deliver.so, built with the library), or None
+    when it is not built: delivery then cuts the views in Python, with the same result."""
+    if not _NATIVE:
+        import importlib.machinery
+        import importlib.util
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), '_dvg_deliver.so')
+        mod = None
+        if os.path.exists(path):
+            loader = importlib.machinery.ExtensionFileLoader('_dvg_deliver', path)
+            spec = importlib.util.spec_from_file_location('_dvg_deliver', path, loader=loader)
+            mod = importlib.util.module_from_spec(spec)
+            loader.exec_module(mod)
+        _NATIVE.append(mod)
+    return _NATIVE[0]
+
+
 _I32 = {}
 
 
@@ -558,6 +579,11 @@ class _SceneTemplate:
                     runs.append([kind, off, cnt, shape, 1, [pos]])
             run_pos = [p for r in runs for p in r[5]]
             self._run_get = _picker(run_pos)
+            # per tensor in run order: its kind and its offset within the kind's slots
+            self._kinds = sorted({r[0] for r in runs})
+            kind_idx = {k: i for i, k in enumerate(self._kinds)}
+            self._leaf_kind = torch.tensor([kind_idx[r[0]] for r in runs for _ in range(r[4])], dtype=torch.int64)
+            self._leaf_off = torch.tensor([r[1] + j * r[2] for r in runs for j in range(r[4])], dtype=torch.int64)
             # float arguments without a gradient slot (the stroke width of a path with
             # per-point thickness) receive zeros, as the engine gives them
             self._zero_pos = sorted(set(self.float_pos) - set(run_pos))
@@ -631,8 +657,14 @@ class _SceneTemplate:
     def deliver(self, block, layout, plan):
         """Writes the gradients of a direct-delivery plan into the tensors' .grad."""
         ts, rg, zeros, dup = plan
+        native = _native()
+        if native is not None:
+            native.deliver(block, ts, rg, self._leaf_kind, self._leaf_off, [layout[k] for k in self._kinds])
+            ops = ()
+        else:
+            ops = self._ops
         acc_t, acc_g = [], []
-        for kind, off, n, i0, i1, run, split in self._ops:
+        for kind, off, n, i0, i1, run, split in ops:
             sel = rg[i0:i1]
             if not any(sel):
                 continue

@@ -1,7 +1,9 @@
 """Direct delivery of scene gradients into the argument tensors' .grad
 (set_direct_leaf_gradients) against the autograd engine's delivery of the same gradient
 block, on the CPU: both routes take one gradient block laid out by kind and must give
-every argument tensor the same .grad, bit for bit (both only route values)."""
+every argument tensor the same .grad, bit for bit (both only route values).  Delivery
+runs through the C++ helper (_dvg_deliver.so) when it is built and through the Python
+cut of the views otherwise; both are checked."""
 import numpy as np
 import pytest
 import torch
@@ -74,10 +76,20 @@ def _engine_grads(args, tpl, leaves, layout, block, sizes):
     return [None if t.grad is None else t.grad.clone() for t in leaves]
 
 
+@pytest.fixture(params=['native', 'python'])
+def route(request, monkeypatch):
+    if request.param == 'native':
+        if rp._native() is None:
+            pytest.skip('_dvg_deliver.so not built (diffvg_amd.build.build_deliver)')
+    else:
+        monkeypatch.setattr(rp, '_NATIVE', [None])
+    return request.param
+
+
 @pytest.mark.parametrize('case', ['primitives', 'gradients', 'transformed', 'thick_stroke', 'mixed_segments',
                                   'strokes', 'overlap_eo'])
 @pytest.mark.parametrize('skip_every', [0, 3])
-def test_direct_delivery_equals_engine(case, skip_every):
+def test_direct_delivery_equals_engine(case, skip_every, route):
     sc, args, tpl, leaves, layout, block, sizes = _setup(case, skip_every)
     want = _engine_grads(args, tpl, leaves, layout, block, sizes)
     for t in leaves:
