@@ -241,6 +241,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     std::vector<float> thick_leaf0(ns, 0.f);  // max_radius of leaf 0 of a thickness path's BVH
     std::vector<HNode> pnodes;
     std::vector<int> long_ids;  // paths the device rebuild gives a workgroup each
+    std::vector<int> big_gids;  // groups whose members the device rebuild sorts with a workgroup
     for (int s = 0; s < ns; s++) {
         shape_type[s] = d.shape_type[s];
         stroke_width[s] = d.stroke_width[s];
@@ -360,6 +361,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
     for (int g = 0; g < ng; g++) {
         g_off[g] = d.group_shape_offset[g];
         g_n[g] = d.group_num_shapes[g];
+        if (g_n[g] > kRefitGroupLane) big_gids.push_back(g);
         if (g_n[g] <= 0 || g_off[g] < 0 || g_off[g] + g_n[g] > d.num_group_shape_ids)
             return set_error("shape group member range out of bounds");
         for (int i = 0; i < g_n[g]; i++) {
@@ -509,7 +511,7 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
         size_t o_sperm = P.add(seg_perm), o_gdesc = P.add(g_ids_desc), o_gperm = P.add(g_perm),
                o_ganchor = P.add(g_anchor), o_gflags = P.add(g_sdf_flags), o_tl0 = P.add(thick_leaf0),
                o_order = P.add(std::vector<int>(nseg, 0)), o_gorder = P.add(std::vector<int>(total, 0)),
-               o_long = P.add(long_ids);
+               o_long = P.add(long_ids), o_bigg = P.add(big_gids);
         sc.block_bytes = P.host.size();
         sc.top_offset = o_top;
         sc.top_bytes = topd.size() * sizeof(BNode);
@@ -583,6 +585,8 @@ int build_scene(const dvg_scene_desc &d, int gpu_index, SceneHost &sc) {
         R.gorder = (int *)(B + o_gorder);
         R.long_ids = (const int *)(B + o_long);
         R.n_long = (int)long_ids.size();
+        R.big_gids = (const int *)(B + o_bigg);
+        R.n_big = (int)big_gids.size();
     }
     DevScene &D = sc.dev;
     D.cw = sc.cw;

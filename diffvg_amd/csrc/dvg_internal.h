@@ -65,7 +65,13 @@ struct RefitTables {
     int *gorder = nullptr;             // [n_ids] sort scratch
     const int *long_ids = nullptr;     // [n_long] paths of more than kRefitSegCap segments
     int n_long = 0;
+    const int *big_gids = nullptr;     // [n_big] groups of more than kRefitGroupLane members
+    int n_big = 0;
 };
+
+// the device rebuild sorts the members of a group of up to kRefitGroupLane shapes on one
+// lane, of a larger group with a workgroup
+constexpr int kRefitGroupLane = 64;
 
 // the device rebuild gives a path of up to kRefitSegCap segments one wave, a longer one
 // a workgroup

@@ -2688,23 +2688,27 @@ DVG_DEV float4 to_f4(const bld::Box &b) { return make_float4(b.minx, b.miny, b.m
 constexpr int kRefitWaves = kBlock / 64;
 // a leaf's sort key beside its index: the introsort then reads keys without an
 // indirection, and yields the permutation it yields for indices with a key comparator
-struct KeyIdx {
-    float k;
+template <class K>
+struct KeyOf {
+    K k;
     int i;
 };
+using KeyIdx = KeyOf<float>;
 struct KeyLess {
-    DVG_DEV bool operator()(const KeyIdx &a, const KeyIdx &b) const { return a.k < b.k; }
+    template <class T>
+    DVG_DEV bool operator()(const T &a, const T &b) const { return a.k < b.k; }
 };
 constexpr int kSortStack = 96;  // >= 3 (2 floor_log2(n) + 2) for n <= 4096
 
 // where the introsort's final insertion pass puts element i of a[0..n): the stable
 // rank, for keys without NaN
-DVG_DEV int stable_rank(const KeyIdx *a, int n, int i) {
-    const float ki = a[i].k;
+template <class K>
+DVG_DEV int stable_rank(const KeyOf<K> *a, int n, int i) {
+    const K ki = a[i].k;
     int pos = 0;
 #pragma unroll 8
     for (int j = 0; j < n; j++) {
-        const float kj = a[j].k;
+        const K kj = a[j].k;
         pos += (kj < ki || (j < i && !(ki < kj))) ? 1 : 0;
     }
     return pos;
@@ -2975,11 +2979,12 @@ DVG_DEV int block_excl_scan(int v, int &total, int *s_scan) {
 // stoppers by a suffix count, and the first M pairs with L_k < R_k swap.  Returns the
 // cut in every thread; a[] is final when it returns.  Lpos holds e - b + 1 ints, Rpos
 // e - b + 2.
-DVG_DEV int block_partition(KeyIdx *a, int b, int e, int p, int *Lpos, int *Rpos, int *s_scan) {
+template <class K>
+DVG_DEV int block_partition(KeyOf<K> *a, int b, int e, int p, int *Lpos, int *Rpos, int *s_scan) {
     const int tid = threadIdx.x;
     const int per = (e - b + kRefitLongThreads - 1) / kRefitLongThreads;
     const int i0 = min(e, b + tid * per), i1 = min(e, i0 + per);
-    const float P = a[p].k;
+    const K P = a[p].k;
     int nl = 0, nr = 0;
     for (int i = i0; i < i1; i++) {
         nl += !(a[i].k < P) ? 1 : 0;
@@ -3001,7 +3006,7 @@ DVG_DEV int block_partition(KeyIdx *a, int b, int e, int p, int *Lpos, int *Rpos
     (void)block_excl_scan(cnt, M, s_scan);
     // the swapped pairs are disjoint: every swapped L lies below every swapped R
     for (int k = tid + 1; k <= M; k += kRefitLongThreads) {
-        const KeyIdx x = a[Lpos[k]], y = a[Rpos[k]];
+        const KeyOf<K> x = a[Lpos[k]], y = a[Rpos[k]];
         a[Lpos[k]] = y;
         a[Rpos[k]] = x;
     }
@@ -3011,104 +3016,32 @@ DVG_DEV int block_partition(KeyIdx *a, int b, int e, int p, int *Lpos, int *Rpos
     return cut;
 }
 
-template <bool MIN>
-DVG_DEV void first_extreme_combine(float &v, int &i, float ov, int oi) {
-    bool better = MIN ? ov < v : ov > v;
-    bool worse = MIN ? v < ov : v > ov;
-    if (better || (!worse && oi < i)) {
-        v = ov;
-        i = oi;
-    }
-}
-
-__global__ __launch_bounds__(kRefitLongThreads) void k_refit_long(RefitArgs A) {
-    const DevScene &S = A.S;
-    const int s = A.R.long_ids[blockIdx.x];
-    const int c0 = S.seg_begin[s], nb = S.nsegs[s], p0 = S.pt_off[s], np = S.npt[s];
-    const int tid = threadIdx.x, wv = tid >> 6;
-    if (nb > kRefitLongCap) {  // uniform over the workgroup: no barrier is reached
-        if (tid == 0) refit_path_serial(A, s);
-        return;
-    }
-    constexpr int kW = kRefitLongThreads / 64;
-    __shared__ KeyIdx kv[kRefitLongCap];
-    __shared__ float lens[kRefitLongCap];
-    __shared__ int ord[kRefitLongCap];  // also the left stoppers of the tie sort
-    __shared__ int rpos[kRefitLongCap + 2];
+// Sorts kv[0..nb) (nb <= kRefitLongCap, in LDS) as dvg_sort.h's restated introsort does
+// and writes the sorted element indices to ord (nb ints; rpos holds nb + 2).  By rank
+// when the keys are distinct numbers; with ties, the introsort's own schedule: ranges
+// longer than kRefitBigRange are partitioned by the workgroup one level at a time
+// (block_partition), the others by the sequential partition phase on one thread each,
+// and the final insertion pass -- a stable sort -- as the stable rank; with a NaN key
+// (a float key of non-finite points, which the update reports) the sequential sort on
+// one thread.  Every thread of the kRefitLongThreads workgroup calls it; the keys must
+// be written before the call (it begins with a barrier).
+template <class K>
+DVG_DEV void block_sort(KeyOf<K> *kv, int nb, int *ord, int *rpos) {
     __shared__ int s_stack[kSortStack];
     __shared__ int s_scan[kRefitLongThreads / 64];
     __shared__ int4 s_big[2][kRefitMaxBig], s_small[kRefitMaxSmall];
-    __shared__ int s_nbig[2], s_nsmall;
-    __shared__ float s_ext[4][kW];
-    __shared__ int s_exti[4][kW];
-    __shared__ int s_nonfinite, s_tie;
-    if (tid == 0) {
-        s_nonfinite = 0;
-        s_tie = 0;
-    }
-    const bool thick = S.has_thick[s] != 0;
-    const float r = S.stroke_width[s];
-    // shape box and finiteness over the points
-    float x0 = __builtin_inff(), y0 = x0, x1 = -x0, y1 = -x0;
-    int ix0 = INT_MAX, iy0 = INT_MAX, ix1 = INT_MAX, iy1 = INT_MAX;
-    bool finite = true;
-    for (int i = tid; i < np; i += kRefitLongThreads) {
-        float2 q = S.points[p0 + i];
-        finite = finite && isfinite(q.x) && isfinite(q.y) && (!thick || isfinite(S.thickness[p0 + i]));
-        if (ix0 == INT_MAX) {
-            x0 = x1 = q.x;
-            y0 = y1 = q.y;
-            ix0 = iy0 = ix1 = iy1 = i;
-        } else {
-            if (q.x < x0) { x0 = q.x; ix0 = i; }
-            if (q.y < y0) { y0 = q.y; iy0 = i; }
-            if (q.x > x1) { x1 = q.x; ix1 = i; }
-            if (q.y > y1) { y1 = q.y; iy1 = i; }
-        }
-    }
-    wave_first_extreme<true>(x0, ix0);
-    wave_first_extreme<true>(y0, iy0);
-    wave_first_extreme<false>(x1, ix1);
-    wave_first_extreme<false>(y1, iy1);
-    __syncthreads();  // s_nonfinite and s_tie are set
-    if ((tid & 63) == 0) {
-        s_ext[0][wv] = x0, s_exti[0][wv] = ix0;
-        s_ext[1][wv] = y0, s_exti[1][wv] = iy0;
-        s_ext[2][wv] = x1, s_exti[2][wv] = ix1;
-        s_ext[3][wv] = y1, s_exti[3][wv] = iy1;
-    }
-    if (!finite) s_nonfinite = 1;
-    // per segment: box, radius, length; y-centre keys for the leaf sort
-    float4 *seg_box = writable(S.seg_box);
-    float *seg_maxr = writable(S.seg_maxr), *seg_pmf = writable(S.seg_pmf), *seg_cdf = writable(S.seg_cdf);
-    for (int i = tid; i < nb; i += kRefitLongThreads) {
-        const int seg = c0 + i, k = S.seg_kind[seg];
-        const int4 ix = S.seg_idx[seg];
-        const int id[4] = {ix.x, ix.y, ix.z, ix.w};
-        bld::P2 p[4];
-        bld::Box b = bld::box_empty();
-        float rr = -1;
-        for (int j = 0; j <= k + 1; j++) {
-            float2 q = S.points[id[j]];
-            p[j] = bld::P2{q.x, q.y};
-            b = bld::merge(b, p[j]);
-            float rj = thick ? S.thickness[id[j]] : r;
-            rr = j == 0 ? rj : bld::mx(rr, rj);
-        }
-        seg_box[seg] = to_f4(b);
-        seg_maxr[seg] = rr;
-        lens[i] = bld::segment_length(k, p);
-        kv[i] = KeyIdx{bld::box_ycentre(b), i};
-    }
+    __shared__ int s_nbig[2], s_nsmall, s_tie;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_tie = 0;
     __syncthreads();
     // leaf order by rank; a tie or a NaN key hands the order to the introsort
     for (int i = tid; i < nb; i += kRefitLongThreads) {
-        const float ki = kv[i].k;
+        const K ki = kv[i].k;
         bool plain = ki == ki;
         int rank = 0;
 #pragma unroll 8
         for (int j = 0; j < nb; j++) {
-            const float kj = kv[j].k;
+            const K kj = kv[j].k;
             rank += kj < ki ? 1 : 0;
             plain = plain && (j == i || kj < ki || ki < kj);
         }
@@ -3169,6 +3102,92 @@ __global__ __launch_bounds__(kRefitLongThreads) void k_refit_long(RefitArgs A) {
         for (int i = tid; i < nb; i += kRefitLongThreads) ord[stable_rank(kv, nb, i)] = kv[i].i;
         __syncthreads();
     }
+}
+
+template <bool MIN>
+DVG_DEV void first_extreme_combine(float &v, int &i, float ov, int oi) {
+    bool better = MIN ? ov < v : ov > v;
+    bool worse = MIN ? v < ov : v > ov;
+    if (better || (!worse && oi < i)) {
+        v = ov;
+        i = oi;
+    }
+}
+
+__global__ __launch_bounds__(kRefitLongThreads) void k_refit_long(RefitArgs A) {
+    const DevScene &S = A.S;
+    const int s = A.R.long_ids[blockIdx.x];
+    const int c0 = S.seg_begin[s], nb = S.nsegs[s], p0 = S.pt_off[s], np = S.npt[s];
+    const int tid = threadIdx.x, wv = tid >> 6;
+    if (nb > kRefitLongCap) {  // uniform over the workgroup: no barrier is reached
+        if (tid == 0) refit_path_serial(A, s);
+        return;
+    }
+    constexpr int kW = kRefitLongThreads / 64;
+    __shared__ KeyIdx kv[kRefitLongCap];
+    __shared__ float lens[kRefitLongCap];
+    __shared__ int ord[kRefitLongCap];  // also the left stoppers of the tie sort
+    __shared__ int rpos[kRefitLongCap + 2];
+    __shared__ float s_ext[4][kW];
+    __shared__ int s_exti[4][kW];
+    __shared__ int s_nonfinite;
+    if (tid == 0) s_nonfinite = 0;
+    const bool thick = S.has_thick[s] != 0;
+    const float r = S.stroke_width[s];
+    // shape box and finiteness over the points
+    float x0 = __builtin_inff(), y0 = x0, x1 = -x0, y1 = -x0;
+    int ix0 = INT_MAX, iy0 = INT_MAX, ix1 = INT_MAX, iy1 = INT_MAX;
+    bool finite = true;
+    for (int i = tid; i < np; i += kRefitLongThreads) {
+        float2 q = S.points[p0 + i];
+        finite = finite && isfinite(q.x) && isfinite(q.y) && (!thick || isfinite(S.thickness[p0 + i]));
+        if (ix0 == INT_MAX) {
+            x0 = x1 = q.x;
+            y0 = y1 = q.y;
+            ix0 = iy0 = ix1 = iy1 = i;
+        } else {
+            if (q.x < x0) { x0 = q.x; ix0 = i; }
+            if (q.y < y0) { y0 = q.y; iy0 = i; }
+            if (q.x > x1) { x1 = q.x; ix1 = i; }
+            if (q.y > y1) { y1 = q.y; iy1 = i; }
+        }
+    }
+    wave_first_extreme<true>(x0, ix0);
+    wave_first_extreme<true>(y0, iy0);
+    wave_first_extreme<false>(x1, ix1);
+    wave_first_extreme<false>(y1, iy1);
+    __syncthreads();  // s_nonfinite is set
+    if ((tid & 63) == 0) {
+        s_ext[0][wv] = x0, s_exti[0][wv] = ix0;
+        s_ext[1][wv] = y0, s_exti[1][wv] = iy0;
+        s_ext[2][wv] = x1, s_exti[2][wv] = ix1;
+        s_ext[3][wv] = y1, s_exti[3][wv] = iy1;
+    }
+    if (!finite) s_nonfinite = 1;
+    // per segment: box, radius, length; y-centre keys for the leaf sort
+    float4 *seg_box = writable(S.seg_box);
+    float *seg_maxr = writable(S.seg_maxr), *seg_pmf = writable(S.seg_pmf), *seg_cdf = writable(S.seg_cdf);
+    for (int i = tid; i < nb; i += kRefitLongThreads) {
+        const int seg = c0 + i, k = S.seg_kind[seg];
+        const int4 ix = S.seg_idx[seg];
+        const int id[4] = {ix.x, ix.y, ix.z, ix.w};
+        bld::P2 p[4];
+        bld::Box b = bld::box_empty();
+        float rr = -1;
+        for (int j = 0; j <= k + 1; j++) {
+            float2 q = S.points[id[j]];
+            p[j] = bld::P2{q.x, q.y};
+            b = bld::merge(b, p[j]);
+            float rj = thick ? S.thickness[id[j]] : r;
+            rr = j == 0 ? rj : bld::mx(rr, rj);
+        }
+        seg_box[seg] = to_f4(b);
+        seg_maxr[seg] = rr;
+        lens[i] = bld::segment_length(k, p);
+        kv[i] = KeyIdx{bld::box_ycentre(b), i};
+    }
+    __syncthreads();
+    block_sort(kv, nb, ord, rpos);
     int *seg_dfs = writable(S.seg_dfs);
     for (int i = tid; i < nb; i += kRefitLongThreads) seg_dfs[c0 + i] = c0 + ord[A.R.seg_perm[c0 + i]];
     if (tid == 0) {
@@ -3206,14 +3225,10 @@ __global__ __launch_bounds__(kRefitLongThreads) void k_refit_long(RefitArgs A) {
     }
 }
 
-// Per group (scene.cpp:621-684, shape.h:123): canvas_to_shape, member visit order of the
-// Morton-sorted group BVH, local root box, canvas-space leaf box and radius, SDF anchor.
-__global__ __launch_bounds__(kBlock) void k_refit_groups(RefitArgs A) {
+// The member visit order of group g's BVH (scene.cpp:632-645): the members sorted by
+// the Morton code of their box centre on one thread, from global memory.
+DVG_DEV void group_order_serial(const RefitArgs &A, int g) {
     const DevScene &S = A.S;
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= S.ng) return;
-    const float *m = S.s2c + 9 * (size_t)g;
-    bld::inverse3(m, writable(S.c2s) + 9 * (size_t)g);
     const int off = S.g_off[g], n = S.g_n[g];
     const int *desc = A.R.g_ids_desc + off;
     int *ord = A.R.gorder + off;
@@ -3228,6 +3243,41 @@ __global__ __launch_bounds__(kBlock) void k_refit_groups(RefitArgs A) {
         stack);
     int *g_ids = writable(S.g_ids);
     for (int i = 0; i < n; i++) g_ids[off + i] = desc[ord[A.R.g_perm[off + i]]];
+}
+
+// The member order of a group of more than kRefitGroupLane shapes, one workgroup per
+// group: the Morton codes in LDS, sorted by block_sort (groups of more than
+// kRefitLongCap members on one thread).
+__global__ __launch_bounds__(kRefitLongThreads) void k_refit_big_groups(RefitArgs A) {
+    const DevScene &S = A.S;
+    const int g = A.R.big_gids[blockIdx.x];
+    const int off = S.g_off[g], n = S.g_n[g];
+    if (n > kRefitLongCap) {  // uniform over the workgroup: no barrier is reached
+        if (threadIdx.x == 0) group_order_serial(A, g);
+        return;
+    }
+    __shared__ KeyOf<uint32_t> kv[kRefitLongCap];
+    __shared__ int ord[kRefitLongCap];
+    __shared__ int rpos[kRefitLongCap + 2];
+    const int *desc = A.R.g_ids_desc + off;
+    for (int i = threadIdx.x; i < n; i += kRefitLongThreads)
+        kv[i] = KeyOf<uint32_t>{bld::morton_of_box(to_box(S.shape_box[desc[i]]), S.cw, S.ch), i};
+    block_sort(kv, n, ord, rpos);
+    int *g_ids = writable(S.g_ids);
+    for (int i = threadIdx.x; i < n; i += kRefitLongThreads) g_ids[off + i] = desc[ord[A.R.g_perm[off + i]]];
+}
+
+// Per group (scene.cpp:621-684, shape.h:123): canvas_to_shape, member visit order of the
+// Morton-sorted group BVH, local root box, canvas-space leaf box and radius, SDF anchor.
+__global__ __launch_bounds__(kBlock) void k_refit_groups(RefitArgs A) {
+    const DevScene &S = A.S;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= S.ng) return;
+    const float *m = S.s2c + 9 * (size_t)g;
+    bld::inverse3(m, writable(S.c2s) + 9 * (size_t)g);
+    const int off = S.g_off[g], n = S.g_n[g];
+    const int *desc = A.R.g_ids_desc + off;
+    if (n <= kRefitGroupLane) group_order_serial(A, g);  // larger groups: k_refit_big_groups
     bld::Box lb = bld::box_empty();
     float maxr = 0.f;
     for (int i = 0; i < n; i++) {
@@ -3340,6 +3390,7 @@ int update_scene(SceneHost &sc, const dvg_scene_params &p, float filter_radius, 
         k_refit_shapes<<<(D.ns + kRefitWaves - 1) / kRefitWaves, kBlock, 0, st>>>(A);
         if (A.R.n_long > 0) k_refit_long<<<A.R.n_long, kRefitLongThreads, 0, st>>>(A);
         if (D.ng > 0) k_refit_groups<<<(D.ng + kBlock - 1) / kBlock, kBlock, 0, st>>>(A);
+        if (A.R.n_big > 0) k_refit_big_groups<<<A.R.n_big, kRefitLongThreads, 0, st>>>(A);
         k_refit_cdf<<<1, kBlock, 0, st>>>(A);
         if (D.nseg > 0) k_seg_prep<<<(D.nseg + kBlock - 1) / kBlock, kBlock, 0, st>>>(D);
     }

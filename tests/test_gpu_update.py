@@ -93,7 +93,23 @@ def long_paths(size=96):
     return dict(canvas=(size, size), shapes=shapes, groups=groups, filter=('box', 0.5))
 
 
-EXTRA = {'ties': ties, 'long_paths': long_paths}
+def big_groups(size=64):
+    """Groups past the rebuild's per-lane member limit: 700 small circles in one group, a
+    quarter of them stacked four to a Morton cell (ties; one workgroup), and 4,500 in
+    another (past the workgroup's limit: one thread)."""
+    rng = np.random.default_rng(11)
+    shapes = []
+    for k in range(700):
+        c = [4 + (k % 25) * 2.2, 4 + (k // 25) * 2.0] if k % 4 else [20.0 + 0.0001 * k, 30.0]
+        shapes.append(pd.Circle(radius=scenes.T(0.4 + 0.001 * k), center=scenes.T(c)))
+    for k in range(4500):
+        shapes.append(pd.Circle(radius=scenes.T(0.2), center=scenes.T(rng.uniform(2, size - 2, 2))))
+    groups = [pd.ShapeGroup(shape_ids=torch.arange(0, 700), fill_color=scenes.T([0.2, 0.4, 0.9, 0.7])),
+              pd.ShapeGroup(shape_ids=torch.arange(700, 5200), fill_color=scenes.T([0.9, 0.3, 0.1, 0.5]))]
+    return dict(canvas=(size, size), shapes=shapes, groups=groups, filter=('box', 0.5))
+
+
+EXTRA = {'ties': ties, 'long_paths': long_paths, 'big_groups': big_groups}
 CASES = sorted(scenes.CASES) + sorted(EXTRA)
 
 
@@ -103,17 +119,20 @@ def _flat(case):
 
 def _perturb(f, seed, rigid=False):
     """The same topology with other float values (`rigid`: points moved by one common
-    offset, which keeps ties between sort keys)."""
+    offset and shape parameters kept, which keeps ties between sort keys)."""
     rng = np.random.default_rng(seed)
     g = dict(f)
     if rigid:
-        g['points'] = (f['points'].reshape(-1, 2) + np.array([0.5, 0.25], np.float32)).reshape(f['points'].shape)
+        pts = f['points']  # a scene without paths holds one placeholder value
+        g['points'] = pts if pts.size % 2 else (pts.reshape(-1, 2) + np.array([0.5, 0.25], np.float32)).reshape(
+            pts.shape)
     else:
         g['points'] = (f['points'] + rng.normal(0, 0.7, f['points'].shape)).astype(np.float32)
     g['stroke_width'] = (f['stroke_width'] * rng.uniform(0.7, 1.3, f['stroke_width'].shape)).astype(np.float32)
     if f['thickness'] is not None and f['thickness'].size:
         g['thickness'] = (f['thickness'] * rng.uniform(0.7, 1.3, f['thickness'].shape)).astype(np.float32)
-    g['shape_params'] = (f['shape_params'] * rng.uniform(0.9, 1.1, f['shape_params'].shape)).astype(np.float32)
+    scale = rng.uniform(0.9, 1.1, f['shape_params'].shape)
+    g['shape_params'] = f['shape_params'].copy() if rigid else (f['shape_params'] * scale).astype(np.float32)
     g['color_params'] = np.clip(f['color_params'] + rng.normal(0, 0.05, f['color_params'].shape), 0, 1).astype(
         np.float32)
     x = f['shape_to_canvas'].reshape(-1, 3, 3).copy()
@@ -142,7 +161,7 @@ def _render(scene, w, h, d_img=None):
 def test_device_rebuild_equals_host_build(case):
     print('rebuild', case, flush=True)
     f1 = _flat(case)
-    f2 = _perturb(f1, 1, rigid=case in ('ties', 'long_paths'))
+    f2 = _perturb(f1, 1, rigid=case in ('ties', 'long_paths', 'big_groups'))
     a = _lib.DeviceScene(f1, 0)
     tensors = {k: _dev(f2[k]) for k in FLOAT_KEYS}
     a.update(f2['filter_radius'], None, **tensors)
